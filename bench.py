@@ -1,0 +1,207 @@
+"""Throughput of the per-rank GEMM hot path on one MI355X.
+
+Default workload: FP64 SUMMA on a virtual 4x4 process grid, M=N=K=16384,
+block-cyclic 512x512 blocks, A, B, C uniform[-1,1) from fixed seeds,
+alpha=1, beta=0.5.  One step = one dg_summa call over the packed
+block-cyclic operands already resident in HBM (every virtual rank's
+local GEMM, all 32 panels, one launch).
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--workload summa|cannon|dgemm]
+
+Prints one JSON line.  `roofline` is the dominant kernel against the FP64
+matrix peak (78.6 TFLOP/s), timed with HIP events on the launch stream;
+`cpu_baseline` is the host BLAS dgemm of the oracle on a bounded sample of
+the same work, on this host's cores.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "distributed-gemm-thesis_amd"))
+
+FP64_PEAK = 78.6  # TFLOP/s, MI355X dense FP64 matrix
+HBM_PEAK = 8000.0  # GB/s
+
+WORKLOADS = {
+    "summa": "FP64 SUMMA 4x4 virtual grid, M=N=K=16384, block-cyclic nb=512, alpha=1, beta=0.5",
+    "cannon": "FP64 2.5D Cannon 4x4x2 virtual grid, M=16384 N=8192 K=32768, opA=N opB=N, layer reduce in HBM",
+    "dgemm": "FP64 DGEMM NN M=N=K=16384, alpha=1, beta=0",
+}
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="summa")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="budget of the CPU baseline sample")
+    return ap.parse_args()
+
+
+def load_traffic(kernel_tag):
+    """HBM bytes per launch measured by rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE),
+    committed as profiles/traffic_<tag>.json; None when not measured."""
+    path = os.path.join(ROOT, "profiles", "traffic_%s.json" % kernel_tag)
+    if not os.path.exists(path):
+        return None
+    try:
+        with open(path) as f:
+            return json.load(f).get("bytes_per_launch")
+    except (OSError, ValueError):
+        return None
+
+
+def setup(workload, dev):
+    import torch
+    import dgamd
+
+    f64 = dict(dtype=torch.float64, device=dev)
+    if workload == "summa":
+        n, nb, P = 16384, 512, 4
+        M = N = K = n
+        G = torch.empty(n * n, **f64)
+        packed = []
+        for seed in (1, 2, 3):
+            dgamd.generate_t(seed, dgamd.DG_DIST_UNIFORM_PM1, G)
+            Pk = torch.empty(n * n, **f64)
+            dgamd.pack_block_cyclic_t(G, n, n, n, nb, nb, P, P, Pk)
+            packed.append(Pk)
+        del G
+        Ap, Bp, Cp = packed
+
+        def step():
+            dgamd.summa_t("N", "N", M, N, K, 1.0, Ap, Bp, 0.5, Cp, nb, nb, nb, P, P)
+
+        cfg = {"workload": WORKLOADS[workload], "M": M, "N": N, "K": K, "grid": "4x4", "block": nb,
+               "alpha": 1.0, "beta": 0.5, "layout": "block-cyclic, column-major local matrices"}
+        return step, (M, N, K), cfg, "summa16384"
+    if workload == "cannon":
+        M, N, K, p, c = 16384, 8192, 32768, 4, 2
+        bufs = []
+        for seed, (r, cc) in zip((1, 2, 3), ((M, K), (K, N), (M, N))):
+            G = torch.empty(r * cc, **f64)
+            dgamd.generate_t(seed, dgamd.DG_DIST_UNIFORM_PM1, G)
+            Pk = torch.empty_like(G)
+            dgamd.pack_block2d_t(G, r, r, cc, dgamd.block_splits(r, p), dgamd.block_splits(cc, p), Pk)
+            del G
+            bufs.append(Pk)
+        Ap, Bp, Cp = bufs
+        ws = torch.empty(dgamd.cannon25d_workspace(M, N, p, c), **f64)
+
+        def step():
+            dgamd.cannon25d_t("N", "N", M, N, K, 1.0, Ap, Bp, 0.0, Cp, p, c, dgamd.DG_SCHED_AUTO, ws)
+
+        cfg = {"workload": WORKLOADS[workload], "M": M, "N": N, "K": K, "grid": "4x4x2", "alpha": 1.0, "beta": 0.0}
+        return step, (M, N, K), cfg, "cannon16384x8192x32768"
+    n = 16384
+    A = torch.empty(n * n, **f64)
+    B = torch.empty(n * n, **f64)
+    C = torch.empty(n * n, **f64)
+    dgamd.generate_t(1, 0, A)
+    dgamd.generate_t(2, 0, B)
+
+    def step():
+        dgamd.dgemm_t("N", "N", n, n, n, 1.0, A, n, B, n, 0.0, C, n)
+
+    return step, (n, n, n), {"workload": WORKLOADS["dgemm"], "M": n, "N": n, "K": n}, "dgemm16384"
+
+
+def cpu_baseline(budget_s):
+    """Host BLAS dgemm (the oracle's numpy/OpenBLAS) on 4096^3 products,
+    repeated for about `budget_s` seconds; TFLOP/s = 2*4096^3*reps / time."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+    import oracle  # noqa: F401  (the checker's host dgemm is numpy's BLAS)
+
+    threads = None
+    try:
+        from threadpoolctl import threadpool_info
+        for info in threadpool_info():
+            if info.get("user_api") == "blas":
+                threads = info.get("num_threads")
+                break
+    except Exception:
+        pass
+    n = 4096
+    rng = np.random.default_rng(1)
+    a = rng.uniform(-1, 1, (n, n))
+    b = rng.uniform(-1, 1, (n, n))
+    c = np.empty((n, n))
+    np.matmul(a, b, out=c)  # warm-up
+    reps, t0 = 0, time.perf_counter()
+    while True:
+        np.matmul(a, b, out=c)
+        reps += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s:
+            break
+    tf = 2.0 * n ** 3 * reps / el / 1e12
+    return {"value": round(tf, 4), "unit": "TFLOP/s", "cores": threads or os.cpu_count(), "kind": "port",
+            "sample": "%d host-BLAS (numpy OpenBLAS) dgemm calls of 4096x4096x4096 in %.1f s; "
+                      "the 16384^3 step is 64 such products" % (reps, el)}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU transport "
+              "(--gpus %d requested)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+    import torch
+    import dgamd
+
+    dev = torch.device("cuda", 0)
+    dgamd.lib()
+    if dgamd.lib().dg_device_ok() != 1:
+        raise SystemExit("no gfx950 device: " + dgamd.last_error())
+    step, (M, N, K), cfg, tag = setup(args.workload, dev)
+    flops = 2.0 * M * N * K
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+
+    stream = torch.cuda.current_stream()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s, e in ev:
+        s.record(stream)
+        step()
+        e.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    per = [s.elapsed_time(e) / 1e3 for s, e in ev]  # seconds per step (events on the launch stream)
+    avg = sum(per) / len(per)
+    value = flops * args.steps / wall / 1e12
+    achieved = flops / avg / 1e12
+
+    out = {
+        "metric": "FP64-equiv TFLOP/s @16384^3 (native vs Ozaki-INT8), % MFMA peak, rel err vs FP64",
+        "value": round(value, 3),
+        "unit": "TFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: SplitMix64-seeded uniform[-1,1) (seeds 1,2,3), generated and packed on the GPU",
+        "config": cfg,
+        "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP64_PEAK, "unit": "TFLOP/s",
+                     "frac": round(achieved / FP64_PEAK, 4), "traffic": load_traffic(tag)},
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,87 @@
+"""Builds libdgamd.so (HIP kernels + C ABI, gfx950 only) in this directory.
+
+    python build.py            # compile every csrc/*.hip and link
+    python build.py --clean    # remove the build products first
+
+Every HIP source is compiled by hipcc for --offload-arch=gfx950 with the
+packed-FP32 target feature disabled; objects go to _build/ and the shared
+library to ./libdgamd.so.
+"""
+import concurrent.futures
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+HOST = os.path.join(HERE, "host")
+OBJ = os.path.join(HERE, "_build")
+LIB = os.path.join(HERE, "libdgamd.so")
+HOSTLIB = os.path.join(HERE, "libdghost.so")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+HIP_FLAGS = [
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unused-function",
+    "-I" + os.path.join(ROOT, "include"), "-I" + CSRC,
+]
+
+
+def _run(cmd):
+    proc = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    # hipcc's host pass does not know the packed-fp32 feature; that warning is harmless.
+    out = "\n".join(l for l in proc.stdout.splitlines()
+                    if "packed-fp32-ops" not in l and "is not a recognized feature" not in l)
+    if proc.returncode != 0:
+        raise RuntimeError("build failed: %s\n%s" % (" ".join(cmd), out))
+    return out
+
+
+def _compile(src):
+    obj = os.path.join(OBJ, os.path.basename(src) + ".o")
+    _run([HIPCC] + HIP_FLAGS + ["-c", src, "-o", obj])
+    return obj
+
+
+def sources():
+    return sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".hip"))
+
+
+def host_sources():
+    if not os.path.isdir(HOST):
+        return []
+    return sorted(os.path.join(HOST, f) for f in os.listdir(HOST) if f.endswith(".cpp"))
+
+
+def build(verbose=False, jobs=None):
+    """Compiles every HIP source and links libdgamd.so (and the C++ host API
+    library libdghost.so on top of it).  Always recompiles."""
+    os.makedirs(OBJ, exist_ok=True)
+    srcs = sources()
+    jobs = jobs or min(8, os.cpu_count() or 4, len(srcs))
+    with concurrent.futures.ThreadPoolExecutor(max_workers=jobs) as ex:
+        objs = list(ex.map(_compile, srcs))
+    tmp = LIB + ".tmp"
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs)
+    os.replace(tmp, LIB)
+    hsrcs = host_sources()
+    if hsrcs:
+        tmp = HOSTLIB + ".tmp"
+        _run([HIPCC, "-O2", "-std=c++17", "-fPIC", "-shared", "-I" + os.path.join(ROOT, "include"),
+              "-o", tmp] + hsrcs + ["-L" + HERE, "-ldgamd", "-Wl,-rpath,$ORIGIN"])
+        os.replace(tmp, HOSTLIB)
+    if verbose:
+        print("built", LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    if "--clean" in sys.argv:
+        shutil.rmtree(OBJ, ignore_errors=True)
+        for f in (LIB, HOSTLIB):
+            if os.path.exists(f):
+                os.remove(f)
+    build(verbose=True)

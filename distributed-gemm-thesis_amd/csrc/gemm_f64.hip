@@ -1,0 +1,387 @@
+// FP64 GEMM on the CDNA4 matrix cores (v_mfma_f64_16x16x4_f64), grouped over
+// problems and segmented along K.
+//
+// One launch computes every 128x128 output tile of every problem in the
+// problem table.  A problem is C = alpha * sum_s opA(seg s) * opB(seg s)
+// + beta*C, where segment s supplies its own A/B base pointers and leading
+// dimensions.  A plain DGEMM is one problem with one segment; a virtual-grid
+// Cannon rank is one problem whose segments are the tiles the rotations
+// would have brought in; a SUMMA rank is one problem whose segments are the
+// owners' panels.
+//
+// Tiling (256 threads = 4 waves of 64 lanes):
+//   block tile 128 x 128, k-step 16, LDS double buffered (2 blocks per CU);
+//   each wave owns a 64 x 64 sub-tile = 4 x 4 MFMA tiles of 16 x 16.
+// LDS holds both operands k-contiguous, [row][k] with a padded stride of 18
+// doubles (144 B), so a lane reads its 4 consecutive k values with two
+// 128-bit LDS reads and 16 lanes of one read touch 64 distinct banks.
+// Lane l of an MFMA supplies k = 4*(l/16) + kk for sub-step kk; A and B use
+// the same k assignment, so the product is exact whatever k order the
+// matrix core sums in.
+// The MFMA is issued as D = B^T A^T, so each output register holds 16
+// consecutive rows of one C column: the epilogue stores are 128-byte runs.
+#include "dg_internal.hpp"
+
+namespace dg {
+
+namespace {
+
+typedef double dv2 __attribute__((ext_vector_type(2)));
+typedef double dv4 __attribute__((ext_vector_type(4)));
+
+// Global-address-space views: loads through them compile to global_load_*
+// (counted by vmcnt only), not flat_load_*, whose lgkmcnt share would make
+// every LDS wait also wait for the prefetch of the next k-step.
+typedef __attribute__((address_space(1))) const double gdouble;
+typedef __attribute__((address_space(1))) const dv2 gdv2;
+__device__ __forceinline__ double gld(const double* p) { return *(gdouble*)p; }
+__device__ __forceinline__ dv2 gld2(const double* p) { return *(gdv2*)p; }
+
+constexpr int BM = 128;
+constexpr int BN = 128;
+constexpr int BK = 16;
+constexpr int LDK = BK + 2;  // padded LDS row stride (doubles)
+constexpr int NT = 256;
+constexpr int GROUP_M = 8;   // tile rows per raster group
+
+// Block id -> tile id so that each XCD (blocks are dealt round-robin to the
+// 8 XCDs) receives one contiguous range of tiles, which the grouped raster
+// below turns into a compact patch of C whose A/B panels share that XCD's L2.
+__device__ __forceinline__ int xcd_remap(int bid, int total) {
+    constexpr int X = 8;
+    const int q = total / X, r = total % X;
+    const int x = bid % X, slot = bid / X;
+    return x * q + min(x, r) + slot;
+}
+
+// Loads one k-step of an operand stored "k-contiguous" (element (row, k) at
+// base[k + row*ld]) into registers: 128 rows x 16 k; thread t takes k pair
+// 2*(t%8) of rows t/8 + 32*q.
+template <bool EDGE, bool VEC>
+__device__ __forceinline__ void load_kc(dv2 (&r)[4], const double* __restrict__ base, int64_t ld,
+                                        int64_t row0, int64_t rows, int64_t k0, int64_t klen) {
+    const int t = threadIdx.x;
+    const int kp = (t & 7) * 2;
+    const int rr = t >> 3;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        int64_t row = row0 + rr + 32 * q;
+        int64_t k = k0 + kp;
+        if (!EDGE) {
+            const double* p = base + k + row * ld;
+            if (VEC) {
+                r[q] = gld2(p);
+            } else {
+                r[q].x = gld(p);
+                r[q].y = gld(p + 1);
+            }
+        } else {
+            row = row < rows ? row : rows - 1;  // clamped rows feed outputs that are never stored
+            const double* p = base + row * ld;
+            r[q].x = (k < klen) ? gld(p + k) : 0.0;
+            r[q].y = (k + 1 < klen) ? gld(p + k + 1) : 0.0;
+        }
+    }
+}
+
+// Loads one k-step of an operand stored "row-contiguous" (element (row, k)
+// at base[row + k*ld]): thread t takes rows 2*(t%64), 2*(t%64)+1 of k
+// column t/64 + 4*q.
+template <bool EDGE, bool VEC>
+__device__ __forceinline__ void load_rc(dv2 (&r)[4], const double* __restrict__ base, int64_t ld,
+                                        int64_t row0, int64_t rows, int64_t k0, int64_t klen) {
+    const int t = threadIdx.x;
+    const int rp = (t & 63) * 2;
+    const int kc = t >> 6;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        int64_t row = row0 + rp;
+        int64_t k = k0 + kc + 4 * q;
+        if (!EDGE) {
+            const double* p = base + row + k * ld;
+            if (VEC) {
+                r[q] = gld2(p);
+            } else {
+                r[q].x = gld(p);
+                r[q].y = gld(p + 1);
+            }
+        } else {
+            if (k < klen) {
+                const double* p = base + k * ld;
+                int64_t ra = row < rows ? row : rows - 1;
+                int64_t rb = row + 1 < rows ? row + 1 : rows - 1;
+                r[q].x = gld(p + ra);
+                r[q].y = gld(p + rb);
+            } else {
+                r[q].x = 0.0;
+                r[q].y = 0.0;
+            }
+        }
+    }
+}
+
+__device__ __forceinline__ void store_kc(double* __restrict__ s, const dv2 (&r)[4]) {
+    const int t = threadIdx.x;
+    const int kp = (t & 7) * 2;
+    const int rr = t >> 3;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) *reinterpret_cast<dv2*>(&s[(rr + 32 * q) * LDK + kp]) = r[q];
+}
+
+__device__ __forceinline__ void store_rc(double* __restrict__ s, const dv2 (&r)[4]) {
+    const int t = threadIdx.x;
+    const int rp = (t & 63) * 2;
+    const int kc = t >> 6;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        s[rp * LDK + kc + 4 * q] = r[q].x;
+        s[(rp + 1) * LDK + kc + 4 * q] = r[q].y;
+    }
+}
+
+// op(A) is k-contiguous in memory when A is transposed; op(B) when B is not.
+template <bool TA, bool EDGE, bool VEC>
+__device__ __forceinline__ void load_a(dv2 (&r)[4], const Seg& s, int64_t m0, int64_t M, int64_t k0) {
+    if (TA) load_kc<EDGE, VEC>(r, s.a, s.lda, m0, M, k0, s.klen);
+    else    load_rc<EDGE, VEC>(r, s.a, s.lda, m0, M, k0, s.klen);
+}
+template <bool TB, bool EDGE, bool VEC>
+__device__ __forceinline__ void load_b(dv2 (&r)[4], const Seg& s, int64_t n0, int64_t N, int64_t k0) {
+    if (TB) load_rc<EDGE, VEC>(r, s.b, s.ldb, n0, N, k0, s.klen);
+    else    load_kc<EDGE, VEC>(r, s.b, s.ldb, n0, N, k0, s.klen);
+}
+
+template <bool TA, bool TB, bool VEC>
+__global__ __launch_bounds__(NT, 2) void gemm_f64_kernel(const Prob* __restrict__ probs, int nprob,
+                                                         const Seg* __restrict__ segs, int total_tiles) {
+    __shared__ double As[2][BM * LDK];
+    __shared__ double Bs[2][BN * LDK];
+
+    const int tile = xcd_remap(blockIdx.x, total_tiles);
+
+    // Problem owning this tile: last problem with tile0 <= tile.
+    int lo = 0, hi = nprob - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (probs[mid].tile0 <= tile) lo = mid; else hi = mid - 1;
+    }
+    const Prob P = probs[lo];
+    const int local = tile - P.tile0;
+
+    // Grouped raster: GROUP_M tile rows sweep the tile columns together.
+    const int per_group = GROUP_M * P.tiles_n;
+    const int group = local / per_group;
+    const int first_m = group * GROUP_M;
+    const int gsize = min(P.tiles_m - first_m, GROUP_M);
+    const int in_group = local - group * per_group;
+    const int tm = first_m + in_group % gsize;
+    const int tn = in_group / gsize;
+
+    const int64_t m0 = (int64_t)tm * BM;
+    const int64_t n0 = (int64_t)tn * BN;
+    const bool interior = (m0 + BM <= P.m) && (n0 + BN <= P.n);
+
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int wm = (wave >> 1) * 64;
+    const int wn = (wave & 1) * 64;
+    const int lr = lane & 15;
+    const int lg = lane >> 4;
+
+    dv4 acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = dv4{0.0, 0.0, 0.0, 0.0};
+
+    int s = P.seg0;
+    const int s_end = P.seg0 + P.nseg;
+    Seg cur = segs[s];
+    int64_t kk = 0;
+
+    dv2 ra[4], rb[4];
+    if (interior && BK <= cur.klen) {
+        load_a<TA, false, VEC>(ra, cur, m0, P.m, 0);
+        load_b<TB, false, VEC>(rb, cur, n0, P.n, 0);
+    } else {
+        load_a<TA, true, false>(ra, cur, m0, P.m, 0);
+        load_b<TB, true, false>(rb, cur, n0, P.n, 0);
+    }
+    int buf = 0;
+    if (TA) store_kc(As[0], ra); else store_rc(As[0], ra);
+    if (TB) store_rc(Bs[0], rb); else store_kc(Bs[0], rb);
+    __syncthreads();
+
+    while (true) {
+        // Next k-step position (possibly in the next segment).
+        int64_t nk = kk + BK;
+        int ns = s;
+        Seg nxt = cur;
+        if (nk >= cur.klen) {
+            ++ns;
+            nk = 0;
+            if (ns < s_end) nxt = segs[ns];
+        }
+        const bool has_next = ns < s_end;
+        if (has_next) {
+            if (interior && nk + BK <= nxt.klen) {
+                load_a<TA, false, VEC>(ra, nxt, m0, P.m, nk);
+                load_b<TB, false, VEC>(rb, nxt, n0, P.n, nk);
+            } else {
+                load_a<TA, true, false>(ra, nxt, m0, P.m, nk);
+                load_b<TB, true, false>(rb, nxt, n0, P.n, nk);
+            }
+        }
+
+        const double* as = As[buf];
+        const double* bs = Bs[buf];
+        // Each lane's 4 consecutive k values are read as two 128-bit LDS
+        // reads, one per half of the k-step, to keep register pressure low.
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            dv2 af[4], bf[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                af[i] = *reinterpret_cast<const dv2*>(&as[(wm + 16 * i + lr) * LDK + 4 * lg + 2 * h]);
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                bf[j] = *reinterpret_cast<const dv2*>(&bs[(wn + 16 * j + lr) * LDK + 4 * lg + 2 * h]);
+#pragma unroll
+            for (int q = 0; q < 2; ++q)
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+#pragma unroll
+                    for (int j = 0; j < 4; ++j)
+                        acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bf[j][q], af[i][q], acc[i][j], 0, 0, 0);
+        }
+
+        if (!has_next) break;
+        buf ^= 1;
+        if (TA) store_kc(As[buf], ra); else store_rc(As[buf], ra);
+        if (TB) store_rc(Bs[buf], rb); else store_kc(Bs[buf], rb);
+        __syncthreads();
+        s = ns;
+        kk = nk;
+        cur = nxt;
+    }
+
+    // Epilogue.  acc[i][j][v] = sum for C(m0 + wm + 16i + lr, n0 + wn + 16j + 4lg + v).
+    const double alpha = P.alpha, beta = P.beta;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int64_t m = m0 + wm + 16 * i + lr;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int64_t n = n0 + wn + 16 * j + 4 * lg + v;
+                if (interior || (m < P.m && n < P.n)) {
+                    __attribute__((address_space(1))) double* cp =
+                        (__attribute__((address_space(1))) double*)(P.c + m + n * P.ldc);
+                    double val = alpha * acc[i][j][v];
+                    if (beta != 0.0) val = fma(beta, *cp, val);
+                    *cp = val;
+                }
+            }
+        }
+    }
+}
+
+__global__ void scale_kernel(int64_t m, int64_t n, double beta, double* __restrict__ C, int64_t ldc) {
+    const int64_t total = m * n;
+    for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x < total;
+         x += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t col = x / m, row = x - col * m;
+        double* p = C + row + col * ldc;
+        *p = (beta == 0.0) ? 0.0 : beta * *p;
+    }
+}
+
+template <bool TA, bool TB>
+void launch_kind(bool vec, const Prob* dp, int np, const Seg* ds, int tiles, hipStream_t st) {
+    if (vec)
+        hipLaunchKernelGGL((gemm_f64_kernel<TA, TB, true>), dim3(tiles), dim3(NT), 0, st, dp, np, ds, tiles);
+    else
+        hipLaunchKernelGGL((gemm_f64_kernel<TA, TB, false>), dim3(tiles), dim3(NT), 0, st, dp, np, ds, tiles);
+}
+
+}  // namespace
+
+int launch_scale(int64_t m, int64_t n, double beta, double* C, int64_t ldc, hipStream_t stream) {
+    if (m <= 0 || n <= 0) return DG_OK;
+    int64_t total = m * n;
+    int64_t blocks = (total + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(scale_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, m, n, beta, C, ldc);
+    DG_LAUNCH_CHECK();
+    return DG_OK;
+}
+
+int launch_gemm_f64(char opA, char opB, const std::vector<Prob>& probs_in, const std::vector<Seg>& segs,
+                    hipStream_t stream) {
+    // Assign tile ranges; drop problems with nothing to compute.
+    std::vector<Prob> probs;
+    probs.reserve(probs_in.size());
+    int64_t tiles = 0;
+    bool vec = true;
+    for (const Prob& p0 : probs_in) {
+        if (p0.m <= 0 || p0.n <= 0 || p0.nseg <= 0) continue;
+        Prob p = p0;
+        p.tiles_m = (int32_t)((p.m + BM - 1) / BM);
+        p.tiles_n = (int32_t)((p.n + BN - 1) / BN);
+        p.tile0 = (int32_t)tiles;
+        tiles += (int64_t)p.tiles_m * p.tiles_n;
+        probs.push_back(p);
+        for (int s = p.seg0; s < p.seg0 + p.nseg; ++s) {
+            const Seg& g = segs[s];
+            if ((reinterpret_cast<uintptr_t>(g.a) & 15) || (reinterpret_cast<uintptr_t>(g.b) & 15) ||
+                (g.lda & 1) || (g.ldb & 1))
+                vec = false;
+        }
+    }
+    if (probs.empty()) return DG_OK;
+    if (tiles > INT32_MAX) return fail(DG_ERR_INVALID, "gemm: %lld output tiles exceed one launch", (long long)tiles);
+
+    const Prob* dp = static_cast<const Prob*>(cached_table(probs.data(), probs.size() * sizeof(Prob), stream));
+    if (!dp) return DG_ERR_ALLOC;
+    const Seg* ds = static_cast<const Seg*>(cached_table(segs.data(), segs.size() * sizeof(Seg), stream));
+    if (!ds) return DG_ERR_ALLOC;
+
+    const int np = (int)probs.size();
+    const int nt = (int)tiles;
+    const bool ta = opA == 'T', tb = opB == 'T';
+    if (!ta && !tb) launch_kind<false, false>(vec, dp, np, ds, nt, stream);
+    else if (!ta && tb) launch_kind<false, true>(vec, dp, np, ds, nt, stream);
+    else if (ta && !tb) launch_kind<true, false>(vec, dp, np, ds, nt, stream);
+    else launch_kind<true, true>(vec, dp, np, ds, nt, stream);
+    DG_LAUNCH_CHECK();
+    return DG_OK;
+}
+
+}  // namespace dg
+
+extern "C" int dg_dgemm(char opA, char opB, int64_t m, int64_t n, int64_t k, double alpha, const double* A,
+                        int64_t lda, const double* B, int64_t ldb, double beta, double* C, int64_t ldc,
+                        void* stream) {
+    using namespace dg;
+    const char oa = norm_op(opA), ob = norm_op(opB);
+    if (!oa) return fail(DG_ERR_INVALID, "dg_dgemm: invalid opA '%c'", opA);
+    if (!ob) return fail(DG_ERR_INVALID, "dg_dgemm: invalid opB '%c'", opB);
+    if (m < 0 || n < 0 || k < 0) return fail(DG_ERR_INVALID, "dg_dgemm: negative dimension");
+    const int64_t rowsA = oa == 'N' ? m : k;
+    const int64_t rowsB = ob == 'N' ? k : n;
+    if (lda < (rowsA > 1 ? rowsA : 1)) return fail(DG_ERR_INVALID, "dg_dgemm: lda %lld < %lld", (long long)lda, (long long)rowsA);
+    if (ldb < (rowsB > 1 ? rowsB : 1)) return fail(DG_ERR_INVALID, "dg_dgemm: ldb %lld < %lld", (long long)ldb, (long long)rowsB);
+    if (ldc < (m > 1 ? m : 1)) return fail(DG_ERR_INVALID, "dg_dgemm: ldc %lld < %lld", (long long)ldc, (long long)m);
+    if (m == 0 || n == 0) return DG_OK;
+    int rc = require_device();
+    if (rc != DG_OK) return rc;
+    hipStream_t st = as_stream(stream);
+    if (k == 0 || alpha == 0.0) return launch_scale(m, n, beta, C, ldc, st);
+    if (!A || !B || !C) return fail(DG_ERR_INVALID, "dg_dgemm: null matrix pointer");
+    std::vector<Seg> segs{Seg{A, B, lda, ldb, k}};
+    Prob p{};
+    p.m = m; p.n = n; p.c = C; p.ldc = ldc; p.alpha = alpha; p.beta = beta; p.seg0 = 0; p.nseg = 1;
+    std::vector<Prob> probs{p};
+    return launch_gemm_f64(oa, ob, probs, segs, st);
+}

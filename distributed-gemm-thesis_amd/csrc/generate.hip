@@ -1,0 +1,58 @@
+// Seeded, counter-based matrix generation (SplitMix64, Steele, Lea and
+// Flood, "Fast splittable pseudorandom number generators", OOPSLA 2014).
+// Element i takes output i of the generator seeded with `seed`, so the whole
+// matrix is produced in parallel and equals the sequential stream.
+#include "dg_internal.hpp"
+
+namespace dg {
+
+namespace {
+
+__device__ __forceinline__ uint64_t splitmix64_at(uint64_t seed, uint64_t i) {
+    uint64_t z = seed + (i + 1) * 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+// Second, independent stream for the exponent of the wide distribution.
+constexpr uint64_t kExpSeedXor = 0xD1B54A32D192ED03ull;
+
+__global__ void generate_kernel(uint64_t seed, int dist, double* __restrict__ out, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const uint64_t z = splitmix64_at(seed, (uint64_t)i);
+        const double u01 = (double)(z >> 11) * 0x1.0p-53;  // exact: 53-bit integer times 2^-53
+        double v;
+        if (dist == DG_DIST_UNIFORM_01) {
+            v = u01;
+        } else {
+            v = 2.0 * u01 - 1.0;  // exact in [-1, 1)
+            if (dist == DG_DIST_WIDE) {
+                const uint64_t ze = splitmix64_at(seed ^ kExpSeedXor, (uint64_t)i);
+                const int e = (int)((ze >> 32) % 41ull) - 20;
+                v = ldexp(v, e);  // exact power-of-two scaling
+            }
+        }
+        out[i] = v;
+    }
+}
+
+}  // namespace
+
+}  // namespace dg
+
+extern "C" int dg_generate(uint64_t seed, int dist, double* ptr, int64_t n, void* stream) {
+    using namespace dg;
+    if (n < 0) return fail(DG_ERR_INVALID, "dg_generate: negative size");
+    if (dist != DG_DIST_UNIFORM_PM1 && dist != DG_DIST_WIDE && dist != DG_DIST_UNIFORM_01)
+        return fail(DG_ERR_INVALID, "dg_generate: unknown distribution %d", dist);
+    if (n == 0) return DG_OK;
+    if (!ptr) return fail(DG_ERR_INVALID, "dg_generate: null pointer");
+    int rc = require_device();
+    if (rc != DG_OK) return rc;
+    int64_t blocks = (n + 255) / 256;
+    if (blocks > 8192) blocks = 8192;
+    hipLaunchKernelGGL(generate_kernel, dim3((unsigned)blocks), dim3(256), 0, as_stream(stream), seed, dist, ptr, n);
+    DG_LAUNCH_CHECK();
+    return DG_OK;
+}

@@ -1,0 +1,195 @@
+/*
+ * dgamd.h — C ABI of the MI355X (gfx950) per-rank GEMM engine.
+ *
+ * Every entry point takes plain pointers and 64-bit sizes, returns a
+ * dg_status, never throws, and is stream-ordered on the HIP stream passed as
+ * `stream` (NULL = the default stream).  Matrices are column-major in the BLAS
+ * convention; device pointers are owned by the caller.  On failure the
+ * message of the last error on the calling thread is available from
+ * dg_last_error().
+ *
+ * Reference interfaces that each entry point replaces are cited as
+ * path:line inside bvrettos/Distributed-GEMM-Thesis.
+ */
+#ifndef DGAMD_H
+#define DGAMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum dg_status {
+    DG_OK = 0,
+    DG_ERR_INVALID = 1,     /* bad dimension, leading dimension, grid or op   */
+    DG_ERR_UNSUPPORTED = 2, /* valid request this build does not implement    */
+    DG_ERR_HIP = 3,         /* a HIP runtime call failed                      */
+    DG_ERR_NO_DEVICE = 4,   /* no usable gfx950 device                        */
+    DG_ERR_ALLOC = 5,       /* device or pinned allocation failed             */
+    DG_ERR_SCHEDULE = 6,    /* Cannon rotation schedule does not cover K      */
+    DG_ERR_WORKSPACE = 7    /* workspace argument too small                   */
+} dg_status;
+
+/* Message describing the last failure on this thread ("" if none). */
+const char* dg_last_error(void);
+/* Library version, major*10000 + minor*100 + patch. */
+int dg_version(void);
+/* Non-zero when a device is present and its architecture is gfx950. */
+int dg_device_ok(void);
+
+/* ------------------------------------------------------------------ GEMM -- */
+
+/* C = alpha*op(A)*op(B) + beta*C on the FP64 matrix cores
+ * (v_mfma_f64_16x16x4_f64).  op is 'N'/'n', 'T'/'t' or 'C'/'c' (real data:
+ * 'C' == 'T').  beta == 0 does not read C (NaN in C does not propagate).
+ * Replaces: cublasDgemm(handle, opA, opB, m, n, k, &alpha, A, lda, B, ldb,
+ *           &beta, C, ldc) at src/25DCannon.cpp:324-325 and 356-357,
+ *           src/25DCannonNCCL.cpp:165-166, 202-203, 260-261, 291-292;
+ *           op parsing as charToCublasTransOp, src/Utilities/TransposeEmblems.cpp:49-65. */
+int dg_dgemm(char opA, char opB, int64_t m, int64_t n, int64_t k,
+             double alpha, const double* A, int64_t lda,
+             const double* B, int64_t ldb,
+             double beta, double* C, int64_t ldc, void* stream);
+
+/* ------------------------------------------------------ 2D block layout -- */
+
+/* Split `n` into `parts` contiguous ranges, the last range absorbing the
+ * remainder: offsets[0..parts], offsets[i] = i*(n/parts), offsets[parts] = n.
+ * Replaces the tile extents of Summa25Decomposer (localM = M/dRow,
+ * src/25DCannon.cpp:44-46) and BlockSequentialDecomposer
+ * (src/Decompositions/2DBlockSequentialDecomposition.cpp:11-25, 77-89). */
+int dg_block_splits(int64_t n, int parts, int64_t* offsets);
+
+/* Most-square process grid for `size` devices: *dcol = largest divisor of
+ * size that is <= floor(sqrt(size)), *drow = size / *dcol.
+ * Replaces: BlockSequentialDecomposer::calculateVirtualDeviceGrid,
+ * src/Decompositions/2DBlockSequentialDecomposition.cpp:27-54 (and defines the
+ * calculateProcessGrid that src/Decompositions/pblasDecomposition.cpp:31 calls
+ * but the reference never defines). */
+int dg_virtual_grid(int size, int* drow, int* dcol);
+
+/* Pack a global column-major matrix G (rows x cols, leading dim ldg) into
+ * per-rank tiles.  Tile (i,j) covers rows [row_off[i], row_off[i+1]) and
+ * columns [col_off[j], col_off[j+1]); tiles are stored contiguously in rank
+ * order r = i*ncb + j, each column-major with leading dimension = its row
+ * count.  row_off / col_off are HOST arrays of nrb+1 / ncb+1 offsets
+ * (nrb, ncb <= 256).  Bit-exact copy.
+ * Replaces: Summa25Decomposer::decompose2D, src/25DCannon.cpp:87-149;
+ *           BlockSequentialDecomposer::deliverMatrix,
+ *           src/Decompositions/2DBlockSequentialDecomposition.cpp:56-162;
+ *           copyBlock/transferBlock, src/Transfers/sharedTransfers.cpp:29-63,
+ *           src/Transfers/mpiTransfers.cpp:16-35. */
+int dg_pack_block2d(const double* G, int64_t ldg, int64_t rows, int64_t cols,
+                    const int64_t* row_off, int nrb, const int64_t* col_off, int ncb,
+                    double* packed, void* stream);
+
+/* Inverse of dg_pack_block2d: scatter the packed tiles back into G.
+ * Replaces: Summa25Decomposer::reorderMatrix, src/25DCannon.cpp:231-284;
+ *           BlockSequentialDecomposer::gatherResult,
+ *           src/Decompositions/2DBlockSequentialDecomposition.cpp:164-253;
+ *           receiveBlock, src/Transfers/mpiTransfers.cpp:37-57. */
+int dg_unpack_block2d(const double* packed,
+                      const int64_t* row_off, int nrb, const int64_t* col_off, int ncb,
+                      double* G, int64_t ldg, int64_t rows, int64_t cols, void* stream);
+
+/* Strided block copy: dst[r + c*ldd] = src[r + c*lds] for r < rows, c < cols
+ * (device to device).  Replaces: copyBlock, src/Transfers/sharedTransfers.cpp:29-63
+ * and memcpy2D, :65-81. */
+int dg_copy_block(int64_t rows, int64_t cols, const double* src, int64_t lds,
+                  double* dst, int64_t ldd, void* stream);
+
+/* ------------------------------------------------- block-cyclic layout -- */
+
+/* ScaLAPACK NUMROC: rows (or columns) of an n-long dimension split in nb
+ * blocks that process iproc owns among nprocs, the first block on isrcproc.
+ * Replaces: numroc, src/Decompositions/pblasDecomposition.cpp:3-23. */
+int64_t dg_numroc(int64_t n, int64_t nb, int iproc, int isrcproc, int nprocs);
+
+/* Elements of the packed block-cyclic buffer: sum over the P x Q process
+ * grid of numroc(rows, mb, pr, 0, P) * numroc(cols, nb, pc, 0, Q). */
+int64_t dg_block_cyclic_size(int64_t rows, int64_t cols, int64_t mb, int64_t nb, int P, int Q);
+
+/* Pack a global column-major matrix into the local matrices of a P x Q
+ * row-major process grid (rank = pr*Q + pc, as procRow = rank/dCol), block
+ * size mb x nb, first block on process (0,0).  Local matrices are stored
+ * contiguously in rank order, each column-major with leading dimension
+ * numroc(rows, mb, pr, 0, P).  Bit-exact.
+ * Replaces: pblasDecomposer::scatterMatrix,
+ *           src/Decompositions/pblasDecomposition.cpp:73-148. */
+int dg_pack_block_cyclic(const double* G, int64_t ldg, int64_t rows, int64_t cols,
+                         int64_t mb, int64_t nb, int P, int Q, double* packed, void* stream);
+
+/* Inverse of dg_pack_block_cyclic.
+ * Replaces: pblasDecomposer::gatherMatrix,
+ *           src/Decompositions/pblasDecomposition.cpp:150-221. */
+int dg_unpack_block_cyclic(const double* packed, int64_t mb, int64_t nb, int P, int Q,
+                           double* G, int64_t ldg, int64_t rows, int64_t cols, void* stream);
+
+/* ------------------------------------------------ virtual-grid executors -- */
+
+#define DG_SCHED_AUTO 0      /* reference rotation where it covers K, else standard */
+#define DG_SCHED_REFERENCE 1 /* the reference's rotation offsets r(r+1)/2          */
+#define DG_SCHED_STANDARD 2  /* textbook Cannon: shift by one per step             */
+
+/* Elements of workspace dg_cannon25d needs: (c-1) * M * N. */
+int64_t dg_cannon25d_workspace(int64_t M, int64_t N, int p, int c);
+
+/* 2D (c = 1) / 2.5D (c > 1) Cannon on a virtual p x p x c grid, one GPU.
+ * A_packed, B_packed, C_packed hold the tiles of the STORED matrices in the
+ * dg_pack_block2d layout over a p x p grid (uniform splits: p divides each
+ * stored dimension).  Layer k, step r, rank (i,j) multiplies op(A)_{i,m} by
+ * op(B)_{m,j} with m = (i + j + off[k][r]) mod p; the shifts of the reference
+ * are these index rotations, nothing is moved.  Layer 0 starts from beta*C,
+ * layers k > 0 from 0 into `workspace` ((c-1)*M*N doubles), and the layers
+ * are summed into C_packed in HBM.  `schedule` is one of DG_SCHED_*.
+ * Replaces: Summa25Decomposer::multiply, src/25DCannon.cpp:286-375 (skew
+ *           303-315, rotations 331-366, beta policy 322/352, reduceResult
+ *           203-228, broadcastToStack 174-194) and serializedMultiplyNCCL,
+ *           src/25DCannonNCCL.cpp:216-304. */
+int dg_cannon25d(char opA, char opB, int64_t M, int64_t N, int64_t K, double alpha,
+                 const double* A_packed, const double* B_packed, double beta,
+                 double* C_packed, int p, int c, int schedule,
+                 double* workspace, int64_t workspace_elems, void* stream);
+
+/* Rank (i,j,k)'s m index at step r under `schedule` (host helper for tests
+ * and callers that print the schedule); -1 if the arguments are invalid.
+ * Replaces the target arithmetic of src/25DCannon.cpp:303-310, 333-344 and
+ * truncatedDivisionRemainer, src/25DCannonNCCL.cpp:6-9. */
+int dg_cannon_m_index(int p, int c, int schedule, int i, int j, int k, int r);
+
+/* SUMMA on a virtual P x Q grid over block-cyclic local matrices
+ * (dg_pack_block_cyclic layout).  A is M x K with blocks mb x kb, B is K x N
+ * with blocks kb x nb, C is M x N with blocks mb x nb.  Panel t of width kb is
+ * read in place from its owners (process column t mod Q of A, process row
+ * t mod P of B): the panel broadcasts are index rotations.  opA = opB = 'N'.
+ * New relative to the reference (include/summa.hpp:1-28 is empty; SUMMA
+ * exists there only inside cublasMpGemm, src/Wrappers/cuBLASMp/cuBLASMpWrapped.cpp:198-223). */
+int dg_summa(char opA, char opB, int64_t M, int64_t N, int64_t K, double alpha,
+             const double* A_packed, const double* B_packed, double beta, double* C_packed,
+             int64_t mb, int64_t nb, int64_t kb, int P, int Q, void* stream);
+
+/* C[x] += sum_{l < nlayers} partial[l*n + x] for x < n (HBM reduction of the
+ * 2.5D layers, summed in layer order).
+ * Replaces: Summa25Decomposer::reduceResult (MPI_Reduce SUM),
+ *           src/25DCannon.cpp:203-228; reduceResultNCCL, src/25DCannonNCCL.cpp:306-320. */
+int dg_layer_reduce(double* C, const double* partial, int64_t n, int nlayers, void* stream);
+
+/* ------------------------------------------------------------ generation -- */
+
+#define DG_DIST_UNIFORM_PM1 0  /* u in [-1, 1): 2*(x>>11)*2^-53 - 1                 */
+#define DG_DIST_WIDE 1         /* u * 2^e, e uniform integer in [-20, 20]          */
+#define DG_DIST_UNIFORM_01 2   /* (x>>11)*2^-53 in [0, 1)                          */
+
+/* Fill ptr[0..n) from SplitMix64 (Steele, Lea, Flood 2014): element i uses
+ * output i of the generator seeded with `seed` (counter based, so any
+ * element can be produced independently).  Bit-identical to the CPU oracle.
+ * Replaces: MatrixInit / generateMatrixGPU (cuRAND, clock seeded),
+ *           src/Utilities/cmatrix.cpp:104-169, 239-297. */
+int dg_generate(uint64_t seed, int dist, double* ptr, int64_t n, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DGAMD_H */

@@ -1,0 +1,221 @@
+"""numpy front end of the CPU oracle.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg as the checker; the product never imports it.
+
+* `lib()` loads liboracle.so, the C restatement (oracle.c, citations in
+  oracle.h), built by oracle/Makefile.
+* `host_dgemm` is the host BLAS dgemm the reference validates against
+  (cblas_dgemm in src/Wrappers/cuBLASMp/cuBLASMpValidator.cpp:48,
+  src/Wrappers/Cosma/cosmaValidators.cpp:52): here numpy's BLAS (OpenBLAS).
+* `ref_lib()` loads _ref/libref_validation.so, the reference's own
+  src/Utilities/Validation.cpp compiled by the Makefile, when present.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+REF_PATH = os.path.join(HERE, "_ref", "libref_validation.so")
+
+_i64 = ctypes.c_int64
+_int = ctypes.c_int
+_dbl = ctypes.c_double
+_dp = ctypes.POINTER(ctypes.c_double)
+_i64p = ctypes.POINTER(ctypes.c_int64)
+_ip = ctypes.POINTER(ctypes.c_int)
+_chr = ctypes.c_char
+
+_SIGS = {
+    "or_numroc": (_i64, [_i64, _i64, _int, _int, _int]),
+    "or_virtual_grid": (None, [_int, _ip, _ip]),
+    "or_block_splits": (None, [_i64, _int, _i64p]),
+    "or_copy_block": (None, [_i64, _i64, _dp, _i64, _dp, _i64]),
+    "or_pack_block2d": (None, [_dp, _i64, _i64, _i64, _i64p, _int, _i64p, _int, _dp]),
+    "or_unpack_block2d": (None, [_dp, _i64p, _int, _i64p, _int, _dp, _i64, _i64, _i64]),
+    "or_pack_block_cyclic": (None, [_dp, _i64, _i64, _i64, _i64, _i64, _int, _int, _dp]),
+    "or_unpack_block_cyclic": (None, [_dp, _i64, _i64, _int, _int, _dp, _i64, _i64, _i64]),
+    "or_cannon_simulate": (_int, [_int, _int, _ip, _ip]),
+    "or_dgemm": (None, [_chr, _chr, _i64, _i64, _i64, _dbl, _dp, _i64, _dp, _i64, _dbl, _dp, _i64]),
+    "or_cannon25d": (None, [_chr, _chr, _i64, _i64, _i64, _dbl, _dp, _dp, _dbl, _dp, _int, _int, _int]),
+    "or_summa": (None, [_i64, _i64, _i64, _dbl, _dp, _dp, _dbl, _dp, _i64, _i64, _i64, _int, _int]),
+    "or_layer_reduce": (None, [_dp, _dp, _i64, _int]),
+    "or_Derror": (_dbl, [_dbl, _dbl]),
+    "or_Dvec_diff": (ctypes.c_long, [_dp, _dp, _i64, _dbl]),
+    "or_Dtest_equality": (ctypes.c_short, [_dp, _dp, _i64]),
+    "or_splitmix64_next": (ctypes.c_uint64, [ctypes.POINTER(ctypes.c_uint64)]),
+    "or_generate": (None, [ctypes.c_uint64, _int, _dp, _i64]),
+}
+
+_lib = None
+_ref = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("oracle not built: run `make -C oracle`")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def ref_lib():
+    """The reference's compiled Validation.cpp, or None if not built."""
+    global _ref
+    if _ref is None and os.path.exists(REF_PATH):
+        R = ctypes.CDLL(REF_PATH)
+        R.ref_Dtest_equality.restype = ctypes.c_short
+        R.ref_Dtest_equality.argtypes = [_dp, _dp, ctypes.c_longlong]
+        R.ref_Dvec_diff.restype = ctypes.c_long
+        R.ref_Dvec_diff.argtypes = [_dp, _dp, ctypes.c_longlong, _dbl]
+        _ref = R
+    return _ref
+
+
+def _p(a):
+    if a.dtype != np.float64 or not (a.flags["C_CONTIGUOUS"] or a.flags["F_CONTIGUOUS"]):
+        raise TypeError("oracle arrays must be contiguous float64")
+    return a.ctypes.data_as(_dp)
+
+
+def _offs(seq):
+    return (ctypes.c_int64 * len(seq))(*[int(v) for v in seq])
+
+
+def _op(c):
+    return c.encode() if isinstance(c, str) else c
+
+
+# ---------------------------------------------------------------- index maps
+
+def numroc(n, nb, iproc, isrc, nprocs):
+    return lib().or_numroc(n, nb, iproc, isrc, nprocs)
+
+
+def virtual_grid(size):
+    r, c = _int(), _int()
+    lib().or_virtual_grid(size, ctypes.byref(r), ctypes.byref(c))
+    return r.value, c.value
+
+
+def block_splits(n, parts):
+    out = (ctypes.c_int64 * (parts + 1))()
+    lib().or_block_splits(n, parts, out)
+    return list(out)
+
+
+def cannon_simulate(p, c):
+    """(mA, mB) arrays of shape (c, p, p, p//c): k-tile index held at each step."""
+    steps = p // c
+    a = np.zeros(c * p * p * steps, dtype=np.int32)
+    b = np.zeros_like(a)
+    lib().or_cannon_simulate(p, c, a.ctypes.data_as(_ip), b.ctypes.data_as(_ip))
+    return a.reshape(c, p, p, steps), b.reshape(c, p, p, steps)
+
+
+# ---------------------------------------------------------------- layouts
+# Matrices are flat float64 arrays in column-major order.
+
+def pack_block2d(G, ldg, rows, cols, row_off, col_off):
+    out = np.zeros(rows * cols, dtype=np.float64)
+    lib().or_pack_block2d(_p(G), ldg, rows, cols, _offs(row_off), len(row_off) - 1, _offs(col_off),
+                          len(col_off) - 1, _p(out))
+    return out
+
+
+def unpack_block2d(packed, row_off, col_off, G, ldg, rows, cols):
+    lib().or_unpack_block2d(_p(packed), _offs(row_off), len(row_off) - 1, _offs(col_off), len(col_off) - 1,
+                            _p(G), ldg, rows, cols)
+    return G
+
+
+def block_cyclic_size(rows, cols, mb, nb, P, Q):
+    return sum(numroc(rows, mb, pr, 0, P) * numroc(cols, nb, pc, 0, Q) for pr in range(P) for pc in range(Q))
+
+
+def pack_block_cyclic(G, ldg, rows, cols, mb, nb, P, Q):
+    out = np.zeros(block_cyclic_size(rows, cols, mb, nb, P, Q), dtype=np.float64)
+    lib().or_pack_block_cyclic(_p(G), ldg, rows, cols, mb, nb, P, Q, _p(out))
+    return out
+
+
+def unpack_block_cyclic(packed, mb, nb, P, Q, G, ldg, rows, cols):
+    lib().or_unpack_block_cyclic(_p(packed), mb, nb, P, Q, _p(G), ldg, rows, cols)
+    return G
+
+
+def layer_reduce(C, partial, n, nlayers):
+    lib().or_layer_reduce(_p(C), _p(partial), n, nlayers)
+    return C
+
+
+# ---------------------------------------------------------------- arithmetic
+
+def dgemm_loops(opA, opB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc):
+    """Textbook triple loop (long double sums), for small sizes."""
+    lib().or_dgemm(_op(opA), _op(opB), m, n, k, alpha, _p(A), lda, _p(B), ldb, beta, _p(C), ldc)
+    return C
+
+
+def host_dgemm(opA, opB, alpha, A, B, beta, C):
+    """C = alpha*op(A) op(B) + beta*C with numpy's BLAS on 2-D arrays (the
+    host dgemm oracle).  beta == 0 ignores C's contents."""
+    a = A.T if opA in ("T", "t", "C", "c") else A
+    b = B.T if opB in ("T", "t", "C", "c") else B
+    prod = alpha * (a @ b)
+    if beta == 0.0:
+        return prod
+    return prod + beta * C
+
+
+def cannon25d(opA, opB, M, N, K, alpha, Ap, Bp, beta, Cp, p, c, sched_std=False):
+    lib().or_cannon25d(_op(opA), _op(opB), M, N, K, alpha, _p(Ap), _p(Bp), beta, _p(Cp), p, c, int(sched_std))
+    return Cp
+
+
+def summa(M, N, K, alpha, Ap, Bp, beta, Cp, mb, nb, kb, P, Q):
+    lib().or_summa(M, N, K, alpha, _p(Ap), _p(Bp), beta, _p(Cp), mb, nb, kb, P, Q)
+    return Cp
+
+
+def rel_frobenius(x, ref):
+    """||x - ref||_F / ||ref||_F."""
+    x = np.asarray(x, dtype=np.float64).ravel()
+    ref = np.asarray(ref, dtype=np.float64).ravel()
+    den = np.linalg.norm(ref)
+    return float(np.linalg.norm(x - ref) / (den if den > 0 else 1.0))
+
+
+# ---------------------------------------------------------------- validation
+
+def Derror(a, b):
+    return lib().or_Derror(a, b)
+
+
+def Dvec_diff(a, b, eps):
+    return lib().or_Dvec_diff(_p(a), _p(b), a.size, eps)
+
+
+def Dtest_equality(C_comp, C):
+    return lib().or_Dtest_equality(_p(C_comp), _p(C), C_comp.size)
+
+
+# ---------------------------------------------------------------- generation
+
+def splitmix64_stream(seed, n):
+    s = ctypes.c_uint64(seed)
+    return [lib().or_splitmix64_next(ctypes.byref(s)) for _ in range(n)]
+
+
+def generate(seed, dist, n):
+    out = np.empty(n, dtype=np.float64)
+    lib().or_generate(seed, dist, _p(out), n)
+    return out

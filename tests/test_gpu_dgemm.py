@@ -1,0 +1,112 @@
+"""FP64 MFMA DGEMM (dg_dgemm) against the host BLAS dgemm oracle.
+
+Tolerance: relative Frobenius error <= 1e-13 against the host dgemm (the
+acceptance bar of the reference's validation against cblas_dgemm); inputs
+whose products and partial sums are exact integers must match bit for bit,
+whatever the summation order.
+"""
+import numpy as np
+import pytest
+import torch
+
+import dgamd
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-13
+
+
+def colmajor(rows, cols, ld, rng, dev, integer=False, fill=None):
+    """A (ld x cols) column-major device buffer whose top rows x cols block is
+    random; returns (device tensor, host 2-D view of the logical matrix)."""
+    buf = np.full((cols, ld), np.nan if fill is None else fill, dtype=np.float64)
+    if integer:
+        vals = rng.integers(-8, 9, size=(cols, rows)).astype(np.float64)
+    else:
+        vals = rng.uniform(-1.0, 1.0, size=(cols, rows))
+    buf[:, :rows] = vals
+    t = torch.from_numpy(buf.reshape(-1).copy()).to(dev)
+    return t, vals.T.copy()
+
+
+def run(dev, opA, opB, m, n, k, alpha, beta, pad=(0, 0, 0), integer=False, seed=0):
+    rng = np.random.default_rng(seed)
+    ra, ca = (m, k) if opA == "N" else (k, m)
+    rb, cb = (k, n) if opB == "N" else (n, k)
+    lda, ldb, ldc = max(ra, 1) + pad[0], max(rb, 1) + pad[1], max(m, 1) + pad[2]
+    A, a = colmajor(ra, ca, lda, rng, dev, integer)
+    B, b = colmajor(rb, cb, ldb, rng, dev, integer)
+    C, c = colmajor(m, n, ldc, rng, dev, integer, fill=7.0)
+    if beta == 0.0:
+        # beta == 0 must not read C: poison the logical block with NaN
+        C.view(n, ldc)[:, :m] = float("nan")
+    dgamd.dgemm_t(opA, opB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc)
+    torch.cuda.synchronize()
+    out = C.cpu().numpy().reshape(n, ldc)
+    got = out[:, :m].T
+    ref = oracle.host_dgemm(opA, opB, alpha, a, b, beta, c)
+    # rows of the padding must be untouched
+    if pad[2]:
+        assert np.all(out[:, m:] == 7.0)
+    return got, ref
+
+
+def test_mfma_layout_exact(dev):
+    # one 16x16 output block, k = 4: checks the MFMA operand/result lane layout
+    got, ref = run(dev, "N", "N", 16, 16, 4, 1.0, 0.0, integer=True)
+    assert np.array_equal(got, ref)
+
+
+@pytest.mark.parametrize("opA,opB", [("N", "N"), ("N", "T"), ("T", "N"), ("T", "T")])
+def test_ops_exact_integer(dev, opA, opB):
+    # interior tiles, edge tiles and a partial k-step, all exact
+    got, ref = run(dev, opA, opB, 300, 200, 77, 1.0, 0.0, integer=True, seed=1)
+    assert np.array_equal(got, ref)
+
+
+@pytest.mark.parametrize("opA,opB", [("N", "N"), ("N", "T"), ("T", "N"), ("T", "T")])
+def test_ops_random(dev, opA, opB):
+    got, ref = run(dev, opA, opB, 512, 384, 640, 1.5, 0.5, seed=2)
+    assert oracle.rel_frobenius(got, ref) <= TOL
+
+
+def test_beta_zero_nan_c(dev):
+    got, ref = run(dev, "N", "N", 256, 256, 256, 1.0, 0.0, seed=3)
+    assert np.all(np.isfinite(got))
+    assert oracle.rel_frobenius(got, ref) <= TOL
+
+
+@pytest.mark.parametrize("pad", [(1, 0, 0), (0, 1, 0), (1, 1, 5)])
+def test_odd_leading_dims(dev, pad):
+    # odd lda (257) or ldb (131) forces the 64-bit load path; ldc padding must stay untouched
+    got, ref = run(dev, "N", "T", 256, 131, 129, -0.75, 2.0, pad=pad, seed=4)
+    assert oracle.rel_frobenius(got, ref) <= TOL
+
+
+@pytest.mark.parametrize("m,n,k", [(1, 1, 1), (1, 300, 17), (300, 1, 33), (15, 17, 1), (129, 129, 16)])
+def test_small_and_ragged(dev, m, n, k):
+    got, ref = run(dev, "N", "N", m, n, k, 1.0, 0.25, seed=5)
+    assert oracle.rel_frobenius(got, ref) <= TOL
+
+
+def test_k_zero_scales_c(dev):
+    got, ref = run(dev, "N", "N", 64, 64, 0, 1.0, 0.5, seed=6)
+    assert np.array_equal(got, ref)
+
+
+def test_config1_shape_2048(dev):
+    # the tile product of the 2048^3 Cannon configuration at full size
+    got, ref = run(dev, "N", "N", 2048, 2048, 2048, 1.0, 0.0, seed=7)
+    err = oracle.rel_frobenius(got, ref)
+    assert err <= TOL, err
+    # the reference's digit-count report (Validation.cpp:57-83), informative only:
+    # its per-element relative error is unbounded near zero outputs
+    digits = oracle.Dtest_equality(np.ascontiguousarray(ref.T).ravel(), np.ascontiguousarray(got.T).ravel())
+    print("Dtest_equality digits:", digits, "rel Frobenius:", err)
+
+
+def test_invalid_arguments(dev):
+    L = dgamd.lib()
+    assert L.dg_dgemm(b"X", b"N", 4, 4, 4, 1.0, None, 4, None, 4, 0.0, None, 4, None) == dgamd.DG_ERR_INVALID
+    assert L.dg_dgemm(b"N", b"N", 4, 4, 4, 1.0, None, 3, None, 4, 0.0, None, 4, None) == dgamd.DG_ERR_INVALID

@@ -1,0 +1,103 @@
+"""Pack/unpack, strided copy, layer reduction and generation on the GPU,
+bit-exact against the CPU oracle (data movement and fixed-order sums)."""
+import numpy as np
+import pytest
+import torch
+
+import dgamd
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def dev_copy(a, dev):
+    return torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+
+
+@pytest.mark.parametrize("dist", [dgamd.DG_DIST_UNIFORM_PM1, dgamd.DG_DIST_WIDE, dgamd.DG_DIST_UNIFORM_01])
+def test_generate_bit_exact(dev, dist):
+    n = 100003
+    t = torch.empty(n, dtype=torch.float64, device=dev)
+    dgamd.generate_t(12345, dist, t)
+    got = t.cpu().numpy()
+    ref = oracle.generate(12345, dist, n)
+    assert np.array_equal(got.view(np.uint64), ref.view(np.uint64))
+
+
+@pytest.mark.parametrize("rows,cols,ldg,pr,pc", [(2048, 2048, 2048, 2, 2), (23, 17, 29, 7, 5), (20000, 40, 20003, 7, 5),
+                                                 (5, 3, 5, 1, 1), (9, 9, 9, 3, 3)])
+def test_block2d_pack_unpack(dev, rows, cols, ldg, pr, pc):
+    rng = np.random.default_rng(rows * 31 + cols)
+    G = rng.standard_normal(ldg * cols)
+    ro, co = oracle.block_splits(rows, pr), oracle.block_splits(cols, pc)
+    ref = oracle.pack_block2d(G, ldg, rows, cols, ro, co)
+    Gd = dev_copy(G, dev)
+    P = torch.full((rows * cols,), float("nan"), dtype=torch.float64, device=dev)
+    dgamd.pack_block2d_t(Gd, ldg, rows, cols, ro, co, P)
+    assert np.array_equal(P.cpu().numpy(), ref)
+    G2 = torch.full((ldg * cols,), 3.5, dtype=torch.float64, device=dev)
+    dgamd.unpack_block2d_t(P, ro, co, G2, ldg, rows, cols)
+    g2 = G2.cpu().numpy().reshape(cols, ldg)
+    assert np.array_equal(g2[:, :rows], G.reshape(cols, ldg)[:, :rows])
+    assert np.all(g2[:, rows:] == 3.5)  # rows beyond the matrix untouched
+
+
+@pytest.mark.parametrize("rows,cols,mb,nb,P,Q", [(23, 17, 3, 2, 7, 5), (64, 64, 8, 8, 4, 4), (1000, 777, 64, 100, 3, 2),
+                                                 (5, 5, 8, 8, 2, 2), (512, 520, 512, 512, 4, 4)])
+def test_block_cyclic_pack_unpack(dev, rows, cols, mb, nb, P, Q):
+    rng = np.random.default_rng(rows + 7 * cols)
+    ldg = rows + 3
+    G = rng.standard_normal(ldg * cols)
+    ref = oracle.pack_block_cyclic(G, ldg, rows, cols, mb, nb, P, Q)
+    assert dgamd.block_cyclic_size(rows, cols, mb, nb, P, Q) == ref.size
+    Gd = dev_copy(G, dev)
+    Pk = torch.full((ref.size,), float("nan"), dtype=torch.float64, device=dev)
+    dgamd.pack_block_cyclic_t(Gd, ldg, rows, cols, mb, nb, P, Q, Pk)
+    assert np.array_equal(Pk.cpu().numpy(), ref)
+    G2 = torch.full((ldg * cols,), -1.0, dtype=torch.float64, device=dev)
+    dgamd.unpack_block_cyclic_t(Pk, mb, nb, P, Q, G2, ldg, rows, cols)
+    g2 = G2.cpu().numpy().reshape(cols, ldg)
+    assert np.array_equal(g2[:, :rows], G.reshape(cols, ldg)[:, :rows])
+    assert np.all(g2[:, rows:] == -1.0)
+
+
+def test_block_cyclic_full_size_round_trip(dev):
+    # the 16384^2, 512x512-block, 4x4-grid layout at full size: unpack(pack(X)) == X
+    n = 16384
+    X = torch.empty(n * n, dtype=torch.float64, device=dev)
+    dgamd.generate_t(99, dgamd.DG_DIST_UNIFORM_PM1, X)
+    Pk = torch.empty_like(X)
+    dgamd.pack_block_cyclic_t(X, n, n, n, 512, 512, 4, 4, Pk)
+    Y = torch.empty_like(X)
+    dgamd.unpack_block_cyclic_t(Pk, 512, 512, 4, 4, Y, n, n, n)
+    assert torch.equal(X, Y)
+    # spot-check a few packed elements against the definition
+    Pk_h = Pk.view(-1)
+    for (I, J) in [(0, 0), (511, 512), (513, 1), (16383, 16383), (4097, 9000)]:
+        pr, pc = (I // 512) % 4, (J // 512) % 4
+        li = (I // 512 // 4) * 512 + I % 512
+        lj = (J // 512 // 4) * 512 + J % 512
+        base = (pr * 4 + pc) * 4096 * 4096
+        assert Pk_h[base + li + lj * 4096].item() == X[I + J * n].item()
+
+
+def test_copy_block(dev):
+    rng = np.random.default_rng(3)
+    src = rng.standard_normal(101 * 50)
+    dst0 = np.full(77 * 50, 9.0)
+    ref = dst0.copy()
+    oracle.lib().or_copy_block(60, 50, oracle._p(src), 101, oracle._p(ref), 77)
+    S, D = dev_copy(src, dev), dev_copy(dst0, dev)
+    dgamd.copy_block_t(60, 50, S, 101, D, 77)
+    assert np.array_equal(D.cpu().numpy(), ref)
+
+
+@pytest.mark.parametrize("n,layers", [(1 << 20, 1), (1000001, 3), (7, 2)])
+def test_layer_reduce_bit_exact(dev, n, layers):
+    rng = np.random.default_rng(n)
+    C = rng.standard_normal(n)
+    part = rng.standard_normal(n * layers)
+    ref = oracle.layer_reduce(C.copy(), part, n, layers)
+    Cd, Pd = dev_copy(C, dev), dev_copy(part, dev)
+    dgamd.layer_reduce_t(Cd, Pd, n, layers)
+    assert np.array_equal(Cd.cpu().numpy(), ref)
