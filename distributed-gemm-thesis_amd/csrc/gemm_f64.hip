@@ -20,6 +20,8 @@
 // matrix core sums in.
 // The MFMA is issued as D = B^T A^T, so each output register holds 16
 // consecutive rows of one C column: the epilogue stores are 128-byte runs.
+// Result layout of v_mfma_f64_16x16x4_f64: lane l, register v holds
+// D(l/16 + 4v, l%16) (checked on the device by the exact-integer tests).
 #include "dg_internal.hpp"
 
 namespace dg {
@@ -265,7 +267,9 @@ __global__ __launch_bounds__(NT, 2) void gemm_f64_kernel(const Prob* __restrict_
         cur = nxt;
     }
 
-    // Epilogue.  acc[i][j][v] = sum for C(m0 + wm + 16i + lr, n0 + wn + 16j + 4lg + v).
+    // Epilogue.  Register v of lane group lg holds MFMA result row lg + 4v
+    // (the f64 16x16x4 result layout), i.e.
+    // acc[i][j][v] = sum for C(m0 + wm + 16i + lr, n0 + wn + 16j + lg + 4v).
     const double alpha = P.alpha, beta = P.beta;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
@@ -274,7 +278,7 @@ __global__ __launch_bounds__(NT, 2) void gemm_f64_kernel(const Prob* __restrict_
         for (int j = 0; j < 4; ++j) {
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const int64_t n = n0 + wn + 16 * j + 4 * lg + v;
+                const int64_t n = n0 + wn + 16 * j + lg + 4 * v;
                 if (interior || (m < P.m && n < P.n)) {
                     __attribute__((address_space(1))) double* cp =
                         (__attribute__((address_space(1))) double*)(P.c + m + n * P.ldc);

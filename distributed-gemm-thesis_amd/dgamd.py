@@ -24,6 +24,9 @@ DG_SCHED_AUTO = 0
 DG_SCHED_REFERENCE = 1
 DG_SCHED_STANDARD = 2
 
+DG_OZAKI_INT8 = 0
+DG_OZAKI_BF16 = 1
+
 DG_DIST_UNIFORM_PM1 = 0
 DG_DIST_WIDE = 1
 DG_DIST_UNIFORM_01 = 2
@@ -59,6 +62,9 @@ SIGNATURES = {
                         _int, _ptr]),
     "dg_layer_reduce": (_int, [_ptr, _ptr, _i64, _int, _ptr]),
     "dg_generate": (_int, [_u64, _int, _ptr, _i64, _ptr]),
+    "dg_ozaki_workspace": (_i64, [_i64, _i64, _i64, _int, _int]),
+    "dg_ozaki_dgemm": (_int, [_chr, _chr, _i64, _i64, _i64, _dbl, _ptr, _i64, _ptr, _i64, _dbl, _ptr, _i64, _int,
+                              _int, _ptr, _i64, ctypes.POINTER(ctypes.c_double), _ptr]),
 }
 
 
@@ -200,3 +206,24 @@ def summa_t(opA, opB, M, N, K, alpha, Ap, Bp, beta, Cp, mb, nb, kb, P, Q):
 
 def layer_reduce_t(C, partial, n, nlayers):
     check("dg_layer_reduce", lib().dg_layer_reduce(_dp(C), _dp(partial), n, nlayers, _stream()))
+
+
+def ozaki_workspace(m, n, k, slices, kind):
+    return lib().dg_ozaki_workspace(m, n, k, slices, kind)
+
+
+def ozaki_dgemm_t(opA, opB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, slices, kind=DG_OZAKI_INT8,
+                  workspace=None):
+    """Runs dg_ozaki_dgemm; allocates the workspace with torch when none is
+    given.  Returns the a-priori error bound factor."""
+    import torch
+    need = ozaki_workspace(m, n, k, slices, kind)
+    if need < 0:
+        raise DGError("dg_ozaki_workspace", DG_ERR_INVALID, "bad arguments")
+    if workspace is None:
+        workspace = torch.empty(max(need, 1), dtype=torch.uint8, device=C.device)
+    bound = ctypes.c_double(0.0)
+    check("dg_ozaki_dgemm", lib().dg_ozaki_dgemm(_op(opA), _op(opB), m, n, k, alpha, _dp(A), lda, _dp(B), ldb, beta,
+                                                  _dp(C), ldc, slices, kind, _dp(workspace), workspace.numel(),
+                                                  ctypes.byref(bound), _stream()))
+    return bound.value

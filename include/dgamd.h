@@ -175,6 +175,30 @@ int dg_summa(char opA, char opB, int64_t M, int64_t N, int64_t K, double alpha,
  *           src/25DCannon.cpp:203-228; reduceResultNCCL, src/25DCannonNCCL.cpp:306-320. */
 int dg_layer_reduce(double* C, const double* partial, int64_t n, int nlayers, void* stream);
 
+/* ---------------------------------------------------- Ozaki split DGEMM -- */
+
+#define DG_OZAKI_INT8 0  /* 7-bit digits in int8, v_mfma_i32_16x16x64_i8, exact INT32 sums  */
+#define DG_OZAKI_BF16 1  /* 5-bit digits in bf16, v_mfma_f32_16x16x32_bf16, exact FP32 sums */
+
+/* Bytes of device workspace dg_ozaki_dgemm needs (-1 on bad arguments). */
+int64_t dg_ozaki_workspace(int64_t m, int64_t n, int64_t k, int slices, int kind);
+
+/* C = alpha*op(A)*op(B) + beta*C emulated with the Ozaki split: every row of
+ * op(A) and column of op(B) is scaled by a power of two and cut into
+ * `slices` digits (INT8 or BF16 kind); the s(s+1)/2 slice products with
+ * t + u <= s + 1 run on the low-precision matrix cores with exact 32-bit
+ * accumulation and are recombined in FP64 on the GPU.  `workspace` is
+ * caller-owned device memory of dg_ozaki_workspace() bytes, 256-byte
+ * aligned.  If err_bound is not NULL it receives k*(s+3)*2^(-b*s) (b = 7 for
+ * INT8, 5 for BF16): |C - op(A)op(B)|_ij <= err_bound * 2^(e_i + f_j) for the
+ * product part, e_i / f_j being the row / column scales.
+ * New relative to the reference (no split DGEMM exists there); it replaces
+ * the same cublasDgemm call sites as dg_dgemm (src/25DCannon.cpp:324-325). */
+int dg_ozaki_dgemm(char opA, char opB, int64_t m, int64_t n, int64_t k, double alpha,
+                   const double* A, int64_t lda, const double* B, int64_t ldb,
+                   double beta, double* C, int64_t ldc, int slices, int kind,
+                   void* workspace, int64_t workspace_bytes, double* err_bound, void* stream);
+
 /* ------------------------------------------------------------ generation -- */
 
 #define DG_DIST_UNIFORM_PM1 0  /* u in [-1, 1): 2*(x>>11)*2^-53 - 1                 */

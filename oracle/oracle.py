@@ -219,3 +219,103 @@ def generate(seed, dist, n):
     out = np.empty(n, dtype=np.float64)
     lib().or_generate(seed, dist, _p(out), n)
     return out
+
+
+# ---------------------------------------------------------------- Ozaki split DGEMM
+# Ozaki, Ogita, Oishi, Rump, "Error-free transformations of matrix
+# multiplication by using fast routines of matrix multiplication and its
+# applications", Numer. Algorithms 59 (2012); Ootomo, Ozaki, Yokota, "DGEMM on
+# integer matrix multiplication unit", IJHPCA (2024).  New relative to the
+# reference, which has no split DGEMM.
+
+OZ_INT8_BITS = 7     # |digit| <= 127: one int8 slice carries 7 bits below the row scale
+OZ_BF16_BITS = 5     # |digit| <= 31: bf16 holds it exactly and products sum exactly in fp32
+
+
+def oz_exponents(X, axis):
+    """Power-of-two scale per row (axis=1) or column (axis=0): the frexp
+    exponent e of max|x| (max|x| < 2^e), 0 for an all-zero line."""
+    amax = np.max(np.abs(X), axis=axis)
+    _, e = np.frexp(amax)
+    e = np.where(amax == 0, 0, e).astype(np.int32)
+    return e
+
+
+def oz_slices(X, e, axis, s, bits):
+    """Digits d_1..d_s with X ~= 2^e * sum_t d_t 2^(-bits*t), by truncation:
+    r_0 = X*2^-e, d_t = trunc(r_{t-1}*2^bits), r_t = r_{t-1}*2^bits - d_t
+    (every step exact in FP64).  Returns an (s, *X.shape) float64 array of
+    integers."""
+    shape = [1, 1]
+    shape[1 - axis] = -1
+    r = np.ldexp(X, -e.reshape(shape))
+    out = np.empty((s,) + X.shape)
+    scale = float(1 << bits)
+    for t in range(s):
+        r = r * scale
+        d = np.trunc(r)
+        out[t] = d
+        r = r - d
+    return out
+
+
+def oz_flush_every(K, bits):
+    """Slice products whose exact integer sums share one 32-bit accumulator."""
+    dmax = (1 << bits) - 1
+    if bits == OZ_BF16_BITS:
+        return max(0, (1 << 24) // (dmax * dmax * K)) if K else 1
+    return max(0, (2 ** 31 - 1) // (dmax * dmax * K)) if K else 1
+
+
+def oz_kchunk(K, bits):
+    """K chunk of one exact product: all of K for INT8 digits; for BF16 digits
+    the largest multiple of the 64-element k-step with 31^2 * chunk < 2^24."""
+    if bits != OZ_BF16_BITS:
+        return K if K else 1
+    dmax = (1 << bits) - 1
+    cap = (1 << 24) // (dmax * dmax)
+    kp = -(-K // 64) * 64
+    return kp if kp <= cap else cap // 64 * 64
+
+
+def ozaki_dgemm(a, b, s, alpha=1.0, beta=0.0, c=None, bits=OZ_INT8_BITS, kchunk=None):
+    """C = alpha * A B + beta * C through s slices per operand and the
+    triangular product set t + u <= s + 1, on 2-D float64 arrays a (M x K)
+    and b (K x N).  Accumulation order (shared with the GPU kernel, so the
+    results agree bit for bit): for level l = 2..s+1, for t = 1..l-1, add the
+    exact integer sum of A_t B_(l-t) to a 32-bit accumulator; every
+    `flush` products (and at the end of the level) add ldexp(acc, -bits*l)
+    into the FP64 total.  For the BF16 digits the K loop is also cut into
+    chunks of `kchunk`, each chunk flushed separately (fp32 sums stay exact
+    below 2^24)."""
+    M, K = a.shape
+    N = b.shape[1]
+    e = oz_exponents(a, 1)
+    f = oz_exponents(b, 0)
+    As = oz_slices(a, e, 1, s, bits)
+    Bs = oz_slices(b, f, 0, s, bits)
+    if kchunk is None:
+        kchunk = oz_kchunk(K, bits)
+    kchunk = max(1, min(kchunk, K if K else 1))
+    flush = oz_flush_every(min(K, kchunk), bits)
+    if flush < 1:
+        raise ValueError("K too large for exact 32-bit slice sums")
+    acc = np.zeros((M, N))
+    for l in range(2, s + 2):
+        for k0 in range(0, max(K, 1), kchunk):
+            k1 = min(K, k0 + kchunk)
+            S = np.zeros((M, N))
+            cnt = 0
+            for t in range(1, l):
+                u = l - t
+                S += As[t - 1][:, k0:k1] @ Bs[u - 1][k0:k1, :]   # exact: integers < 2^53
+                cnt += 1
+                if cnt == flush or t == l - 1:
+                    acc += np.ldexp(S, -bits * l)
+                    S[:] = 0.0
+                    cnt = 0
+    prod = np.ldexp(acc, e.reshape(-1, 1) + f.reshape(1, -1))
+    out = alpha * prod
+    if beta != 0.0:
+        out = out + beta * c
+    return out
