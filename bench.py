@@ -40,6 +40,9 @@ def parse():
     ap.add_argument("--workload", choices=sorted(WORKLOADS), default="summa")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="budget of the CPU baseline sample")
+    ap.add_argument("--ozaki-slices", default="5,6,7,8,9",
+                    help="INT8 slice counts of the Ozaki leg (empty: skip the leg)")
+    ap.add_argument("--ozaki-bf16-slices", default="11", help="BF16 slice counts of the Ozaki leg")
     return ap.parse_args()
 
 
@@ -109,6 +112,54 @@ def setup(workload, dev):
         dgamd.dgemm_t("N", "N", n, n, n, 1.0, A, n, B, n, 0.0, C, n)
 
     return step, (n, n, n), {"workload": WORKLOADS["dgemm"], "M": n, "N": n, "K": n}, "dgemm16384"
+
+
+INT8_PEAK = 5030.0   # TOPS, MI355X dense INT8 matrix
+BF16_PEAK = 2516.0   # TFLOP/s, MI355X dense BF16 matrix
+
+
+def ozaki_leg(dev, int8_slices, bf16_slices, reps=2):
+    """Ozaki split DGEMM at M=N=K=16384 on wide-range inputs u*2^e
+    (u uniform[-1,1), e uniform in [-20,20]): FP64-equivalent TFLOP/s
+    (2MNK/t), utilisation of the low-precision matrix peak counting the
+    s(s+1)/2 slice products, and relative Frobenius error against the
+    native FP64 MFMA DGEMM of the same inputs.  One call includes the
+    exponent scan, the slicing and the recombination."""
+    import torch
+    import dgamd
+    n = 16384
+    f64 = dict(dtype=torch.float64, device=dev)
+    A = torch.empty(n * n, **f64)
+    B = torch.empty(n * n, **f64)
+    dgamd.generate_t(4, dgamd.DG_DIST_WIDE, A)
+    dgamd.generate_t(5, dgamd.DG_DIST_WIDE, B)
+    ref = torch.empty(n * n, **f64)
+    dgamd.dgemm_t("N", "N", n, n, n, 1.0, A, n, B, n, 0.0, ref, n)
+    C = torch.empty(n * n, **f64)
+    flops = 2.0 * n ** 3
+    ref_norm = torch.linalg.vector_norm(ref).item()
+    out = {"config": "M=N=K=16384, A,B = u*2^e wide-range (seeds 4,5), alpha=1, beta=0; error vs native FP64 MFMA",
+           "int8": {}, "bf16": {}}
+    legs = [("int8", dgamd.DG_OZAKI_INT8, int8_slices, INT8_PEAK), ("bf16", dgamd.DG_OZAKI_BF16, bf16_slices, BF16_PEAK)]
+    for name, kind, slices, peak in legs:
+        for s in slices:
+            ws = torch.empty(dgamd.ozaki_workspace(n, n, n, s, kind), dtype=torch.uint8, device=dev)
+            dgamd.ozaki_dgemm_t("N", "N", n, n, n, 1.0, A, n, B, n, 0.0, C, n, s, kind, ws)  # warm-up
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(reps):
+                dgamd.ozaki_dgemm_t("N", "N", n, n, n, 1.0, A, n, B, n, 0.0, C, n, s, kind, ws)
+            e1.record()
+            torch.cuda.synchronize()
+            t = e0.elapsed_time(e1) / 1e3 / reps
+            err = torch.linalg.vector_norm(C - ref).item() / ref_norm
+            nprod = s * (s + 1) // 2
+            out[name][str(s)] = {"tflops_fp64_equiv": round(flops / t / 1e12, 2), "ms": round(t * 1e3, 2),
+                                 "products": nprod, "lowp_frac": round(nprod * flops / t / 1e12 / peak, 4),
+                                 "rel_err_vs_fp64": float("%.3e" % err)}
+            del ws
+    return out
 
 
 def cpu_baseline(budget_s):
@@ -198,6 +249,12 @@ def main():
         "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP64_PEAK, "unit": "TFLOP/s",
                      "frac": round(achieved / FP64_PEAK, 4), "traffic": load_traffic(tag)},
     }
+    int8_s = [int(x) for x in args.ozaki_slices.split(",") if x.strip()]
+    bf16_s = [int(x) for x in args.ozaki_bf16_slices.split(",") if x.strip()]
+    if int8_s or bf16_s:
+        del step
+        torch.cuda.empty_cache()
+        out["ozaki"] = ozaki_leg(dev, int8_s, bf16_s)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
     print(json.dumps(out), flush=True)

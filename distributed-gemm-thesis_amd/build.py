@@ -20,6 +20,8 @@ HOST = os.path.join(HERE, "host")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libdgamd.so")
 HOSTLIB = os.path.join(HERE, "libdghost.so")
+APPS = os.path.join(HERE, "apps")
+BIN = os.path.join(HERE, "bin")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = [
@@ -28,6 +30,13 @@ HIP_FLAGS = [
     "-Wall", "-Wno-unused-function",
     "-I" + os.path.join(ROOT, "include"), "-I" + CSRC,
 ]
+
+
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
+HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-D__HIP_PLATFORM_AMD__",
+              "-I" + os.path.join(ROOT, "include"), "-I" + os.path.join(ROCM, "include")]
+HOST_LIBS = ["-L" + HERE, "-ldgamd", "-L" + os.path.join(ROCM, "lib"), "-lamdhip64",
+             "-Wl,-rpath,$ORIGIN", "-Wl,-rpath," + os.path.join(ROCM, "lib")]
 
 
 def _run(cmd):
@@ -69,10 +78,17 @@ def build(verbose=False, jobs=None):
     os.replace(tmp, LIB)
     hsrcs = host_sources()
     if hsrcs:
+        # The host API is plain C++ over the HIP runtime: no device code.
         tmp = HOSTLIB + ".tmp"
-        _run([HIPCC, "-O2", "-std=c++17", "-fPIC", "-shared", "-I" + os.path.join(ROOT, "include"),
-              "-o", tmp] + hsrcs + ["-L" + HERE, "-ldgamd", "-Wl,-rpath,$ORIGIN"])
+        _run(["g++"] + HOST_FLAGS + ["-shared", "-o", tmp] + hsrcs + HOST_LIBS)
         os.replace(tmp, HOSTLIB)
+        # Command-line drivers and the host-API self-test.
+        os.makedirs(BIN, exist_ok=True)
+        for app in sorted(f for f in os.listdir(APPS) if f.endswith(".cpp")):
+            exe = os.path.join(BIN, app[:-4])
+            _run(["g++"] + HOST_FLAGS + ["-o", exe + ".tmp", os.path.join(APPS, app), "-L" + HERE, "-ldghost"]
+                 + HOST_LIBS + ["-Wl,-rpath,$ORIGIN/.."])
+            os.replace(exe + ".tmp", exe)
     if verbose:
         print("built", LIB)
     return LIB

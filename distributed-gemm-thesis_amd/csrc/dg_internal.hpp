@@ -78,10 +78,26 @@ struct Prob {
     int32_t pad_;
 };
 
+// Tile scheduling of the FP64 GEMM launch.
+enum {
+    DG_GEMM_MODE_ROUND_ROBIN = 0,  // tile = workgroup id (tiles dealt round-robin over XCDs)
+    DG_GEMM_MODE_XCD = 1,          // each XCD gets one contiguous tile range (L2 reuse)
+    DG_GEMM_MODE_PERSISTENT = 2,   // CU-count workgroups pull tiles from an atomic counter
+};
+
 // Launches the FP64 MFMA GEMM over `probs` (host vector; uploaded through
 // the table cache).  All problems share opA/opB.
 int launch_gemm_f64(char opA, char opB, const std::vector<Prob>& probs,
-                    const std::vector<Seg>& segs, hipStream_t stream);
+                    const std::vector<Seg>& segs, hipStream_t stream, int mode);
+
+// Host half of a launch: drops empty problems, assigns tile ranges and tells
+// whether every segment allows 128-bit loads.
+int prepare_gemm(const std::vector<Prob>& probs_in, const std::vector<Seg>& segs, std::vector<Prob>& probs,
+                 bool* vec, int* tiles);
+// Device half: tables already resident.  The persistent mode needs a device
+// int `counter`, which it zeroes with a kernel before the launch (graph safe).
+int launch_gemm_f64_dev(char opA, char opB, bool vec, int mode, const Prob* dp, int np, const Seg* ds,
+                        int tiles, int* counter, hipStream_t stream);
 
 // C[x] = beta*C[x] over an m x n column-major matrix (beta == 0 writes 0).
 int launch_scale(int64_t m, int64_t n, double beta, double* C, int64_t ldc, hipStream_t stream);

@@ -144,7 +144,7 @@ int dg_cannon25d(char opA, char opB, int64_t M, int64_t N, int64_t K, double alp
                 }
                 probs.push_back(pr);
             }
-    rc = launch_gemm_f64(oa, ob, probs, segs, st);
+    rc = launch_gemm_f64(oa, ob, probs, segs, st, DG_GEMM_MODE_XCD);
     if (rc) return rc;
     if (c > 1) return dg_layer_reduce(C_packed, workspace, M * N, c - 1, stream);
     return DG_OK;
@@ -222,7 +222,7 @@ int dg_summa(char opA, char opB, int64_t M, int64_t N, int64_t K, double alpha, 
             }
             probs.push_back(p);
         }
-    return launch_gemm_f64('N', 'N', probs, segs, st);
+    return launch_gemm_f64('N', 'N', probs, segs, st, DG_GEMM_MODE_XCD);
 }
 
 }  // extern "C"

@@ -153,13 +153,10 @@ __device__ __forceinline__ void load_b(dv2 (&r)[4], const Seg& s, int64_t n0, in
     else    load_kc<EDGE, VEC>(r, s.b, s.ldb, n0, N, k0, s.klen);
 }
 
+// Computes one output tile of the grouped problem set.
 template <bool TA, bool TB, bool VEC>
-__global__ __launch_bounds__(NT, 2) void gemm_f64_kernel(const Prob* __restrict__ probs, int nprob,
-                                                         const Seg* __restrict__ segs, int total_tiles) {
-    __shared__ double As[2][BM * LDK];
-    __shared__ double Bs[2][BN * LDK];
-
-    const int tile = xcd_remap(blockIdx.x, total_tiles);
+__device__ __forceinline__ void gemm_tile(const Prob* __restrict__ probs, int nprob, const Seg* __restrict__ segs,
+                                          int tile, double (&As)[2][BM * LDK], double (&Bs)[2][BN * LDK]) {
 
     // Problem owning this tile: last problem with tile0 <= tile.
     int lo = 0, hi = nprob - 1;
@@ -291,6 +288,35 @@ __global__ __launch_bounds__(NT, 2) void gemm_f64_kernel(const Prob* __restrict_
     }
 }
 
+// MODE 0: one workgroup per tile, tile = blockIdx.x (round-robin over XCDs);
+// MODE 1: one workgroup per tile, XCD-contiguous tile ranges (xcd_remap);
+// MODE 2: persistent: every workgroup takes tiles from `counter` (zeroed
+//         before the launch) until the tile space is exhausted.
+template <bool TA, bool TB, bool VEC, int MODE>
+__global__ __launch_bounds__(NT, 2) void gemm_f64_kernel(const Prob* __restrict__ probs, int nprob,
+                                                         const Seg* __restrict__ segs, int total_tiles,
+                                                         int* __restrict__ counter) {
+    __shared__ double As[2][BM * LDK];
+    __shared__ double Bs[2][BN * LDK];
+    if (MODE == 0) {
+        gemm_tile<TA, TB, VEC>(probs, nprob, segs, blockIdx.x, As, Bs);
+    } else if (MODE == 1) {
+        gemm_tile<TA, TB, VEC>(probs, nprob, segs, xcd_remap(blockIdx.x, total_tiles), As, Bs);
+    } else {
+        __shared__ int next;
+        while (true) {
+            __syncthreads();  // previous tile's LDS reads and `next` reads are complete
+            if (threadIdx.x == 0) next = atomicAdd(counter, 1);
+            __syncthreads();
+            const int tile = next;
+            if (tile >= total_tiles) break;
+            gemm_tile<TA, TB, VEC>(probs, nprob, segs, tile, As, Bs);
+        }
+    }
+}
+
+__global__ void zero_counter_kernel(int* counter) { *counter = 0; }
+
 __global__ void scale_kernel(int64_t m, int64_t n, double beta, double* __restrict__ C, int64_t ldc) {
     const int64_t total = m * n;
     for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x < total;
@@ -301,12 +327,22 @@ __global__ void scale_kernel(int64_t m, int64_t n, double beta, double* __restri
     }
 }
 
-template <bool TA, bool TB>
-void launch_kind(bool vec, const Prob* dp, int np, const Seg* ds, int tiles, hipStream_t st) {
-    if (vec)
-        hipLaunchKernelGGL((gemm_f64_kernel<TA, TB, true>), dim3(tiles), dim3(NT), 0, st, dp, np, ds, tiles);
+template <bool TA, bool TB, bool VEC>
+void launch_mode(int mode, const Prob* dp, int np, const Seg* ds, int tiles, int grid, int* counter,
+                 hipStream_t st) {
+    if (mode == DG_GEMM_MODE_ROUND_ROBIN)
+        hipLaunchKernelGGL((gemm_f64_kernel<TA, TB, VEC, 0>), dim3(tiles), dim3(NT), 0, st, dp, np, ds, tiles, counter);
+    else if (mode == DG_GEMM_MODE_XCD)
+        hipLaunchKernelGGL((gemm_f64_kernel<TA, TB, VEC, 1>), dim3(tiles), dim3(NT), 0, st, dp, np, ds, tiles, counter);
     else
-        hipLaunchKernelGGL((gemm_f64_kernel<TA, TB, false>), dim3(tiles), dim3(NT), 0, st, dp, np, ds, tiles);
+        hipLaunchKernelGGL((gemm_f64_kernel<TA, TB, VEC, 2>), dim3(grid), dim3(NT), 0, st, dp, np, ds, tiles, counter);
+}
+
+template <bool TA, bool TB>
+void launch_kind(bool vec, int mode, const Prob* dp, int np, const Seg* ds, int tiles, int grid, int* counter,
+                 hipStream_t st) {
+    if (vec) launch_mode<TA, TB, true>(mode, dp, np, ds, tiles, grid, counter, st);
+    else     launch_mode<TA, TB, false>(mode, dp, np, ds, tiles, grid, counter, st);
 }
 
 }  // namespace
@@ -321,13 +357,12 @@ int launch_scale(int64_t m, int64_t n, double beta, double* C, int64_t ldc, hipS
     return DG_OK;
 }
 
-int launch_gemm_f64(char opA, char opB, const std::vector<Prob>& probs_in, const std::vector<Seg>& segs,
-                    hipStream_t stream) {
-    // Assign tile ranges; drop problems with nothing to compute.
-    std::vector<Prob> probs;
+int prepare_gemm(const std::vector<Prob>& probs_in, const std::vector<Seg>& segs, std::vector<Prob>& probs,
+                 bool* vec, int* tiles_out) {
+    probs.clear();
     probs.reserve(probs_in.size());
     int64_t tiles = 0;
-    bool vec = true;
+    bool v = true;
     for (const Prob& p0 : probs_in) {
         if (p0.m <= 0 || p0.n <= 0 || p0.nseg <= 0) continue;
         Prob p = p0;
@@ -335,31 +370,61 @@ int launch_gemm_f64(char opA, char opB, const std::vector<Prob>& probs_in, const
         p.tiles_n = (int32_t)((p.n + BN - 1) / BN);
         p.tile0 = (int32_t)tiles;
         tiles += (int64_t)p.tiles_m * p.tiles_n;
+        if (tiles > INT32_MAX) return fail(DG_ERR_INVALID, "gemm: %lld output tiles exceed one launch", (long long)tiles);
         probs.push_back(p);
         for (int s = p.seg0; s < p.seg0 + p.nseg; ++s) {
             const Seg& g = segs[s];
-            if ((reinterpret_cast<uintptr_t>(g.a) & 15) || (reinterpret_cast<uintptr_t>(g.b) & 15) ||
-                (g.lda & 1) || (g.ldb & 1))
-                vec = false;
+            if ((reinterpret_cast<uintptr_t>(g.a) & 15) || (reinterpret_cast<uintptr_t>(g.b) & 15) || (g.lda & 1) ||
+                (g.ldb & 1))
+                v = false;
         }
     }
-    if (probs.empty()) return DG_OK;
-    if (tiles > INT32_MAX) return fail(DG_ERR_INVALID, "gemm: %lld output tiles exceed one launch", (long long)tiles);
+    *vec = v;
+    *tiles_out = (int)tiles;
+    return DG_OK;
+}
 
+int persistent_grid() {
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+        cus = 256;
+    return cus * 2;  // two 73.7 KB-LDS workgroups fit one CU
+}
+
+int launch_gemm_f64_dev(char opA, char opB, bool vec, int mode, const Prob* dp, int np, const Seg* ds, int tiles,
+                        int* counter, hipStream_t stream) {
+    if (tiles <= 0) return DG_OK;
+    int grid = 0;
+    if (mode == DG_GEMM_MODE_PERSISTENT) {
+        if (!counter) return fail(DG_ERR_INVALID, "gemm: persistent launch without a work counter");
+        grid = persistent_grid();
+        if (grid > tiles) grid = tiles;
+        hipLaunchKernelGGL(zero_counter_kernel, dim3(1), dim3(1), 0, stream, counter);
+    }
+    const bool ta = opA == 'T', tb = opB == 'T';
+    if (!ta && !tb) launch_kind<false, false>(vec, mode, dp, np, ds, tiles, grid, counter, stream);
+    else if (!ta && tb) launch_kind<false, true>(vec, mode, dp, np, ds, tiles, grid, counter, stream);
+    else if (ta && !tb) launch_kind<true, false>(vec, mode, dp, np, ds, tiles, grid, counter, stream);
+    else launch_kind<true, true>(vec, mode, dp, np, ds, tiles, grid, counter, stream);
+    DG_LAUNCH_CHECK();
+    return DG_OK;
+}
+
+int launch_gemm_f64(char opA, char opB, const std::vector<Prob>& probs_in, const std::vector<Seg>& segs,
+                    hipStream_t stream, int mode) {
+    std::vector<Prob> probs;
+    bool vec = true;
+    int tiles = 0;
+    int rc = prepare_gemm(probs_in, segs, probs, &vec, &tiles);
+    if (rc) return rc;
+    if (probs.empty()) return DG_OK;
     const Prob* dp = static_cast<const Prob*>(cached_table(probs.data(), probs.size() * sizeof(Prob), stream));
     if (!dp) return DG_ERR_ALLOC;
     const Seg* ds = static_cast<const Seg*>(cached_table(segs.data(), segs.size() * sizeof(Seg), stream));
     if (!ds) return DG_ERR_ALLOC;
-
-    const int np = (int)probs.size();
-    const int nt = (int)tiles;
-    const bool ta = opA == 'T', tb = opB == 'T';
-    if (!ta && !tb) launch_kind<false, false>(vec, dp, np, ds, nt, stream);
-    else if (!ta && tb) launch_kind<false, true>(vec, dp, np, ds, nt, stream);
-    else if (ta && !tb) launch_kind<true, false>(vec, dp, np, ds, nt, stream);
-    else launch_kind<true, true>(vec, dp, np, ds, nt, stream);
-    DG_LAUNCH_CHECK();
-    return DG_OK;
+    if (mode == DG_GEMM_MODE_PERSISTENT) mode = DG_GEMM_MODE_XCD;  // needs a counter: plans only
+    return launch_gemm_f64_dev(opA, opB, vec, mode, dp, (int)probs.size(), ds, tiles, nullptr, stream);
 }
 
 }  // namespace dg
@@ -387,5 +452,5 @@ extern "C" int dg_dgemm(char opA, char opB, int64_t m, int64_t n, int64_t k, dou
     Prob p{};
     p.m = m; p.n = n; p.c = C; p.ldc = ldc; p.alpha = alpha; p.beta = beta; p.seg0 = 0; p.nseg = 1;
     std::vector<Prob> probs{p};
-    return launch_gemm_f64(oa, ob, probs, segs, st);
+    return launch_gemm_f64(oa, ob, probs, segs, st, DG_GEMM_MODE_XCD);
 }

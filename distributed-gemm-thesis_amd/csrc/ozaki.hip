@@ -358,7 +358,9 @@ Plan make_plan(int64_t m, int64_t n, int64_t k, int s, bool bf16) {
     p.Kp = round_up(k > 0 ? k : 1, KSB / p.esz);
     p.a_slice = p.Mp * p.Kp * p.esz;
     p.b_slice = p.Np * p.Kp * p.esz;
-    p.nchunk_a = (k + kMaxChunk - 1) / kMaxChunk;
+    p.kchunk_max = kMaxChunk;
+    while ((k + p.kchunk_max - 1) / p.kchunk_max > 4096) p.kchunk_max *= 2;  // keep grid.y small
+    p.nchunk_a = (k + p.kchunk_max - 1) / p.kchunk_max;
     if (p.nchunk_a < 1) p.nchunk_a = 1;
     p.nchunk_b = p.nchunk_a;
     size_t off = 0;
@@ -470,13 +472,13 @@ int dg_ozaki_dgemm(char opA, char opB, int64_t m, int64_t n, int64_t k, double a
     // Row exponents of op(A), column exponents of op(B).
     {
         unsigned gx = a_ks == 1 ? (unsigned)(m < 65535 ? m : 65535) : (unsigned)((m + NT - 1) / NT < 65535 ? (m + NT - 1) / NT : 65535);
-        hipLaunchKernelGGL(oz_partial_max_kernel, dim3(gx, gy), dim3(NT), 0, st, A, m, k, a_rs, a_ks, kMaxChunk, part);
+        hipLaunchKernelGGL(oz_partial_max_kernel, dim3(gx, gy), dim3(NT), 0, st, A, m, k, a_rs, a_ks, p.kchunk_max, part);
         DG_LAUNCH_CHECK();
         hipLaunchKernelGGL(oz_exponent_kernel, dim3((unsigned)((p.Mp + NT - 1) / NT)), dim3(NT), 0, st, part, m, p.Mp,
                            nchunk, ea);
         DG_LAUNCH_CHECK();
         gx = b_ks == 1 ? (unsigned)(n < 65535 ? n : 65535) : (unsigned)((n + NT - 1) / NT < 65535 ? (n + NT - 1) / NT : 65535);
-        hipLaunchKernelGGL(oz_partial_max_kernel, dim3(gx, gy), dim3(NT), 0, st, B, n, k, b_rs, b_ks, kMaxChunk, part);
+        hipLaunchKernelGGL(oz_partial_max_kernel, dim3(gx, gy), dim3(NT), 0, st, B, n, k, b_rs, b_ks, p.kchunk_max, part);
         DG_LAUNCH_CHECK();
         hipLaunchKernelGGL(oz_exponent_kernel, dim3((unsigned)((p.Np + NT - 1) / NT)), dim3(NT), 0, st, part, n, p.Np,
                            nchunk, eb);

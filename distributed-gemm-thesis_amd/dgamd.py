@@ -39,6 +39,23 @@ _chr = ctypes.c_char
 _u64 = ctypes.c_uint64
 _i64p = ctypes.POINTER(ctypes.c_int64)
 
+DG_PLAN_AUTOTUNE = 0
+DG_PLAN_NONPERSISTENT = 1
+DG_PLAN_PERSISTENT = 2
+
+
+class GemmDesc(ctypes.Structure):
+    """struct dg_gemm_desc of include/dgamd.h."""
+    _fields_ = [("opA", ctypes.c_char), ("opB", ctypes.c_char), ("reserved", ctypes.c_char * 6),
+                ("m", ctypes.c_int64), ("n", ctypes.c_int64), ("k", ctypes.c_int64),
+                ("alpha", ctypes.c_double), ("beta", ctypes.c_double),
+                ("A", ctypes.c_void_p), ("lda", ctypes.c_int64),
+                ("B", ctypes.c_void_p), ("ldb", ctypes.c_int64),
+                ("C", ctypes.c_void_p), ("ldc", ctypes.c_int64)]
+
+
+_descp = ctypes.POINTER(GemmDesc)
+
 # name -> (restype, argtypes); every symbol declared in include/dgamd.h.
 SIGNATURES = {
     "dg_last_error": (ctypes.c_char_p, []),
@@ -62,6 +79,12 @@ SIGNATURES = {
                         _int, _ptr]),
     "dg_layer_reduce": (_int, [_ptr, _ptr, _i64, _int, _ptr]),
     "dg_generate": (_int, [_u64, _int, _ptr, _i64, _ptr]),
+    "dg_grouped_dgemm": (_int, [_descp, _int, _ptr]),
+    "dg_grouped_plan_create": (_int, [_descp, _int, _int, ctypes.POINTER(ctypes.c_void_p)]),
+    "dg_grouped_plan_execute": (_int, [_ptr, _ptr]),
+    "dg_grouped_plan_graph_launch": (_int, [_ptr, _ptr]),
+    "dg_grouped_plan_variant": (_int, [_ptr]),
+    "dg_grouped_plan_destroy": (_int, [_ptr]),
     "dg_ozaki_workspace": (_i64, [_i64, _i64, _i64, _int, _int]),
     "dg_ozaki_dgemm": (_int, [_chr, _chr, _i64, _i64, _i64, _dbl, _ptr, _i64, _ptr, _i64, _dbl, _ptr, _i64, _int,
                               _int, _ptr, _i64, ctypes.POINTER(ctypes.c_double), _ptr]),
@@ -227,3 +250,49 @@ def ozaki_dgemm_t(opA, opB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, slices
                                                   _dp(C), ldc, slices, kind, _dp(workspace), workspace.numel(),
                                                   ctypes.byref(bound), _stream()))
     return bound.value
+
+
+def gemm_desc(opA, opB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc):
+    """A GemmDesc over torch tensors (or raw device addresses as ints)."""
+    addr = lambda t: t if isinstance(t, int) or t is None else t.data_ptr()
+    return GemmDesc(_op(opA), _op(opB), b"\0" * 6, m, n, k, alpha, beta, addr(A), lda, addr(B), ldb, addr(C), ldc)
+
+
+def _desc_array(descs):
+    arr = (GemmDesc * len(descs))(*descs)
+    return arr
+
+
+def grouped_dgemm_t(descs):
+    arr = _desc_array(descs)
+    check("dg_grouped_dgemm", lib().dg_grouped_dgemm(arr, len(descs), _stream()))
+
+
+class GroupedPlan:
+    """dg_grouped_plan over a list of GemmDesc; destroyed with close()."""
+
+    def __init__(self, descs, mode=DG_PLAN_AUTOTUNE):
+        arr = _desc_array(descs)
+        h = ctypes.c_void_p()
+        check("dg_grouped_plan_create", lib().dg_grouped_plan_create(arr, len(descs), mode, ctypes.byref(h)))
+        self.handle = h
+
+    def execute(self):
+        check("dg_grouped_plan_execute", lib().dg_grouped_plan_execute(self.handle, _stream()))
+
+    def graph_launch(self):
+        check("dg_grouped_plan_graph_launch", lib().dg_grouped_plan_graph_launch(self.handle, _stream()))
+
+    def variant(self):
+        return lib().dg_grouped_plan_variant(self.handle)
+
+    def close(self):
+        if self.handle:
+            lib().dg_grouped_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -175,6 +175,52 @@ int dg_summa(char opA, char opB, int64_t M, int64_t N, int64_t K, double alpha,
  *           src/25DCannon.cpp:203-228; reduceResultNCCL, src/25DCannonNCCL.cpp:306-320. */
 int dg_layer_reduce(double* C, const double* partial, int64_t n, int nlayers, void* stream);
 
+/* ------------------------------------------------------- grouped batch -- */
+
+/* One GEMM of a grouped batch: C = alpha*op(A)*op(B) + beta*C. */
+typedef struct dg_gemm_desc {
+    char opA, opB;     /* 'N'/'T'/'C' as in dg_dgemm */
+    char reserved[6];
+    int64_t m, n, k;
+    double alpha, beta;
+    const double* A;
+    int64_t lda;
+    const double* B;
+    int64_t ldb;
+    double* C;
+    int64_t ldc;
+} dg_gemm_desc;
+
+/* Every GEMM of `descs` in one launch per distinct (opA, opB) pair: the
+ * tiles of all problems are dealt from one tile space, largest K first.
+ * Output ranges must not overlap.
+ * Replaces the per-rank local GEMMs the reference runs one rank at a time
+ * on BlockSequentialDecomposer tiles (PARALiADgemm per rank,
+ * src/distributedParalia.cpp:57-64, 159-167) and the per-step cublasDgemm
+ * calls of src/25DCannon.cpp:324-357. */
+int dg_grouped_dgemm(const dg_gemm_desc* descs, int count, void* stream);
+
+/* A grouped batch prepared once and executed many times: descriptor tables
+ * are uploaded at creation, the tile shape / scheduling variant is chosen by
+ * the autotuner (timed on first use, cached per shape class), and
+ * dg_grouped_plan_graph_launch captures the launch into a HIP graph on its
+ * first call and replays it afterwards.  The operand pointers are fixed at
+ * creation.  New relative to the reference (no batching exists there). */
+typedef struct dg_grouped_plan dg_grouped_plan;
+
+#define DG_PLAN_AUTOTUNE 0       /* time the variants on first execution     */
+#define DG_PLAN_NONPERSISTENT 1  /* one workgroup per output tile           */
+#define DG_PLAN_PERSISTENT 2     /* one workgroup per CU slot, tiles taken from
+                                    an atomic work counter, largest K first */
+
+int dg_grouped_plan_create(const dg_gemm_desc* descs, int count, int mode, dg_grouped_plan** plan);
+int dg_grouped_plan_execute(dg_grouped_plan* plan, void* stream);
+int dg_grouped_plan_graph_launch(dg_grouped_plan* plan, void* stream);
+/* Variant the plan runs (DG_PLAN_NONPERSISTENT or DG_PLAN_PERSISTENT; the
+ * autotuner decides on the first execution when created with AUTOTUNE). */
+int dg_grouped_plan_variant(const dg_grouped_plan* plan);
+int dg_grouped_plan_destroy(dg_grouped_plan* plan);
+
 /* ---------------------------------------------------- Ozaki split DGEMM -- */
 
 #define DG_OZAKI_INT8 0  /* 7-bit digits in int8, v_mfma_i32_16x16x64_i8, exact INT32 sums  */
