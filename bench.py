@@ -43,6 +43,8 @@ def parse():
     ap.add_argument("--ozaki-slices", default="5,6,7,8,9",
                     help="INT8 slice counts of the Ozaki leg (empty: skip the leg)")
     ap.add_argument("--ozaki-bf16-slices", default="11", help="BF16 slice counts of the Ozaki leg")
+    ap.add_argument("--ozaki-crt-moduli", default="13,14,15,16",
+                    help="moduli counts of the Ozaki scheme II (INT8, CRT) leg")
     return ap.parse_args()
 
 
@@ -118,7 +120,7 @@ INT8_PEAK = 5030.0   # TOPS, MI355X dense INT8 matrix
 BF16_PEAK = 2516.0   # TFLOP/s, MI355X dense BF16 matrix
 
 
-def ozaki_leg(dev, int8_slices, bf16_slices, reps=2):
+def ozaki_leg(dev, int8_slices, bf16_slices, crt_moduli=(), reps=2):
     """Ozaki split DGEMM at M=N=K=16384 on wide-range inputs u*2^e
     (u uniform[-1,1), e uniform in [-20,20]): FP64-equivalent TFLOP/s
     (2MNK/t), utilisation of the low-precision matrix peak counting the
@@ -139,8 +141,9 @@ def ozaki_leg(dev, int8_slices, bf16_slices, reps=2):
     flops = 2.0 * n ** 3
     ref_norm = torch.linalg.vector_norm(ref).item()
     out = {"config": "M=N=K=16384, A,B = u*2^e wide-range (seeds 4,5), alpha=1, beta=0; error vs native FP64 MFMA",
-           "int8": {}, "bf16": {}}
-    legs = [("int8", dgamd.DG_OZAKI_INT8, int8_slices, INT8_PEAK), ("bf16", dgamd.DG_OZAKI_BF16, bf16_slices, BF16_PEAK)]
+           "int8": {}, "bf16": {}, "int8_crt": {}}
+    legs = [("int8", dgamd.DG_OZAKI_INT8, int8_slices, INT8_PEAK), ("bf16", dgamd.DG_OZAKI_BF16, bf16_slices, BF16_PEAK),
+            ("int8_crt", dgamd.DG_OZAKI_INT8_CRT, crt_moduli, INT8_PEAK)]
     for name, kind, slices, peak in legs:
         for s in slices:
             ws = torch.empty(dgamd.ozaki_workspace(n, n, n, s, kind), dtype=torch.uint8, device=dev)
@@ -154,7 +157,7 @@ def ozaki_leg(dev, int8_slices, bf16_slices, reps=2):
             torch.cuda.synchronize()
             t = e0.elapsed_time(e1) / 1e3 / reps
             err = torch.linalg.vector_norm(C - ref).item() / ref_norm
-            nprod = s * (s + 1) // 2
+            nprod = s if kind == dgamd.DG_OZAKI_INT8_CRT else s * (s + 1) // 2
             out[name][str(s)] = {"tflops_fp64_equiv": round(flops / t / 1e12, 2), "ms": round(t * 1e3, 2),
                                  "products": nprod, "lowp_frac": round(nprod * flops / t / 1e12 / peak, 4),
                                  "rel_err_vs_fp64": float("%.3e" % err)}
@@ -251,10 +254,11 @@ def main():
     }
     int8_s = [int(x) for x in args.ozaki_slices.split(",") if x.strip()]
     bf16_s = [int(x) for x in args.ozaki_bf16_slices.split(",") if x.strip()]
-    if int8_s or bf16_s:
+    crt_m = [int(x) for x in args.ozaki_crt_moduli.split(",") if x.strip()]
+    if int8_s or bf16_s or crt_m:
         del step
         torch.cuda.empty_cache()
-        out["ozaki"] = ozaki_leg(dev, int8_s, bf16_s)
+        out["ozaki"] = ozaki_leg(dev, int8_s, bf16_s, crt_m)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
     print(json.dumps(out), flush=True)

@@ -13,17 +13,21 @@
 //   3. For level l = 2..s+1 and t = 1..l-1 the exact integer product
 //      A_t B_{l-t} is accumulated on the matrix cores (INT8 -> INT32,
 //      BF16 -> FP32 below 2^24, both exact); the 32-bit accumulator is added
-//      into an FP64 register accumulator as ldexp(acc, -b*l) whenever it
-//      could overflow next (the host computes the flush points) and at the
-//      end of each level.
-//   4. C = alpha * ldexp(acc, e_i + f_j) + beta * C, without fused multiply-add
+//      into an FP64 partial sum W as ldexp(acc, -b*l) whenever it could
+//      overflow next (the host computes the flush points) and at the end of
+//      each level.
+//   4. C = alpha * ldexp(W, e_i + f_j) + beta * C, without fused multiply-add
 //      so that the CPU oracle reproduces every bit.
 //
-// Slices are stored k-contiguous, [t][row][Kp] with rows padded to 128 and K
+// Slices are stored k-contiguous, [t][row][Kp] with rows padded to 256 and K
 // padded to one 128-byte k-step with zeros, so the GEMM kernel has no edge
-// cases until the final store.  The GEMM kernel uses 128x128 output tiles,
-// 4 waves of 64x64, 128-byte k-steps double buffered in LDS with a 144-byte
-// row stride (conflict-free 128-bit fragment reads).
+// cases until the final store.  The GEMM kernel uses 256x256 output tiles,
+// 8 waves of 128x64 (8x4 MFMA tiles of 16x16, 128 accumulator registers per
+// lane), 128-byte k-steps double buffered in LDS (144 KB) with a 144-byte row
+// stride (conflict-free 128-bit fragment reads).  The FP64 partial sums W
+// live in HBM (workspace): a flush is one coalesced read-modify-write of the
+// wave's 128x128 block, a handful per tile, so the registers hold only the
+// 32-bit accumulators.
 #include "dg_internal.hpp"
 
 // Every FP64 operation of the recombination is rounded on its own (no fused
@@ -39,9 +43,15 @@ typedef float fv4 __attribute__((ext_vector_type(4)));
 typedef double dv4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 
-constexpr int OT = 128;          // output tile (rows and columns)
+constexpr int OT = 256;          // output tile (rows and columns)
+constexpr int WTM = 128;         // wave tile rows
+constexpr int WTN = 64;          // wave tile columns
+constexpr int FRM = WTM / 16;    // 16x16 MFMA tiles per wave, rows
+constexpr int FRN = WTN / 16;    // 16x16 MFMA tiles per wave, columns
+constexpr int NTG = 512;         // threads of the slice-product kernel (8 waves, 2 x 4)
 constexpr int KSB = 128;         // k-step in bytes
 constexpr int LDB = KSB + 16;    // LDS row stride in bytes
+constexpr int LQ = OT * (KSB / 16) / NTG;  // 16-byte chunks per thread per operand and k-step
 constexpr int NT = 256;
 constexpr int GROUP_M = 8;
 
@@ -61,12 +71,14 @@ struct OzSeg {
     int32_t t, u;       // 0-based slice indices
     int32_t k0, klen;   // in elements, multiples of the k-step
     int32_t level;      // t + u + 2 (1-based level)
-    int32_t flush;      // add the accumulator into FP64 after this product
+    int32_t flush;      // 0: keep accumulating; 1: add into W; 2: first flush (W = x);
+                        // 3: last flush (C = alpha*ldexp(W + x, e+f) + beta*C); 4: first and last
 };
 
 struct OzArgs {
     const int8_t* a;      // A slices, [s][Mp][Kp] elements
     const int8_t* b;      // B slices, [s][Np][Kp]
+    double* w;            // FP64 partial sums, Mp x Np column-major
     const int32_t* ea;    // row exponents (Mp)
     const int32_t* eb;    // column exponents (Np)
     const OzSeg* segs;
@@ -82,7 +94,7 @@ struct OzArgs {
 
 template <bool BF16>
 __device__ __forceinline__ void mma(iv4& acc_i, fv4& acc_f, const iv4& x, const iv4& y) {
-    if (BF16) {
+    if constexpr (BF16) {
         acc_f = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf8, x), __builtin_bit_cast(bf8, y), acc_f,
                                                         0, 0, 0);
     } else {
@@ -90,32 +102,77 @@ __device__ __forceinline__ void mma(iv4& acc_i, fv4& acc_f, const iv4& x, const 
     }
 }
 
-// Loads one 128-row x 128-byte k-step: thread t takes 16-byte chunk t%8 of
-// rows t/8 + 32*q.
-__device__ __forceinline__ void oz_load(iv4 (&r)[4], const int8_t* __restrict__ base, int64_t row_bytes,
+// Loads one 256-row x 128-byte k-step: thread t takes 16-byte chunk t%8 of
+// rows t/8 + 64*q.
+__device__ __forceinline__ void oz_load(iv4 (&r)[LQ], const int8_t* __restrict__ base, int64_t row_bytes,
                                         int64_t row0, int64_t kbyte0) {
     const int t = threadIdx.x;
     const int ch = t & 7;
     const int rr = t >> 3;
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const int8_t* p = base + (row0 + rr + 32 * q) * row_bytes + kbyte0 + ch * 16;
+    for (int q = 0; q < LQ; ++q) {
+        const int8_t* p = base + (row0 + rr + (NTG / 8) * q) * row_bytes + kbyte0 + ch * 16;
         r[q] = *(giv4*)p;
     }
 }
 
-__device__ __forceinline__ void oz_store(int8_t* __restrict__ s, const iv4 (&r)[4]) {
+__device__ __forceinline__ void oz_store(int8_t* __restrict__ s, const iv4 (&r)[LQ]) {
     const int t = threadIdx.x;
     const int ch = t & 7;
     const int rr = t >> 3;
 #pragma unroll
-    for (int q = 0; q < 4; ++q) *reinterpret_cast<iv4*>(s + (rr + 32 * q) * LDB + ch * 16) = r[q];
+    for (int q = 0; q < LQ; ++q) *reinterpret_cast<iv4*>(s + (rr + (NTG / 8) * q) * LDB + ch * 16) = r[q];
+}
+
+// Flush kinds: first flush writes W = x, later ones W += x, the last one
+// writes C = alpha * ldexp(W + x, e_i + f_j) + beta * C (FL_ONLY: the first
+// is also the last, C from x alone).  x = acc * 2^-(bits*level) is exact.
+enum { FL_FIRST = 0, FL_ADD = 1, FL_LAST = 2, FL_ONLY = 3 };
+
+// Register v of lane group lg holds C(mrow + 16i, ncol + 16j + v) with
+// mrow = m0 + wm + l%16, ncol = n0 + wn + 4*(l/16): the 16x16 i32/f32 result
+// layout D(4*(l/16) + v, l%16) of D = B^T A^T.
+template <int KIND, bool BF16, bool GUARD>
+__device__ __forceinline__ void oz_flush(const iv4 (&acci)[FRM][FRN], const fv4 (&accf)[FRM][FRN], const OzArgs& g,
+                                         double w, int64_t mrow, int64_t ncol) {
+#pragma unroll
+    for (int j = 0; j < FRN; ++j) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int64_t n = ncol + 16 * j + v;
+            double* wcol = g.w + n * g.Mp + mrow;
+            double* ccol = g.c + n * g.ldc + mrow;
+            const int ef = (KIND == FL_LAST || KIND == FL_ONLY) && (!GUARD || n < g.n) ? g.eb[n] : 0;
+#pragma unroll
+            for (int i = 0; i < FRM; ++i) {
+                double x;
+                if constexpr (BF16) x = (double)accf[i][j][v] * w;
+                else x = (double)acci[i][j][v] * w;
+                if constexpr (KIND == FL_FIRST) {
+                    wcol[16 * i] = x;
+                } else if constexpr (KIND == FL_ADD) {
+                    wcol[16 * i] = __dadd_rn(wcol[16 * i], x);
+                } else {
+                    if (!GUARD || (mrow + 16 * i < g.m && n < g.n)) {
+                        const double acc = KIND == FL_LAST ? __dadd_rn(wcol[16 * i], x) : x;
+                        double val = __dmul_rn(g.alpha, ldexp(acc, g.ea[mrow + 16 * i] + ef));
+                        if (g.beta != 0.0) val = __dadd_rn(val, __dmul_rn(g.beta, ccol[16 * i]));
+                        ccol[16 * i] = val;
+                    }
+                }
+            }
+            // one column group's read-modify-write at a time
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    }
 }
 
 template <bool BF16>
-__global__ __launch_bounds__(NT, 1) void oz_gemm_kernel(OzArgs g) {
-    __shared__ __attribute__((aligned(16))) int8_t As[2][OT * LDB];
-    __shared__ __attribute__((aligned(16))) int8_t Bs[2][OT * LDB];
+__global__ __launch_bounds__(NTG, 1) void oz_gemm_kernel(OzArgs g) {
+    // LDS: [A buffer 0][A buffer 1][B buffer 0][B buffer 1], OT rows of LDB bytes each.
+    extern __shared__ __attribute__((aligned(16))) int8_t lds[];
+    auto As = [&](int b) { return lds + b * (OT * LDB); };
+    auto Bs = [&](int b) { return lds + (2 + b) * (OT * LDB); };
 
     const int total = g.tiles_m * g.tiles_n;
     const int tile = xcd_remap(blockIdx.x, total);
@@ -129,38 +186,34 @@ __global__ __launch_bounds__(NT, 1) void oz_gemm_kernel(OzArgs g) {
 
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    const int wm = (wave >> 1) * 64;
-    const int wn = (wave & 1) * 64;
+    const int wm = (wave >> 2) * WTM;
+    const int wn = (wave & 3) * WTN;
     const int lr = lane & 15;
     const int lg = lane >> 4;
 
     const int esz = BF16 ? 2 : 1;
     const int64_t row_bytes = g.Kp * esz;
 
-    dv4 accd[4][4];
-    iv4 acci[4][4];
-    fv4 accf[4][4];
+    iv4 acci[FRM][FRN];
+    fv4 accf[FRM][FRN];
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < FRM; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            accd[i][j] = dv4{0.0, 0.0, 0.0, 0.0};
-            acci[i][j] = iv4{0, 0, 0, 0};
-            accf[i][j] = fv4{0.f, 0.f, 0.f, 0.f};
+        for (int j = 0; j < FRN; ++j) {
+            if constexpr (BF16) accf[i][j] = fv4{0.f, 0.f, 0.f, 0.f};
+            else acci[i][j] = iv4{0, 0, 0, 0};
         }
 
     int sidx = 0;
     OzSeg cur = g.segs[0];
     int64_t kb = 0;  // byte offset within the current segment
-    const int64_t seg_bytes0 = (int64_t)cur.klen * esz;
-    (void)seg_bytes0;
 
-    iv4 ra[4], rb[4];
+    iv4 ra[LQ], rb[LQ];
     oz_load(ra, g.a + cur.t * g.a_slice, row_bytes, m0, (int64_t)cur.k0 * esz);
     oz_load(rb, g.b + cur.u * g.b_slice, row_bytes, n0, (int64_t)cur.k0 * esz);
     int buf = 0;
-    oz_store(As[0], ra);
-    oz_store(Bs[0], rb);
+    oz_store(As(0), ra);
+    oz_store(Bs(0), rb);
     __syncthreads();
 
     while (true) {
@@ -178,76 +231,285 @@ __global__ __launch_bounds__(NT, 1) void oz_gemm_kernel(OzArgs g) {
             oz_load(rb, g.b + nxt.u * g.b_slice, row_bytes, n0, (int64_t)nxt.k0 * esz + nkb);
         }
 
-        const int8_t* as = As[buf];
-        const int8_t* bs = Bs[buf];
+        const int8_t* as = As(buf);
+        const int8_t* bs = Bs(buf);
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
-            iv4 af[4], bfr[4];
+            iv4 bfr[FRN];
 #pragma unroll
-            for (int i = 0; i < 4; ++i)
-                af[i] = *reinterpret_cast<const iv4*>(as + (wm + 16 * i + lr) * LDB + h * 64 + lg * 16);
-#pragma unroll
-            for (int j = 0; j < 4; ++j)
+            for (int j = 0; j < FRN; ++j)
                 bfr[j] = *reinterpret_cast<const iv4*>(bs + (wn + 16 * j + lr) * LDB + h * 64 + lg * 16);
 #pragma unroll
-            for (int i = 0; i < 4; ++i)
+            for (int i = 0; i < FRM; ++i) {
+                const iv4 af = *reinterpret_cast<const iv4*>(as + (wm + 16 * i + lr) * LDB + h * 64 + lg * 16);
 #pragma unroll
-                for (int j = 0; j < 4; ++j) mma<BF16>(acci[i][j], accf[i][j], bfr[j], af[i]);
+                for (int j = 0; j < FRN; ++j) mma<BF16>(acci[i][j], accf[i][j], bfr[j], af);
+            }
         }
 
-        // End of the current product: flush the 32-bit sums into FP64.
+        // End of the current product: flush the 32-bit sums into W (or C).
         const bool seg_end = (kb + KSB >= (int64_t)cur.klen * esz);
         if (seg_end && cur.flush) {
             const double w = ldexp(1.0, -g.bits * cur.level);
+            const int64_t mrow = m0 + wm + lr;
+            const int64_t ncol = n0 + wn + 4 * lg;
+            const bool full = (m0 + OT <= g.m) && (n0 + OT <= g.n);
+            switch (cur.flush) {
+                case 2: oz_flush<FL_FIRST, BF16, false>(acci, accf, g, w, mrow, ncol); break;
+                case 1: oz_flush<FL_ADD, BF16, false>(acci, accf, g, w, mrow, ncol); break;
+                case 3:
+                    if (full) oz_flush<FL_LAST, BF16, false>(acci, accf, g, w, mrow, ncol);
+                    else      oz_flush<FL_LAST, BF16, true>(acci, accf, g, w, mrow, ncol);
+                    break;
+                default:
+                    if (full) oz_flush<FL_ONLY, BF16, false>(acci, accf, g, w, mrow, ncol);
+                    else      oz_flush<FL_ONLY, BF16, true>(acci, accf, g, w, mrow, ncol);
+                    break;
+            }
 #pragma unroll
-            for (int i = 0; i < 4; ++i)
+            for (int i = 0; i < FRM; ++i)
 #pragma unroll
-                for (int j = 0; j < 4; ++j)
-#pragma unroll
-                    for (int v = 0; v < 4; ++v) {
-                        const double x = BF16 ? (double)accf[i][j][v] : (double)acci[i][j][v];
-                        accd[i][j][v] = __dadd_rn(accd[i][j][v], x * w);  // x*w exact (power of two)
-                    }
-#pragma unroll
-            for (int i = 0; i < 4; ++i)
-#pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    acci[i][j] = iv4{0, 0, 0, 0};
-                    accf[i][j] = fv4{0.f, 0.f, 0.f, 0.f};
+                for (int j = 0; j < FRN; ++j) {
+                    if constexpr (BF16) accf[i][j] = fv4{0.f, 0.f, 0.f, 0.f};
+                    else acci[i][j] = iv4{0, 0, 0, 0};
                 }
         }
 
         if (!has_next) break;
         buf ^= 1;
-        oz_store(As[buf], ra);
-        oz_store(Bs[buf], rb);
+        oz_store(As(buf), ra);
+        oz_store(Bs(buf), rb);
         __syncthreads();
         sidx = ns;
         kb = nkb;
         cur = nxt;
     }
+}
 
-    // Epilogue.  16x16 i32/f32 result layout: lane l, register v holds
-    // D(4*(l/16) + v, l%16); D = B^T A^T, so acc[i][j][v] is
-    // C(m0 + wm + 16i + lr, n0 + wn + 16j + 4lg + v).
+// ------------------------------------------------------------ Ozaki scheme II
+// Ozaki, Uchino, Imamura, "Ozaki Scheme II: A GEMM-oriented emulation of
+// floating-point matrix multiplication using an integer modular technique"
+// (2025).  A' = trunc(op(A)_ik * 2^(q - e_i)) and B' likewise are integers
+// below 2^q; V = A'B' is computed modulo N pairwise coprime moduli p_t <= 256
+// (one INT8 GEMM each, residues balanced into [-128, 127]) and rebuilt
+// exactly by Garner's algorithm with balanced mixed-radix digits; q is chosen
+// so that k * 4^q <= P/4 (P = prod p_t), hence |V| < P/4 is recovered
+// without ambiguity.  C = alpha * ldexp(double(V), e_i + f_j - 2q) + beta*C
+// with double(V) correctly rounded.
+
+constexpr int kMaxModuli = 16;
+// Pairwise coprime: 2^8, 3*5*17, 11*23, 13*19, 7*31 and primes.
+__constant__ int c_moduli[kMaxModuli] = {256, 255, 253, 251, 247, 241, 239, 233, 229, 227, 223, 217, 211, 199, 197, 193};
+constexpr int h_moduli[kMaxModuli] = {256, 255, 253, 251, 247, 241, 239, 233, 229, 227, 223, 217, 211, 199, 197, 193};
+
+struct CrtTables {
+    int32_t n;                          // moduli in use
+    int32_t rinv[kMaxModuli];           // R_t^{-1} mod p_t
+    int16_t rmod[kMaxModuli][kMaxModuli];  // R_t mod p_u
+    uint64_t rhi[kMaxModuli], rlo[kMaxModuli];  // R_t = prod_{u<t} p_u as 128-bit
+};
+
+// x mod p in [0, p) for an integer-valued double |x| < 2^53 (exact steps).
+__device__ __forceinline__ int mod_exact(double x, int p, double inv_p) {
+    const double q = floor(x * inv_p);
+    double r = fma(-q, (double)p, x);  // exact: q*p and the difference are integers below 2^53
+    if (r < 0) r += p;
+    if (r >= p) r -= p;
+    return (int)r;
+}
+
+// Balanced residues of A' = trunc(op(X)(row, k) * 2^(q - e_row)) for every
+// modulus, [t][row][Kp] int8 (zeros in the padding); 64x64 tiles through LDS
+// as oz_slice_kernel.
+__global__ void crt_residue_kernel(const double* __restrict__ x, int64_t rows, int64_t K, int64_t rs, int64_t ks,
+                                   const int32_t* __restrict__ e, int q, int nmod, int8_t* __restrict__ out,
+                                   int64_t Kp, int64_t slice_bytes) {
+    __shared__ double tile[64][65];
+    const int64_t r0 = (int64_t)blockIdx.y * 64;
+    const int64_t k0 = (int64_t)blockIdx.x * 64;
+    const int tx = threadIdx.x & 63;
+    const int ty = threadIdx.x >> 6;
+    if (ks == 1) {
+        for (int rr = ty; rr < 64; rr += 4) {
+            const int64_t r = r0 + rr, k = k0 + tx;
+            tile[rr][tx] = (r < rows && k < K) ? x[r * rs + k] : 0.0;
+        }
+    } else {
+        for (int kk = ty; kk < 64; kk += 4) {
+            const int64_t r = r0 + tx, k = k0 + kk;
+            tile[tx][kk] = (r < rows && k < K) ? x[r * rs + k * ks] : 0.0;
+        }
+    }
+    __syncthreads();
+    for (int rr = ty; rr < 64; rr += 4) {
+        const int64_t r = r0 + rr, k = k0 + tx;
+        const double a = (r < rows) ? trunc(ldexp(tile[rr][tx], q - e[r])) : 0.0;
+        for (int t = 0; t < nmod; ++t) {
+            const int p = c_moduli[t];
+            int v = mod_exact(a, p, 1.0 / p);
+            if (2 * v >= p) v -= p;
+            out[t * slice_bytes + r * Kp + k] = (int8_t)v;
+        }
+    }
+}
+
+struct CrtArgs {
+    const int8_t* a;     // residues of A', [N][Mp][Kp]
+    const int8_t* b;     // residues of B', [N][Np][Kp]
+    uint8_t* res;        // residues of V, [N][Np][Mp] (column-major per modulus)
+    int64_t Mp, Np, Kp;
+    int64_t a_slice, b_slice;
+    int32_t tiles_m, tiles_n, nmod;
+};
+
+// One INT8 GEMM per modulus in one launch: workgroup b computes tile
+// b % tiles of modulus b / tiles (256x256 tiles, 8 waves of 128x64, same
+// pipeline as oz_gemm_kernel) and stores (A'_t B'_t) mod p_t as bytes.
+__global__ __launch_bounds__(NTG, 1) void crt_gemm_kernel(CrtArgs g) {
+    extern __shared__ __attribute__((aligned(16))) int8_t lds[];
+    auto As = [&](int b) { return lds + b * (OT * LDB); };
+    auto Bs = [&](int b) { return lds + (2 + b) * (OT * LDB); };
+
+    const int tiles = g.tiles_m * g.tiles_n;
+    const int tmod = blockIdx.x / tiles;
+    const int tile = xcd_remap(blockIdx.x - tmod * tiles, tiles);
+    const int per_group = GROUP_M * g.tiles_n;
+    const int group = tile / per_group;
+    const int first_m = group * GROUP_M;
+    const int gsize = min(g.tiles_m - first_m, GROUP_M);
+    const int in_group = tile - group * per_group;
+    const int64_t m0 = (int64_t)(first_m + in_group % gsize) * OT;
+    const int64_t n0 = (int64_t)(in_group / gsize) * OT;
+
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int wm = (wave >> 2) * WTM;
+    const int wn = (wave & 3) * WTN;
+    const int lr = lane & 15;
+    const int lg = lane >> 4;
+
+    const int8_t* abase = g.a + tmod * g.a_slice;
+    const int8_t* bbase = g.b + tmod * g.b_slice;
+    iv4 acc[FRM][FRN];
+    fv4 unused[FRM][FRN];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        const int64_t m = m0 + wm + 16 * i + lr;
-        if (m >= g.m) continue;
-        const int ea = g.ea[m];
+    for (int i = 0; i < FRM; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
+        for (int j = 0; j < FRN; ++j) acc[i][j] = iv4{0, 0, 0, 0};
+
+    iv4 ra[LQ], rb[LQ];
+    oz_load(ra, abase, g.Kp, m0, 0);
+    oz_load(rb, bbase, g.Kp, n0, 0);
+    int buf = 0;
+    oz_store(As(0), ra);
+    oz_store(Bs(0), rb);
+    __syncthreads();
+    for (int64_t kb = 0; kb < g.Kp; kb += KSB) {
+        const bool has_next = kb + KSB < g.Kp;
+        if (has_next) {
+            oz_load(ra, abase, g.Kp, m0, kb + KSB);
+            oz_load(rb, bbase, g.Kp, n0, kb + KSB);
+        }
+        const int8_t* as = As(buf);
+        const int8_t* bs = Bs(buf);
 #pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int64_t n = n0 + wn + 16 * j + 4 * lg + v;
-                if (n < g.n) {
-                    double* cp = g.c + m + n * g.ldc;
-                    double val = __dmul_rn(g.alpha, ldexp(accd[i][j][v], ea + g.eb[n]));
-                    if (g.beta != 0.0) val = __dadd_rn(val, __dmul_rn(g.beta, *cp));
-                    *cp = val;
-                }
+        for (int h = 0; h < 2; ++h) {
+            iv4 bfr[FRN];
+#pragma unroll
+            for (int j = 0; j < FRN; ++j)
+                bfr[j] = *reinterpret_cast<const iv4*>(bs + (wn + 16 * j + lr) * LDB + h * 64 + lg * 16);
+#pragma unroll
+            for (int i = 0; i < FRM; ++i) {
+                const iv4 af = *reinterpret_cast<const iv4*>(as + (wm + 16 * i + lr) * LDB + h * 64 + lg * 16);
+#pragma unroll
+                for (int j = 0; j < FRN; ++j) mma<false>(acc[i][j], unused[i][j], bfr[j], af);
             }
         }
+        if (!has_next) break;
+        buf ^= 1;
+        oz_store(As(buf), ra);
+        oz_store(Bs(buf), rb);
+        __syncthreads();
+    }
+    // acc[i][j][v] = V_t(m0 + wm + 16i + lr, n0 + wn + 16j + 4lg + v) (see oz_flush)
+    const int p = c_moduli[tmod];
+    const double inv_p = 1.0 / p;
+    uint8_t* out = g.res + (int64_t)tmod * g.Mp * g.Np;
+#pragma unroll
+    for (int j = 0; j < FRN; ++j)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int64_t n = n0 + wn + 16 * j + 4 * lg + v;
+#pragma unroll
+            for (int i = 0; i < FRM; ++i) {
+                const int64_t m = m0 + wm + 16 * i + lr;
+                out[n * g.Mp + m] = (uint8_t)mod_exact((double)acc[i][j][v], p, inv_p);
+            }
+        }
+}
+
+// Garner's algorithm with balanced digits, then the exact 128-bit value and
+// its correctly rounded double; C = alpha * ldexp(V, e_i + f_j - 2q) + beta*C.
+__global__ void crt_reconstruct_kernel(const uint8_t* __restrict__ res, CrtTables tb, int64_t m, int64_t n,
+                                       int64_t Mp, int64_t Np, const int32_t* __restrict__ ea,
+                                       const int32_t* __restrict__ eb, int q, double alpha, double beta,
+                                       double* __restrict__ C, int64_t ldc) {
+    const int64_t total = m * n;
+    for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x < total; x += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t j = x / m, i = x - j * m;
+        // Fully unrolled over kMaxModuli so that the partial sums stay in registers.
+        int s[kMaxModuli];
+#pragma unroll
+        for (int u = 0; u < kMaxModuli; ++u) s[u] = 0;
+        uint64_t hi = 0, lo = 0;  // two's complement 128-bit accumulator
+#pragma unroll
+        for (int t = 0; t < kMaxModuli; ++t) {
+            if (t < tb.n) {
+                const int p = c_moduli[t];
+                int d = ((int)res[(int64_t)t * Mp * Np + j * Mp + i] - s[t]) * tb.rinv[t];
+                d %= p;
+                if (d < 0) d += p;
+                if (2 * d >= p) d -= p;  // balanced digit
+#pragma unroll
+                for (int u = t + 1; u < kMaxModuli; ++u) {
+                    if (u < tb.n) {
+                        const int w = (s[u] + d * tb.rmod[t][u]) % c_moduli[u];
+                        s[u] = w < 0 ? w + c_moduli[u] : w;
+                    }
+                }
+                // (hi, lo) += d * R_t
+                const uint64_t ad = (uint64_t)(d < 0 ? -d : d);
+                uint64_t plo = ad * tb.rlo[t];
+                uint64_t phi = __umul64hi(ad, tb.rlo[t]) + ad * tb.rhi[t];
+                if (d < 0) {  // negate the 128-bit product
+                    plo = ~plo + 1;
+                    phi = ~phi + (plo == 0 ? 1 : 0);
+                }
+                const uint64_t nlo = lo + plo;
+                hi = hi + phi + (nlo < lo ? 1 : 0);
+                lo = nlo;
+            }
+        }
+        // correctly rounded double of the signed 128-bit (hi, lo)
+        const bool neg = (int64_t)hi < 0;
+        if (neg) {
+            lo = ~lo + 1;
+            hi = ~hi + (lo == 0 ? 1 : 0);
+        }
+        double v;
+        if (hi == 0) {
+            v = (double)lo;
+        } else {
+            const int z = __clzll((long long)hi);
+            const uint64_t top = z ? (hi << z) | (lo >> (64 - z)) : hi;
+            const uint64_t rest = z ? (lo << z) : lo;
+            v = ldexp((double)(top | (rest != 0 ? 1ull : 0ull)), 64 - z);
+        }
+        if (neg) v = -v;
+        double val = __dmul_rn(alpha, ldexp(v, ea[i] + eb[j] - 2 * q));
+        double* cp = C + i + j * ldc;
+        if (beta != 0.0) val = __dadd_rn(val, __dmul_rn(beta, *cp));
+        *cp = val;
     }
 }
 
@@ -341,14 +603,73 @@ __global__ void oz_slice_kernel(const double* __restrict__ x, int64_t rows, int6
     }
 }
 
-struct Plan {
-    int64_t Mp, Np, Kp, esz, a_slice, b_slice, nchunk_a, nchunk_b, kchunk_max;
-    size_t off_a, off_b, off_ea, off_eb, off_part, total;
-};
-
 constexpr int64_t kMaxChunk = 1024;  // k-chunk of the max reduction
 
 int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
+
+struct Plan {
+    int64_t Mp, Np, Kp, esz, a_slice, b_slice, nchunk_a, nchunk_b, kchunk_max;
+    size_t off_a, off_b, off_ea, off_eb, off_part, off_w, total;
+};
+
+// Host side of the CRT form: q and the Garner tables for n moduli.
+int crt_setup(int64_t k, int nmod, CrtTables* tb, int* q_out) {
+    unsigned __int128 P = 1;
+    for (int t = 0; t < nmod; ++t) P *= (unsigned)h_moduli[t];
+    // largest q <= 53 with k * 4^q * 4 <= P
+    int q = 53;
+    while (q > 0) {
+        unsigned __int128 need = (unsigned __int128)(k > 0 ? k : 1) << (2 * q + 2);
+        if (2 * q + 2 < 127 && need <= P) break;
+        --q;
+    }
+    if (q < 1) return fail(DG_ERR_UNSUPPORTED, "dg_ozaki_dgemm: %d moduli leave no bits for k=%lld", nmod, (long long)k);
+    *q_out = q;
+    tb->n = nmod;
+    unsigned __int128 R = 1;
+    for (int t = 0; t < nmod; ++t) {
+        tb->rhi[t] = (uint64_t)(R >> 64);
+        tb->rlo[t] = (uint64_t)R;
+        for (int u = 0; u < nmod; ++u) tb->rmod[t][u] = (int16_t)(R % (unsigned)h_moduli[u]);
+        // inverse of R mod p_t by extended Euclid (R and p_t are coprime)
+        const int p = h_moduli[t];
+        int64_t a = (int64_t)(R % (unsigned)p), b = p, x0 = 1, x1 = 0;
+        while (b) {
+            const int64_t qq = a / b, tmp = a - qq * b;
+            a = b; b = tmp;
+            const int64_t tx = x0 - qq * x1;
+            x0 = x1; x1 = tx;
+        }
+        tb->rinv[t] = (int32_t)(((x0 % p) + p) % p);
+        R *= (unsigned)h_moduli[t];
+    }
+    return DG_OK;
+}
+
+Plan make_crt_plan(int64_t m, int64_t n, int64_t k, int nmod) {
+    Plan p{};
+    p.esz = 1;
+    p.Mp = round_up(m > 0 ? m : 1, OT);
+    p.Np = round_up(n > 0 ? n : 1, OT);
+    p.Kp = round_up(k > 0 ? k : 1, KSB);
+    p.a_slice = p.Mp * p.Kp;
+    p.b_slice = p.Np * p.Kp;
+    p.kchunk_max = kMaxChunk;
+    while ((k + p.kchunk_max - 1) / p.kchunk_max > 4096) p.kchunk_max *= 2;
+    p.nchunk_a = (k + p.kchunk_max - 1) / p.kchunk_max;
+    if (p.nchunk_a < 1) p.nchunk_a = 1;
+    p.nchunk_b = p.nchunk_a;
+    size_t off = 0;
+    auto take = [&](size_t bytes) { size_t o = off; off = (off + bytes + 255) / 256 * 256; return o; };
+    p.off_a = take((size_t)nmod * p.a_slice);
+    p.off_b = take((size_t)nmod * p.b_slice);
+    p.off_ea = take((size_t)p.Mp * 4);
+    p.off_eb = take((size_t)p.Np * 4);
+    p.off_part = take((size_t)p.nchunk_a * (size_t)(m > n ? m : n) * 8);
+    p.off_w = take((size_t)nmod * (size_t)p.Mp * (size_t)p.Np);  // residues of V
+    p.total = off;
+    return p;
+}
 
 Plan make_plan(int64_t m, int64_t n, int64_t k, int s, bool bf16) {
     Plan p{};
@@ -370,6 +691,7 @@ Plan make_plan(int64_t m, int64_t n, int64_t k, int s, bool bf16) {
     p.off_ea = take((size_t)p.Mp * 4);
     p.off_eb = take((size_t)p.Np * 4);
     p.off_part = take((size_t)p.nchunk_a * (size_t)(m > n ? m : n) * 8);
+    p.off_w = take((size_t)p.Mp * (size_t)p.Np * 8);
     p.total = off;
     return p;
 }
@@ -408,6 +730,15 @@ std::vector<OzSeg> make_schedule(int s, int bits, int64_t Kp, int64_t K, bool bf
                 segs.push_back(g);
             }
         }
+    // Mark the first and the last flush (W is written, not read, by the first;
+    // the last one writes C).
+    bool seen = false;
+    for (auto& g : segs)
+        if (g.flush) {
+            if (!seen) { g.flush = 2; seen = true; }
+        }
+    for (auto it = segs.rbegin(); it != segs.rend(); ++it)
+        if (it->flush) { it->flush = it->flush == 2 ? 4 : 3; break; }
     return segs;
 }
 
@@ -418,8 +749,12 @@ std::vector<OzSeg> make_schedule(int s, int bits, int64_t Kp, int64_t K, bool bf
 extern "C" {
 
 int64_t dg_ozaki_workspace(int64_t m, int64_t n, int64_t k, int slices, int kind) {
-    if (m < 0 || n < 0 || k < 0 || slices < 1 || slices > 16 || (kind != DG_OZAKI_INT8 && kind != DG_OZAKI_BF16))
-        return -1;
+    if (m < 0 || n < 0 || k < 0 || slices < 1 || slices > 16) return -1;
+    if (kind == DG_OZAKI_INT8_CRT) {
+        if (slices < 2) return -1;
+        return (int64_t)dg::make_crt_plan(m, n, k, slices).total;
+    }
+    if (kind != DG_OZAKI_INT8 && kind != DG_OZAKI_BF16) return -1;
     return (int64_t)dg::make_plan(m, n, k, slices, kind == DG_OZAKI_BF16).total;
 }
 
@@ -429,15 +764,28 @@ int dg_ozaki_dgemm(char opA, char opB, int64_t m, int64_t n, int64_t k, double a
     using namespace dg;
     const char oa = norm_op(opA), ob = norm_op(opB);
     if (!oa || !ob) return fail(DG_ERR_INVALID, "dg_ozaki_dgemm: invalid op '%c%c'", opA, opB);
-    if (kind != DG_OZAKI_INT8 && kind != DG_OZAKI_BF16) return fail(DG_ERR_INVALID, "dg_ozaki_dgemm: kind %d", kind);
+    if (kind != DG_OZAKI_INT8 && kind != DG_OZAKI_BF16 && kind != DG_OZAKI_INT8_CRT)
+        return fail(DG_ERR_INVALID, "dg_ozaki_dgemm: kind %d", kind);
     if (slices < 1 || slices > 16) return fail(DG_ERR_INVALID, "dg_ozaki_dgemm: slices %d (1..16)", slices);
+    const bool crt = kind == DG_OZAKI_INT8_CRT;
+    if (crt && slices < 2) return fail(DG_ERR_INVALID, "dg_ozaki_dgemm: the CRT form needs 2..16 moduli");
     if (m < 0 || n < 0 || k < 0) return fail(DG_ERR_INVALID, "dg_ozaki_dgemm: negative dimension");
     const int64_t rowsA = oa == 'N' ? m : k, rowsB = ob == 'N' ? k : n;
     if (lda < (rowsA > 1 ? rowsA : 1) || ldb < (rowsB > 1 ? rowsB : 1) || ldc < (m > 1 ? m : 1))
         return fail(DG_ERR_INVALID, "dg_ozaki_dgemm: leading dimension too small");
     const bool bf16 = kind == DG_OZAKI_BF16;
     const int bits = bf16 ? 5 : 7;
-    if (err_bound) *err_bound = (double)k * (slices + 3) * ldexp(1.0, -bits * slices);
+    CrtTables tables{};
+    int q = 0;
+    if (crt) {
+        if (k > 131071) return fail(DG_ERR_UNSUPPORTED, "dg_ozaki_dgemm: CRT form needs k < 131072 (exact int32 sums)");
+        int rc0 = crt_setup(k, slices, &tables, &q);
+        if (rc0) return rc0;
+        // |A - 2^(e-q) A'| < 2^(e-q) per entry: the product error is below k * 2^(1-q) * 2^(e_i+f_j)
+        if (err_bound) *err_bound = (double)k * ldexp(1.0, 1 - q);
+    } else if (err_bound) {
+        *err_bound = (double)k * (slices + 3) * ldexp(1.0, -bits * slices);
+    }
     if (m == 0 || n == 0) return DG_OK;
     int rc = require_device();
     if (rc) return rc;
@@ -445,7 +793,7 @@ int dg_ozaki_dgemm(char opA, char opB, int64_t m, int64_t n, int64_t k, double a
     if (k == 0 || alpha == 0.0) return launch_scale(m, n, beta, C, ldc, st);
     if (!A || !B || !C) return fail(DG_ERR_INVALID, "dg_ozaki_dgemm: null pointer");
     if (k > INT32_MAX / 2) return fail(DG_ERR_INVALID, "dg_ozaki_dgemm: k too large");
-    const Plan p = make_plan(m, n, k, slices, bf16);
+    const Plan p = crt ? make_crt_plan(m, n, k, slices) : make_plan(m, n, k, slices, bf16);
     if (p.Mp / 64 > 65535 || p.Np / 64 > 65535 || p.Kp / 64 > INT32_MAX)
         return fail(DG_ERR_UNSUPPORTED, "dg_ozaki_dgemm: matrix too large for one slicing launch");
     if (!workspace || workspace_bytes < (int64_t)p.total)
@@ -453,8 +801,12 @@ int dg_ozaki_dgemm(char opA, char opB, int64_t m, int64_t n, int64_t k, double a
     if (reinterpret_cast<uintptr_t>(workspace) & 255)
         return fail(DG_ERR_INVALID, "dg_ozaki_dgemm: workspace must be 256-byte aligned");
     int serr = 0;
-    std::vector<OzSeg> sched = make_schedule(slices, bits, p.Kp, k, bf16, &serr);
-    if (serr) return fail(DG_ERR_UNSUPPORTED, "dg_ozaki_dgemm: k=%lld too large for exact 32-bit slice sums", (long long)k);
+    std::vector<OzSeg> sched;
+    if (!crt) {
+        sched = make_schedule(slices, bits, p.Kp, k, bf16, &serr);
+        if (serr)
+            return fail(DG_ERR_UNSUPPORTED, "dg_ozaki_dgemm: k=%lld too large for exact 32-bit slice sums", (long long)k);
+    }
 
     char* ws = static_cast<char*>(workspace);
     int8_t* sa = reinterpret_cast<int8_t*>(ws + p.off_a);
@@ -484,6 +836,30 @@ int dg_ozaki_dgemm(char opA, char opB, int64_t m, int64_t n, int64_t k, double a
                            nchunk, eb);
         DG_LAUNCH_CHECK();
     }
+    const size_t lds_bytes = (size_t)4 * OT * LDB;  // 144 KB: two buffers of A and B
+    if (crt) {
+        dim3 ga((unsigned)(p.Kp / 64), (unsigned)(p.Mp / 64));
+        dim3 gb((unsigned)(p.Kp / 64), (unsigned)(p.Np / 64));
+        hipLaunchKernelGGL(crt_residue_kernel, ga, dim3(NT), 0, st, A, m, k, a_rs, a_ks, ea, q, slices, sa, p.Kp,
+                           p.a_slice);
+        hipLaunchKernelGGL(crt_residue_kernel, gb, dim3(NT), 0, st, B, n, k, b_rs, b_ks, eb, q, slices, sb, p.Kp,
+                           p.b_slice);
+        DG_LAUNCH_CHECK();
+        CrtArgs cg{};
+        cg.a = sa; cg.b = sb; cg.res = reinterpret_cast<uint8_t*>(ws + p.off_w);
+        cg.Mp = p.Mp; cg.Np = p.Np; cg.Kp = p.Kp; cg.a_slice = p.a_slice; cg.b_slice = p.b_slice;
+        cg.tiles_m = (int32_t)(p.Mp / OT); cg.tiles_n = (int32_t)(p.Np / OT); cg.nmod = slices;
+        const int64_t grid = (int64_t)cg.tiles_m * cg.tiles_n * slices;
+        if (grid > INT32_MAX) return fail(DG_ERR_INVALID, "dg_ozaki_dgemm: too many tiles");
+        hipLaunchKernelGGL(crt_gemm_kernel, dim3((unsigned)grid), dim3(NTG), lds_bytes, st, cg);
+        DG_LAUNCH_CHECK();
+        int64_t blocks = (m * n + 255) / 256;
+        if (blocks > 65536) blocks = 65536;
+        hipLaunchKernelGGL(crt_reconstruct_kernel, dim3((unsigned)blocks), dim3(256), 0, st, cg.res, tables, m, n, p.Mp,
+                           p.Np, ea, eb, q, alpha, beta, C, ldc);
+        DG_LAUNCH_CHECK();
+        return DG_OK;
+    }
     // Slices (the padding is written as zeros by the same kernels).
     {
         dim3 ga((unsigned)(p.Kp / 64), (unsigned)(p.Mp / 64));
@@ -504,15 +880,16 @@ int dg_ozaki_dgemm(char opA, char opB, int64_t m, int64_t n, int64_t k, double a
     const OzSeg* dsched = static_cast<const OzSeg*>(cached_table(sched.data(), sched.size() * sizeof(OzSeg), st));
     if (!dsched) return DG_ERR_ALLOC;
     OzArgs g{};
-    g.a = sa; g.b = sb; g.ea = ea; g.eb = eb; g.segs = dsched; g.nseg = (int32_t)sched.size(); g.bits = bits;
+    g.a = sa; g.b = sb; g.w = reinterpret_cast<double*>(ws + p.off_w);
+    g.ea = ea; g.eb = eb; g.segs = dsched; g.nseg = (int32_t)sched.size(); g.bits = bits;
     g.m = m; g.n = n; g.Mp = p.Mp; g.Np = p.Np; g.Kp = p.Kp; g.a_slice = p.a_slice; g.b_slice = p.b_slice;
     g.alpha = alpha; g.beta = beta; g.c = C; g.ldc = ldc;
     g.tiles_m = (int32_t)(p.Mp / OT);
     g.tiles_n = (int32_t)(p.Np / OT);
     const int64_t tiles = (int64_t)g.tiles_m * g.tiles_n;
     if (tiles > INT32_MAX) return fail(DG_ERR_INVALID, "dg_ozaki_dgemm: too many tiles");
-    if (bf16) hipLaunchKernelGGL(oz_gemm_kernel<true>, dim3((unsigned)tiles), dim3(NT), 0, st, g);
-    else      hipLaunchKernelGGL(oz_gemm_kernel<false>, dim3((unsigned)tiles), dim3(NT), 0, st, g);
+    if (bf16) hipLaunchKernelGGL(oz_gemm_kernel<true>, dim3((unsigned)tiles), dim3(NTG), lds_bytes, st, g);
+    else      hipLaunchKernelGGL(oz_gemm_kernel<false>, dim3((unsigned)tiles), dim3(NTG), lds_bytes, st, g);
     DG_LAUNCH_CHECK();
     return DG_OK;
 }

@@ -26,6 +26,7 @@ DG_SCHED_STANDARD = 2
 
 DG_OZAKI_INT8 = 0
 DG_OZAKI_BF16 = 1
+DG_OZAKI_INT8_CRT = 2
 
 DG_DIST_UNIFORM_PM1 = 0
 DG_DIST_WIDE = 1

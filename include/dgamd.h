@@ -223,8 +223,10 @@ int dg_grouped_plan_destroy(dg_grouped_plan* plan);
 
 /* ---------------------------------------------------- Ozaki split DGEMM -- */
 
-#define DG_OZAKI_INT8 0  /* 7-bit digits in int8, v_mfma_i32_16x16x64_i8, exact INT32 sums  */
-#define DG_OZAKI_BF16 1  /* 5-bit digits in bf16, v_mfma_f32_16x16x32_bf16, exact FP32 sums */
+#define DG_OZAKI_INT8 0      /* 7-bit digits in int8, v_mfma_i32_16x16x64_i8, exact INT32 sums  */
+#define DG_OZAKI_BF16 1      /* 5-bit digits in bf16, v_mfma_f32_16x16x32_bf16, exact FP32 sums */
+#define DG_OZAKI_INT8_CRT 2  /* Ozaki scheme II: `slices` = moduli (2..16), one INT8 GEMM each,
+                                exact CRT reconstruction (Ozaki, Uchino, Imamura 2025)       */
 
 /* Bytes of device workspace dg_ozaki_dgemm needs (-1 on bad arguments). */
 int64_t dg_ozaki_workspace(int64_t m, int64_t n, int64_t k, int slices, int kind);
@@ -237,7 +239,11 @@ int64_t dg_ozaki_workspace(int64_t m, int64_t n, int64_t k, int slices, int kind
  * caller-owned device memory of dg_ozaki_workspace() bytes, 256-byte
  * aligned.  If err_bound is not NULL it receives k*(s+3)*2^(-b*s) (b = 7 for
  * INT8, 5 for BF16): |C - op(A)op(B)|_ij <= err_bound * 2^(e_i + f_j) for the
- * product part, e_i / f_j being the row / column scales.
+ * product part, e_i / f_j being the row / column scales.  With
+ * DG_OZAKI_INT8_CRT, `slices` is the number of moduli N; rows and columns are
+ * scaled to q-bit integers (q <= 53, the largest with k*4^q*4 <= the moduli's
+ * product), their integer product is formed exactly from N INT8 GEMMs and
+ * rebuilt by the Chinese remainder theorem, and err_bound = k*2^(1-q).
  * New relative to the reference (no split DGEMM exists there); it replaces
  * the same cublasDgemm call sites as dg_dgemm (src/25DCannon.cpp:324-325). */
 int dg_ozaki_dgemm(char opA, char opB, int64_t m, int64_t n, int64_t k, double alpha,

@@ -319,3 +319,100 @@ def ozaki_dgemm(a, b, s, alpha=1.0, beta=0.0, c=None, bits=OZ_INT8_BITS, kchunk=
     if beta != 0.0:
         out = out + beta * c
     return out
+
+
+# ---------------------------------------------------------------- Ozaki scheme II (CRT)
+# Ozaki, Uchino, Imamura, "Ozaki Scheme II: A GEMM-oriented emulation of
+# floating-point matrix multiplication using an integer modular technique"
+# (2025).  Rows of A / columns of B are scaled to integers of q bits; their
+# exact integer product is computed modulo N pairwise coprime moduli <= 256
+# (one INT8 GEMM each) and rebuilt by the Chinese remainder theorem.
+
+CRT_MODULI = (256, 255, 253, 251, 247, 241, 239, 233, 229, 227, 223, 217, 211, 199, 197, 193, 191, 181, 179, 173)
+
+
+def crt_q(K, nmod):
+    """Bits q per operand so that k * 2^(2q) stays below P/4 (P = product of
+    the first nmod moduli), capped at 53 (integers exact in FP64)."""
+    import math
+    P = 1
+    for p in CRT_MODULI[:nmod]:
+        P *= p
+    logp = math.log2(P)
+    q = int((logp - 2 - math.log2(max(K, 1))) // 2)
+    while q > 0 and K * 4 ** q * 4 > P:
+        q -= 1
+    return min(q, 53)
+
+
+def crt_bal(x, p):
+    """x mod p in [-floor(p/2), ceil(p/2) - 1] (numpy int64 arrays)."""
+    r = np.mod(x, p)
+    return np.where(2 * r >= p, r - p, r)
+
+
+def crt_scale(X, e, axis, q):
+    """Integers trunc(x * 2^(q - e)) (exact in FP64 for q <= 53) as int64."""
+    shape = [1, 1]
+    shape[1 - axis] = -1
+    return np.trunc(np.ldexp(X, (q - e).reshape(shape))).astype(np.int64)
+
+
+def crt_tables(nmod):
+    """R_t = prod_{u<t} p_u (Python ints), R_t mod p_u and R_t^{-1} mod p_t."""
+    ps = CRT_MODULI[:nmod]
+    R = [1]
+    for p in ps[:-1]:
+        R.append(R[-1] * p)
+    rmod = [[R[t] % ps[u] for u in range(nmod)] for t in range(nmod)]
+    rinv = [pow(R[t] % ps[t], -1, ps[t]) if t else 1 for t in range(nmod)]
+    return ps, R, rmod, rinv
+
+
+def crt_reconstruct(res, nmod):
+    """Exact integers (Python ints, object array) from residues res[t] in
+    [0, p_t) by Garner's algorithm with balanced mixed-radix digits."""
+    ps, R, rmod, rinv = crt_tables(nmod)
+    shape = res[0].shape
+    s = [np.zeros(shape, dtype=np.int64) for _ in range(nmod)]  # partial sums mod p_u
+    digits = []
+    for t in range(nmod):
+        v = crt_bal((res[t].astype(np.int64) - s[t]) * rinv[t], ps[t])
+        digits.append(v)
+        for u in range(t + 1, nmod):
+            s[u] = np.mod(s[u] + v * rmod[t][u], ps[u])
+    out = np.zeros(shape, dtype=object)
+    for t in range(nmod - 1, -1, -1):
+        out = out * (ps[t - 1] if t else 1) if False else out
+    total = np.zeros(shape, dtype=object)
+    for t in range(nmod):
+        total = total + digits[t].astype(object) * R[t]
+    return total
+
+
+def ozaki2_dgemm(a, b, nmod, alpha=1.0, beta=0.0, c=None):
+    """C = alpha * A B + beta * C through nmod INT8 modular products (Ozaki
+    scheme II) on 2-D float64 arrays a (M x K), b (K x N).  The rebuilt
+    integer V = A'B' is exact; the result is alpha * ldexp(double(V),
+    e_i + f_j - 2q) (+ beta * C), double(V) correctly rounded."""
+    M, K = a.shape
+    N = b.shape[1]
+    q = crt_q(K, nmod)
+    e = oz_exponents(a, 1)
+    f = oz_exponents(b, 0)
+    Ai = crt_scale(a, e, 1, q)
+    Bi = crt_scale(b, f, 0, q)
+    ps = CRT_MODULI[:nmod]
+    res = []
+    for p in ps:
+        ar = crt_bal(Ai, p).astype(np.float64)
+        br = crt_bal(Bi, p).astype(np.float64)
+        W = ar @ br                      # exact: |entries| <= 128^2 * K < 2^53
+        res.append(np.mod(W.astype(np.int64), p))
+    V = crt_reconstruct(res, nmod)
+    Vd = np.vectorize(float, otypes=[np.float64])(V)     # correctly rounded
+    prod = np.ldexp(Vd, (e.reshape(-1, 1) + f.reshape(1, -1) - 2 * q).astype(np.int64))
+    out = alpha * prod
+    if beta != 0.0:
+        out = out + beta * c
+    return out

@@ -112,3 +112,75 @@ def test_error_per_slice_count(dev, dist):
         assert errs[s + 1] < errs[s] / 10
     assert errs[8] <= 2 * native
     assert errs[9] <= 2 * native
+
+
+# ---------------------------------------------------------------- Ozaki scheme II (CRT)
+
+def run_crt(dev, opA, opB, m, n, k, nmod, alpha=1.0, beta=0.0, gen=None, seed=0):
+    rng = np.random.default_rng(seed)
+    gen = gen or (lambda shape: rng.uniform(-1, 1, shape))
+    a = gen((m, k) if opA == "N" else (k, m))
+    b = gen((k, n) if opB == "N" else (n, k))
+    c = rng.uniform(-1, 1, (m, n))
+    A, B, C = upload(a, dev), upload(b, dev), upload(c, dev)
+    if beta == 0.0:
+        C.fill_(float("nan"))
+    bound = dgamd.ozaki_dgemm_t(opA, opB, m, n, k, alpha, A, a.shape[0], B, b.shape[0], beta, C, m, nmod,
+                                dgamd.DG_OZAKI_INT8_CRT)
+    got = C.cpu().numpy().reshape(n, m).T
+    aa = np.ascontiguousarray(a if opA == "N" else a.T)
+    bb = np.ascontiguousarray(b if opB == "N" else b.T)
+    ref = oracle.ozaki2_dgemm(aa, bb, nmod, alpha, beta, c)
+    return got, ref, bound
+
+
+@pytest.mark.parametrize("opA,opB", [("N", "N"), ("N", "T"), ("T", "N"), ("T", "T")])
+def test_crt_bit_exact_ops(dev, opA, opB):
+    got, ref, _ = run_crt(dev, opA, opB, 130, 90, 200, 14, alpha=1.5, beta=-0.5)
+    assert np.array_equal(got.view(np.uint64), ref.view(np.uint64))
+
+
+@pytest.mark.parametrize("nmod", [2, 8, 12, 15, 16])
+def test_crt_bit_exact_moduli(dev, nmod):
+    rng = np.random.default_rng(nmod)
+    got, ref, _ = run_crt(dev, "N", "N", 96, 64, 300, nmod, gen=lambda sh: wide(rng, sh), seed=nmod)
+    assert np.array_equal(got.view(np.uint64), ref.view(np.uint64))
+
+
+def test_crt_ragged_and_zero_lines(dev):
+    got, ref, _ = run_crt(dev, "N", "N", 1, 300, 17, 16)
+    assert np.array_equal(got.view(np.uint64), ref.view(np.uint64))
+    m = n = 40
+    k = 33
+    a = np.zeros((m, k)); a[5] = np.linspace(-3, 3, k)
+    b = np.zeros((k, n)); b[:, 2] = 1.0
+    A, B = upload(a, dev), upload(b, dev)
+    C = torch.zeros(m * n, dtype=torch.float64, device=dev)
+    dgamd.ozaki_dgemm_t("N", "N", m, n, k, 1.0, A, m, B, k, 0.0, C, m, 16, dgamd.DG_OZAKI_INT8_CRT)
+    got = C.cpu().numpy().reshape(n, m).T
+    assert np.array_equal(got, oracle.ozaki2_dgemm(a, b, 16))
+    assert np.array_equal(got, a @ b)
+
+
+@pytest.mark.parametrize("dist", ["uniform", "wide"])
+def test_crt_error_per_moduli(dev, dist):
+    """1024 x 1024 x 16384: error vs the host FP64 dgemm falls ~2^-8 per
+    modulus; with 16 moduli it is within 2x the native FP64 MFMA error."""
+    m = n = 1024
+    k = 16384
+    rng = np.random.default_rng(9)
+    gen = (lambda sh: rng.uniform(-1, 1, sh)) if dist == "uniform" else (lambda sh: wide(rng, sh))
+    a, b = gen((m, k)), gen((k, n))
+    A, B = upload(a, dev), upload(b, dev)
+    ref = a @ b
+    C = torch.empty(m * n, dtype=torch.float64, device=dev)
+    dgamd.dgemm_t("N", "N", m, n, k, 1.0, A, m, B, k, 0.0, C, m)
+    native = oracle.rel_frobenius(C.cpu().numpy().reshape(n, m).T, ref)
+    errs = {}
+    for nm in (12, 13, 14, 15, 16):
+        dgamd.ozaki_dgemm_t("N", "N", m, n, k, 1.0, A, m, B, k, 0.0, C, m, nm, dgamd.DG_OZAKI_INT8_CRT)
+        errs[nm] = oracle.rel_frobenius(C.cpu().numpy().reshape(n, m).T, ref)
+    print(dist, "native %.3e" % native, {s: "%.3e" % e for s, e in errs.items()})
+    for nm in (12, 13):
+        assert errs[nm + 1] < errs[nm] / 8
+    assert errs[16] <= 2 * native

@@ -104,3 +104,61 @@ def test_bf16_chunking():
     b = rng.integers(-30, 31, (20000, 2)).astype(float)
     got = oracle.ozaki_dgemm(a, b, 2, bits=5)   # 10 bits >= 5 bits of the entries
     assert np.array_equal(got, a @ b)
+
+
+# ---------------------------------------------------------------- Ozaki scheme II (CRT)
+
+def test_crt_moduli_pairwise_coprime():
+    from math import gcd
+    ps = oracle.CRT_MODULI[:16]
+    assert all(p <= 256 for p in ps)
+    assert all(gcd(a, b) == 1 for i, a in enumerate(ps) for b in ps[i + 1:])
+
+
+@pytest.mark.parametrize("nmod", [2, 5, 10, 16])
+def test_crt_reconstruct_exact(nmod):
+    import random
+    P = 1
+    for p in oracle.CRT_MODULI[:nmod]:
+        P *= p
+    rnd = random.Random(nmod)
+    vals = [rnd.randrange(-(P // 4), P // 4) for _ in range(200)] + [0, 1, -1, P // 4 - 1, -(P // 4)]
+    res = [np.array([v % p for v in vals], dtype=np.int64) for p in oracle.CRT_MODULI[:nmod]]
+    got = oracle.crt_reconstruct(res, nmod)
+    assert list(got) == vals
+
+
+def test_crt_q_bound():
+    for K in (1, 100, 4096, 16384, 100000):
+        for nm in range(2, 17):
+            q = oracle.crt_q(K, nm)
+            P = 1
+            for p in oracle.CRT_MODULI[:nm]:
+                P *= p
+            if q > 0:
+                assert K * 4 ** q * 4 <= P
+                assert q == 53 or K * 4 ** (q + 1) * 4 > P
+
+
+def test_crt_exact_for_short_integers():
+    rng = np.random.default_rng(11)
+    a = rng.integers(-1000, 1000, (7, 50)).astype(float)
+    b = rng.integers(-1000, 1000, (50, 9)).astype(float)
+    assert np.array_equal(oracle.ozaki2_dgemm(a, b, 16), a @ b)
+
+
+@pytest.mark.parametrize("nmod", [8, 12, 16])
+def test_crt_error_within_bound(nmod):
+    rng = np.random.default_rng(nmod)
+    a = wide(rng, (5, 40))
+    b = wide(rng, (40, 6))
+    got = oracle.ozaki2_dgemm(a, b, nmod)
+    q = oracle.crt_q(40, nmod)
+    e = oracle.oz_exponents(a, 1)
+    f = oracle.oz_exponents(b, 0)
+    for i in range(5):
+        for j in range(6):
+            exact = sum(Fraction(a[i, k]) * Fraction(b[k, j]) for k in range(40))
+            bound = Fraction(40 * 2) * Fraction(2) ** (int(e[i] + f[j]) - q)
+            slack = Fraction(abs(float(exact))) * Fraction(2) ** -52
+            assert abs(Fraction(got[i, j]) - exact) <= bound + slack
