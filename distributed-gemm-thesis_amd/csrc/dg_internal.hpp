@@ -73,8 +73,8 @@ struct Prob {
     int64_t ldc;
     double alpha, beta;
     int32_t seg0, nseg;    // segment range in the segment table
-    int32_t tile0;         // first global tile id of this problem
-    int32_t tiles_m, tiles_n;
+    int32_t tile0[2];      // first global tile id of this problem, per tile shape
+    int32_t tiles_m[2], tiles_n[2];
     int32_t pad_;
 };
 
@@ -85,6 +85,13 @@ enum {
     DG_GEMM_MODE_PERSISTENT = 2,   // CU-count workgroups pull tiles from an atomic counter
 };
 
+// Output tile shapes of the FP64 GEMM.
+enum {
+    DG_GEMM_TILE_128x128 = 0,  // 4 waves, two workgroups per CU
+    DG_GEMM_TILE_256x128 = 1,  // 8 waves, one workgroup per CU
+    DG_GEMM_TILE_DEFAULT = DG_GEMM_TILE_128x128,
+};
+
 // Launches the FP64 MFMA GEMM over `probs` (host vector; uploaded through
 // the table cache).  All problems share opA/opB.
 int launch_gemm_f64(char opA, char opB, const std::vector<Prob>& probs,
@@ -93,11 +100,13 @@ int launch_gemm_f64(char opA, char opB, const std::vector<Prob>& probs,
 // Host half of a launch: drops empty problems, assigns tile ranges and tells
 // whether every segment allows 128-bit loads.
 int prepare_gemm(const std::vector<Prob>& probs_in, const std::vector<Seg>& segs, std::vector<Prob>& probs,
-                 bool* vec, int* tiles);
+                 bool* vec, int tiles[2]);
 // Device half: tables already resident.  The persistent mode needs a device
 // int `counter`, which it zeroes with a kernel before the launch (graph safe).
-int launch_gemm_f64_dev(char opA, char opB, bool vec, int mode, const Prob* dp, int np, const Seg* ds,
-                        int tiles, int* counter, hipStream_t stream);
+int launch_gemm_f64_dev(char opA, char opB, bool vec, int mode, int cfg, const Prob* dp, int np, const Seg* ds,
+                        const int tiles[2], int* counter, hipStream_t stream);
+// Tile shape used by the one-shot entry points (DG_GEMM_TILE env override).
+int default_tile_cfg();
 
 // C[x] = beta*C[x] over an m x n column-major matrix (beta == 0 writes 0).
 int launch_scale(int64_t m, int64_t n, double beta, double* C, int64_t ldc, hipStream_t stream);

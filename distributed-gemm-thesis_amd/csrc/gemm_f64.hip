@@ -22,6 +22,8 @@
 // consecutive rows of one C column: the epilogue stores are 128-byte runs.
 // Result layout of v_mfma_f64_16x16x4_f64: lane l, register v holds
 // D(l/16 + 4v, l%16) (checked on the device by the exact-integer tests).
+#include <string>
+
 #include "dg_internal.hpp"
 
 namespace dg {
@@ -39,12 +41,22 @@ typedef __attribute__((address_space(1))) const dv2 gdv2;
 __device__ __forceinline__ double gld(const double* p) { return *(gdouble*)p; }
 __device__ __forceinline__ dv2 gld2(const double* p) { return *(gdv2*)p; }
 
-constexpr int BM = 128;
-constexpr int BN = 128;
 constexpr int BK = 16;
 constexpr int LDK = BK + 2;  // padded LDS row stride (doubles)
-constexpr int NT = 256;
 constexpr int GROUP_M = 8;   // tile rows per raster group
+
+// Tile shapes: BM x BN output tiles computed by NTH threads, each wave owning
+// a 64 x 64 sub-tile (4 x 4 MFMA tiles of 16 x 16); OCC workgroups per CU.
+template <int BM_, int BN_, int NTH_, int OCC_>
+struct TileCfg {
+    static constexpr int BM = BM_, BN = BN_, NTH = NTH_, OCC = OCC_;
+    static constexpr int WAVES_N = BN / 64;
+    static constexpr int PA = BM * BK / (2 * NTH);  // 128-bit loads per thread per k-step (A)
+    static constexpr int PB = BN * BK / (2 * NTH);  // (B)
+    static_assert((BM / 64) * (BN / 64) * 64 == NTH, "one wave per 64x64 sub-tile");
+};
+using Cfg128 = TileCfg<128, 128, 256, 2>;  // 73.7 KB LDS, two workgroups per CU
+using Cfg256 = TileCfg<256, 128, 512, 1>;  // 110.6 KB LDS, 8 waves, half the B traffic per flop
 
 // Block id -> tile id so that each XCD (blocks are dealt round-robin to the
 // 8 XCDs) receives one contiguous range of tiles, which the grouped raster
@@ -57,17 +69,17 @@ __device__ __forceinline__ int xcd_remap(int bid, int total) {
 }
 
 // Loads one k-step of an operand stored "k-contiguous" (element (row, k) at
-// base[k + row*ld]) into registers: 128 rows x 16 k; thread t takes k pair
-// 2*(t%8) of rows t/8 + 32*q.
-template <bool EDGE, bool VEC>
-__device__ __forceinline__ void load_kc(dv2 (&r)[4], const double* __restrict__ base, int64_t ld,
+// base[k + row*ld]) into registers: ROWS rows x 16 k; thread t takes k pair
+// 2*(t%8) of rows t/8 + (NTH/8)*q.
+template <int ROWS, int NTH, bool EDGE, bool VEC>
+__device__ __forceinline__ void load_kc(dv2 (&r)[ROWS * BK / (2 * NTH)], const double* __restrict__ base, int64_t ld,
                                         int64_t row0, int64_t rows, int64_t k0, int64_t klen) {
     const int t = threadIdx.x;
     const int kp = (t & 7) * 2;
     const int rr = t >> 3;
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        int64_t row = row0 + rr + 32 * q;
+    for (int q = 0; q < ROWS * BK / (2 * NTH); ++q) {
+        int64_t row = row0 + rr + (NTH / 8) * q;
         int64_t k = k0 + kp;
         if (!EDGE) {
             const double* p = base + k + row * ld;
@@ -87,18 +99,19 @@ __device__ __forceinline__ void load_kc(dv2 (&r)[4], const double* __restrict__ 
 }
 
 // Loads one k-step of an operand stored "row-contiguous" (element (row, k)
-// at base[row + k*ld]): thread t takes rows 2*(t%64), 2*(t%64)+1 of k
-// column t/64 + 4*q.
-template <bool EDGE, bool VEC>
-__device__ __forceinline__ void load_rc(dv2 (&r)[4], const double* __restrict__ base, int64_t ld,
+// at base[row + k*ld]): thread t takes rows 2*(t%(ROWS/2)), +1 of k column
+// t/(ROWS/2) + (NTH/(ROWS/2))*q.
+template <int ROWS, int NTH, bool EDGE, bool VEC>
+__device__ __forceinline__ void load_rc(dv2 (&r)[ROWS * BK / (2 * NTH)], const double* __restrict__ base, int64_t ld,
                                         int64_t row0, int64_t rows, int64_t k0, int64_t klen) {
+    constexpr int HP = ROWS / 2;
     const int t = threadIdx.x;
-    const int rp = (t & 63) * 2;
-    const int kc = t >> 6;
+    const int rp = (t % HP) * 2;
+    const int kc = t / HP;
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
+    for (int q = 0; q < ROWS * BK / (2 * NTH); ++q) {
         int64_t row = row0 + rp;
-        int64_t k = k0 + kc + 4 * q;
+        int64_t k = k0 + kc + (NTH / HP) * q;
         if (!EDGE) {
             const double* p = base + row + k * ld;
             if (VEC) {
@@ -122,56 +135,69 @@ __device__ __forceinline__ void load_rc(dv2 (&r)[4], const double* __restrict__ 
     }
 }
 
-__device__ __forceinline__ void store_kc(double* __restrict__ s, const dv2 (&r)[4]) {
+template <int ROWS, int NTH>
+__device__ __forceinline__ void store_kc(double* __restrict__ s, const dv2 (&r)[ROWS * BK / (2 * NTH)]) {
     const int t = threadIdx.x;
     const int kp = (t & 7) * 2;
     const int rr = t >> 3;
 #pragma unroll
-    for (int q = 0; q < 4; ++q) *reinterpret_cast<dv2*>(&s[(rr + 32 * q) * LDK + kp]) = r[q];
+    for (int q = 0; q < ROWS * BK / (2 * NTH); ++q) *reinterpret_cast<dv2*>(&s[(rr + (NTH / 8) * q) * LDK + kp]) = r[q];
 }
 
-__device__ __forceinline__ void store_rc(double* __restrict__ s, const dv2 (&r)[4]) {
+template <int ROWS, int NTH>
+__device__ __forceinline__ void store_rc(double* __restrict__ s, const dv2 (&r)[ROWS * BK / (2 * NTH)]) {
+    constexpr int HP = ROWS / 2;
     const int t = threadIdx.x;
-    const int rp = (t & 63) * 2;
-    const int kc = t >> 6;
+    const int rp = (t % HP) * 2;
+    const int kc = t / HP;
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        s[rp * LDK + kc + 4 * q] = r[q].x;
-        s[(rp + 1) * LDK + kc + 4 * q] = r[q].y;
+    for (int q = 0; q < ROWS * BK / (2 * NTH); ++q) {
+        s[rp * LDK + kc + (NTH / HP) * q] = r[q].x;
+        s[(rp + 1) * LDK + kc + (NTH / HP) * q] = r[q].y;
     }
 }
 
 // op(A) is k-contiguous in memory when A is transposed; op(B) when B is not.
-template <bool TA, bool EDGE, bool VEC>
-__device__ __forceinline__ void load_a(dv2 (&r)[4], const Seg& s, int64_t m0, int64_t M, int64_t k0) {
-    if (TA) load_kc<EDGE, VEC>(r, s.a, s.lda, m0, M, k0, s.klen);
-    else    load_rc<EDGE, VEC>(r, s.a, s.lda, m0, M, k0, s.klen);
+template <class C, bool TA, bool EDGE, bool VEC>
+__device__ __forceinline__ void load_a(dv2 (&r)[C::PA], const Seg& s, int64_t m0, int64_t M, int64_t k0) {
+    if (TA) load_kc<C::BM, C::NTH, EDGE, VEC>(r, s.a, s.lda, m0, M, k0, s.klen);
+    else    load_rc<C::BM, C::NTH, EDGE, VEC>(r, s.a, s.lda, m0, M, k0, s.klen);
 }
-template <bool TB, bool EDGE, bool VEC>
-__device__ __forceinline__ void load_b(dv2 (&r)[4], const Seg& s, int64_t n0, int64_t N, int64_t k0) {
-    if (TB) load_rc<EDGE, VEC>(r, s.b, s.ldb, n0, N, k0, s.klen);
-    else    load_kc<EDGE, VEC>(r, s.b, s.ldb, n0, N, k0, s.klen);
+template <class C, bool TB, bool EDGE, bool VEC>
+__device__ __forceinline__ void load_b(dv2 (&r)[C::PB], const Seg& s, int64_t n0, int64_t N, int64_t k0) {
+    if (TB) load_rc<C::BN, C::NTH, EDGE, VEC>(r, s.b, s.ldb, n0, N, k0, s.klen);
+    else    load_kc<C::BN, C::NTH, EDGE, VEC>(r, s.b, s.ldb, n0, N, k0, s.klen);
+}
+template <class C, bool TA>
+__device__ __forceinline__ void store_a(double* s, const dv2 (&r)[C::PA]) {
+    if (TA) store_kc<C::BM, C::NTH>(s, r); else store_rc<C::BM, C::NTH>(s, r);
+}
+template <class C, bool TB>
+__device__ __forceinline__ void store_b(double* s, const dv2 (&r)[C::PB]) {
+    if (TB) store_rc<C::BN, C::NTH>(s, r); else store_kc<C::BN, C::NTH>(s, r);
 }
 
 // Computes one output tile of the grouped problem set.
-template <bool TA, bool TB, bool VEC>
+template <class C, bool TA, bool TB, bool VEC>
 __device__ __forceinline__ void gemm_tile(const Prob* __restrict__ probs, int nprob, const Seg* __restrict__ segs,
-                                          int tile, double (&As)[2][BM * LDK], double (&Bs)[2][BN * LDK]) {
+                                          int tile, double (&As)[2][C::BM * LDK], double (&Bs)[2][C::BN * LDK]) {
+    constexpr int BM = C::BM, BN = C::BN;
+    constexpr int ci = BM == 128 ? 0 : 1;  // which tile range of the problem table
 
     // Problem owning this tile: last problem with tile0 <= tile.
     int lo = 0, hi = nprob - 1;
     while (lo < hi) {
         int mid = (lo + hi + 1) >> 1;
-        if (probs[mid].tile0 <= tile) lo = mid; else hi = mid - 1;
+        if (probs[mid].tile0[ci] <= tile) lo = mid; else hi = mid - 1;
     }
     const Prob P = probs[lo];
-    const int local = tile - P.tile0;
+    const int local = tile - P.tile0[ci];
 
     // Grouped raster: GROUP_M tile rows sweep the tile columns together.
-    const int per_group = GROUP_M * P.tiles_n;
+    const int per_group = GROUP_M * P.tiles_n[ci];
     const int group = local / per_group;
     const int first_m = group * GROUP_M;
-    const int gsize = min(P.tiles_m - first_m, GROUP_M);
+    const int gsize = min(P.tiles_m[ci] - first_m, GROUP_M);
     const int in_group = local - group * per_group;
     const int tm = first_m + in_group % gsize;
     const int tn = in_group / gsize;
@@ -182,8 +208,8 @@ __device__ __forceinline__ void gemm_tile(const Prob* __restrict__ probs, int np
 
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    const int wm = (wave >> 1) * 64;
-    const int wn = (wave & 1) * 64;
+    const int wm = (wave / C::WAVES_N) * 64;
+    const int wn = (wave % C::WAVES_N) * 64;
     const int lr = lane & 15;
     const int lg = lane >> 4;
 
@@ -198,17 +224,17 @@ __device__ __forceinline__ void gemm_tile(const Prob* __restrict__ probs, int np
     Seg cur = segs[s];
     int64_t kk = 0;
 
-    dv2 ra[4], rb[4];
+    dv2 ra[C::PA], rb[C::PB];
     if (interior && BK <= cur.klen) {
-        load_a<TA, false, VEC>(ra, cur, m0, P.m, 0);
-        load_b<TB, false, VEC>(rb, cur, n0, P.n, 0);
+        load_a<C, TA, false, VEC>(ra, cur, m0, P.m, 0);
+        load_b<C, TB, false, VEC>(rb, cur, n0, P.n, 0);
     } else {
-        load_a<TA, true, false>(ra, cur, m0, P.m, 0);
-        load_b<TB, true, false>(rb, cur, n0, P.n, 0);
+        load_a<C, TA, true, false>(ra, cur, m0, P.m, 0);
+        load_b<C, TB, true, false>(rb, cur, n0, P.n, 0);
     }
     int buf = 0;
-    if (TA) store_kc(As[0], ra); else store_rc(As[0], ra);
-    if (TB) store_rc(Bs[0], rb); else store_kc(Bs[0], rb);
+    store_a<C, TA>(As[0], ra);
+    store_b<C, TB>(Bs[0], rb);
     __syncthreads();
 
     while (true) {
@@ -224,11 +250,11 @@ __device__ __forceinline__ void gemm_tile(const Prob* __restrict__ probs, int np
         const bool has_next = ns < s_end;
         if (has_next) {
             if (interior && nk + BK <= nxt.klen) {
-                load_a<TA, false, VEC>(ra, nxt, m0, P.m, nk);
-                load_b<TB, false, VEC>(rb, nxt, n0, P.n, nk);
+                load_a<C, TA, false, VEC>(ra, nxt, m0, P.m, nk);
+                load_b<C, TB, false, VEC>(rb, nxt, n0, P.n, nk);
             } else {
-                load_a<TA, true, false>(ra, nxt, m0, P.m, nk);
-                load_b<TB, true, false>(rb, nxt, n0, P.n, nk);
+                load_a<C, TA, true, false>(ra, nxt, m0, P.m, nk);
+                load_b<C, TB, true, false>(rb, nxt, n0, P.n, nk);
             }
         }
 
@@ -256,8 +282,8 @@ __device__ __forceinline__ void gemm_tile(const Prob* __restrict__ probs, int np
 
         if (!has_next) break;
         buf ^= 1;
-        if (TA) store_kc(As[buf], ra); else store_rc(As[buf], ra);
-        if (TB) store_rc(Bs[buf], rb); else store_kc(Bs[buf], rb);
+        store_a<C, TA>(As[buf], ra);
+        store_b<C, TB>(Bs[buf], rb);
         __syncthreads();
         s = ns;
         kk = nk;
@@ -292,16 +318,16 @@ __device__ __forceinline__ void gemm_tile(const Prob* __restrict__ probs, int np
 // MODE 1: one workgroup per tile, XCD-contiguous tile ranges (xcd_remap);
 // MODE 2: persistent: every workgroup takes tiles from `counter` (zeroed
 //         before the launch) until the tile space is exhausted.
-template <bool TA, bool TB, bool VEC, int MODE>
-__global__ __launch_bounds__(NT, 2) void gemm_f64_kernel(const Prob* __restrict__ probs, int nprob,
-                                                         const Seg* __restrict__ segs, int total_tiles,
-                                                         int* __restrict__ counter) {
-    __shared__ double As[2][BM * LDK];
-    __shared__ double Bs[2][BN * LDK];
+template <class C, bool TA, bool TB, bool VEC, int MODE>
+__global__ __launch_bounds__(C::NTH, C::OCC) void gemm_f64_kernel(const Prob* __restrict__ probs, int nprob,
+                                                                  const Seg* __restrict__ segs, int total_tiles,
+                                                                  int* __restrict__ counter) {
+    __shared__ double As[2][C::BM * LDK];
+    __shared__ double Bs[2][C::BN * LDK];
     if (MODE == 0) {
-        gemm_tile<TA, TB, VEC>(probs, nprob, segs, blockIdx.x, As, Bs);
+        gemm_tile<C, TA, TB, VEC>(probs, nprob, segs, blockIdx.x, As, Bs);
     } else if (MODE == 1) {
-        gemm_tile<TA, TB, VEC>(probs, nprob, segs, xcd_remap(blockIdx.x, total_tiles), As, Bs);
+        gemm_tile<C, TA, TB, VEC>(probs, nprob, segs, xcd_remap(blockIdx.x, total_tiles), As, Bs);
     } else {
         __shared__ int next;
         while (true) {
@@ -310,7 +336,7 @@ __global__ __launch_bounds__(NT, 2) void gemm_f64_kernel(const Prob* __restrict_
             __syncthreads();
             const int tile = next;
             if (tile >= total_tiles) break;
-            gemm_tile<TA, TB, VEC>(probs, nprob, segs, tile, As, Bs);
+            gemm_tile<C, TA, TB, VEC>(probs, nprob, segs, tile, As, Bs);
         }
     }
 }
@@ -327,22 +353,35 @@ __global__ void scale_kernel(int64_t m, int64_t n, double beta, double* __restri
     }
 }
 
-template <bool TA, bool TB, bool VEC>
+template <class C, bool TA, bool TB, bool VEC>
 void launch_mode(int mode, const Prob* dp, int np, const Seg* ds, int tiles, int grid, int* counter,
                  hipStream_t st) {
     if (mode == DG_GEMM_MODE_ROUND_ROBIN)
-        hipLaunchKernelGGL((gemm_f64_kernel<TA, TB, VEC, 0>), dim3(tiles), dim3(NT), 0, st, dp, np, ds, tiles, counter);
+        hipLaunchKernelGGL((gemm_f64_kernel<C, TA, TB, VEC, 0>), dim3(tiles), dim3(C::NTH), 0, st, dp, np, ds, tiles,
+                           counter);
     else if (mode == DG_GEMM_MODE_XCD)
-        hipLaunchKernelGGL((gemm_f64_kernel<TA, TB, VEC, 1>), dim3(tiles), dim3(NT), 0, st, dp, np, ds, tiles, counter);
+        hipLaunchKernelGGL((gemm_f64_kernel<C, TA, TB, VEC, 1>), dim3(tiles), dim3(C::NTH), 0, st, dp, np, ds, tiles,
+                           counter);
     else
-        hipLaunchKernelGGL((gemm_f64_kernel<TA, TB, VEC, 2>), dim3(grid), dim3(NT), 0, st, dp, np, ds, tiles, counter);
+        hipLaunchKernelGGL((gemm_f64_kernel<C, TA, TB, VEC, 2>), dim3(grid), dim3(C::NTH), 0, st, dp, np, ds, tiles,
+                           counter);
 }
 
-template <bool TA, bool TB>
+template <class C, bool TA, bool TB>
 void launch_kind(bool vec, int mode, const Prob* dp, int np, const Seg* ds, int tiles, int grid, int* counter,
                  hipStream_t st) {
-    if (vec) launch_mode<TA, TB, true>(mode, dp, np, ds, tiles, grid, counter, st);
-    else     launch_mode<TA, TB, false>(mode, dp, np, ds, tiles, grid, counter, st);
+    if (vec) launch_mode<C, TA, TB, true>(mode, dp, np, ds, tiles, grid, counter, st);
+    else     launch_mode<C, TA, TB, false>(mode, dp, np, ds, tiles, grid, counter, st);
+}
+
+template <class C>
+void launch_cfg(char opA, char opB, bool vec, int mode, const Prob* dp, int np, const Seg* ds, int tiles, int grid,
+                int* counter, hipStream_t st) {
+    const bool ta = opA == 'T', tb = opB == 'T';
+    if (!ta && !tb) launch_kind<C, false, false>(vec, mode, dp, np, ds, tiles, grid, counter, st);
+    else if (!ta && tb) launch_kind<C, false, true>(vec, mode, dp, np, ds, tiles, grid, counter, st);
+    else if (ta && !tb) launch_kind<C, true, false>(vec, mode, dp, np, ds, tiles, grid, counter, st);
+    else launch_kind<C, true, true>(vec, mode, dp, np, ds, tiles, grid, counter, st);
 }
 
 }  // namespace
@@ -358,19 +397,22 @@ int launch_scale(int64_t m, int64_t n, double beta, double* C, int64_t ldc, hipS
 }
 
 int prepare_gemm(const std::vector<Prob>& probs_in, const std::vector<Seg>& segs, std::vector<Prob>& probs,
-                 bool* vec, int* tiles_out) {
+                 bool* vec, int tiles_out[2]) {
     probs.clear();
     probs.reserve(probs_in.size());
-    int64_t tiles = 0;
+    int64_t tiles[2] = {0, 0};
+    const int bm[2] = {Cfg128::BM, Cfg256::BM}, bn[2] = {Cfg128::BN, Cfg256::BN};
     bool v = true;
     for (const Prob& p0 : probs_in) {
         if (p0.m <= 0 || p0.n <= 0 || p0.nseg <= 0) continue;
         Prob p = p0;
-        p.tiles_m = (int32_t)((p.m + BM - 1) / BM);
-        p.tiles_n = (int32_t)((p.n + BN - 1) / BN);
-        p.tile0 = (int32_t)tiles;
-        tiles += (int64_t)p.tiles_m * p.tiles_n;
-        if (tiles > INT32_MAX) return fail(DG_ERR_INVALID, "gemm: %lld output tiles exceed one launch", (long long)tiles);
+        for (int c = 0; c < 2; ++c) {
+            p.tiles_m[c] = (int32_t)((p.m + bm[c] - 1) / bm[c]);
+            p.tiles_n[c] = (int32_t)((p.n + bn[c] - 1) / bn[c]);
+            p.tile0[c] = (int32_t)tiles[c];
+            tiles[c] += (int64_t)p.tiles_m[c] * p.tiles_n[c];
+        }
+        if (tiles[0] > INT32_MAX) return fail(DG_ERR_INVALID, "gemm: %lld output tiles exceed one launch", (long long)tiles[0]);
         probs.push_back(p);
         for (int s = p.seg0; s < p.seg0 + p.nseg; ++s) {
             const Seg& g = segs[s];
@@ -380,43 +422,52 @@ int prepare_gemm(const std::vector<Prob>& probs_in, const std::vector<Seg>& segs
         }
     }
     *vec = v;
-    *tiles_out = (int)tiles;
+    tiles_out[0] = (int)tiles[0];
+    tiles_out[1] = (int)tiles[1];
     return DG_OK;
 }
 
-int persistent_grid() {
+int persistent_grid(int cfg) {
     int dev = 0, cus = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
         cus = 256;
-    return cus * 2;  // two 73.7 KB-LDS workgroups fit one CU
+    return cus * (cfg == DG_GEMM_TILE_256x128 ? Cfg256::OCC : Cfg128::OCC);
 }
 
-int launch_gemm_f64_dev(char opA, char opB, bool vec, int mode, const Prob* dp, int np, const Seg* ds, int tiles,
-                        int* counter, hipStream_t stream) {
-    if (tiles <= 0) return DG_OK;
+int launch_gemm_f64_dev(char opA, char opB, bool vec, int mode, int cfg, const Prob* dp, int np, const Seg* ds,
+                        const int tiles[2], int* counter, hipStream_t stream) {
+    const int t = tiles[cfg == DG_GEMM_TILE_256x128 ? 1 : 0];
+    if (t <= 0) return DG_OK;
     int grid = 0;
     if (mode == DG_GEMM_MODE_PERSISTENT) {
         if (!counter) return fail(DG_ERR_INVALID, "gemm: persistent launch without a work counter");
-        grid = persistent_grid();
-        if (grid > tiles) grid = tiles;
+        grid = persistent_grid(cfg);
+        if (grid > t) grid = t;
         hipLaunchKernelGGL(zero_counter_kernel, dim3(1), dim3(1), 0, stream, counter);
     }
-    const bool ta = opA == 'T', tb = opB == 'T';
-    if (!ta && !tb) launch_kind<false, false>(vec, mode, dp, np, ds, tiles, grid, counter, stream);
-    else if (!ta && tb) launch_kind<false, true>(vec, mode, dp, np, ds, tiles, grid, counter, stream);
-    else if (ta && !tb) launch_kind<true, false>(vec, mode, dp, np, ds, tiles, grid, counter, stream);
-    else launch_kind<true, true>(vec, mode, dp, np, ds, tiles, grid, counter, stream);
+    if (cfg == DG_GEMM_TILE_256x128) launch_cfg<Cfg256>(opA, opB, vec, mode, dp, np, ds, t, grid, counter, stream);
+    else launch_cfg<Cfg128>(opA, opB, vec, mode, dp, np, ds, t, grid, counter, stream);
     DG_LAUNCH_CHECK();
     return DG_OK;
+}
+
+int default_tile_cfg() {
+    static const int cfg = [] {
+        const char* e = getenv("DG_GEMM_TILE");
+        if (e && std::string(e) == "256x128") return (int)DG_GEMM_TILE_256x128;
+        if (e && std::string(e) == "128x128") return (int)DG_GEMM_TILE_128x128;
+        return (int)DG_GEMM_TILE_DEFAULT;
+    }();
+    return cfg;
 }
 
 int launch_gemm_f64(char opA, char opB, const std::vector<Prob>& probs_in, const std::vector<Seg>& segs,
                     hipStream_t stream, int mode) {
     std::vector<Prob> probs;
     bool vec = true;
-    int tiles = 0;
-    int rc = prepare_gemm(probs_in, segs, probs, &vec, &tiles);
+    int tiles[2] = {0, 0};
+    int rc = prepare_gemm(probs_in, segs, probs, &vec, tiles);
     if (rc) return rc;
     if (probs.empty()) return DG_OK;
     const Prob* dp = static_cast<const Prob*>(cached_table(probs.data(), probs.size() * sizeof(Prob), stream));
@@ -424,7 +475,8 @@ int launch_gemm_f64(char opA, char opB, const std::vector<Prob>& probs_in, const
     const Seg* ds = static_cast<const Seg*>(cached_table(segs.data(), segs.size() * sizeof(Seg), stream));
     if (!ds) return DG_ERR_ALLOC;
     if (mode == DG_GEMM_MODE_PERSISTENT) mode = DG_GEMM_MODE_XCD;  // needs a counter: plans only
-    return launch_gemm_f64_dev(opA, opB, vec, mode, dp, (int)probs.size(), ds, tiles, nullptr, stream);
+    return launch_gemm_f64_dev(opA, opB, vec, mode, default_tile_cfg(), dp, (int)probs.size(), ds, tiles, nullptr,
+                               stream);
 }
 
 }  // namespace dg

@@ -14,13 +14,14 @@ struct dg_grouped_plan {
     struct Group {
         char opA, opB;
         bool vec;
-        int tiles, np;
+        int tiles[2], np;
         dg::Prob* dprobs = nullptr;
         dg::Seg* dsegs = nullptr;
     };
     std::vector<Group> groups;
     int mode = DG_PLAN_AUTOTUNE;   // requested
     int variant = -1;              // DG_GEMM_MODE_* in use (-1: not tuned yet)
+    int cfg = 0;                   // DG_GEMM_TILE_* in use
     std::string shape_key;
     int* counter = nullptr;        // device work counter (persistent mode)
     hipGraphExec_t graph = nullptr;
@@ -134,9 +135,9 @@ std::map<std::pair<char, char>, std::vector<int>> by_ops(const dg_gemm_desc* d, 
     return m;
 }
 
-int plan_launch(dg_grouped_plan* p, int variant, hipStream_t st) {
+int plan_launch(dg_grouped_plan* p, int variant, int cfg, hipStream_t st) {
     for (auto& g : p->groups) {
-        int rc = launch_gemm_f64_dev(g.opA, g.opB, g.vec, variant, g.dprobs, g.np, g.dsegs, g.tiles, p->counter, st);
+        int rc = launch_gemm_f64_dev(g.opA, g.opB, g.vec, variant, cfg, g.dprobs, g.np, g.dsegs, g.tiles, p->counter, st);
         if (rc) return rc;
     }
     return DG_OK;
@@ -151,9 +152,10 @@ int autotune(dg_grouped_plan* p, hipStream_t st) {
         std::lock_guard<std::mutex> lock(c.mu);
         tune_load_locked(c);
         auto it = c.best.find(p->shape_key);
-        if (it != c.best.end()) { p->variant = it->second; return DG_OK; }
+        if (it != c.best.end()) { p->variant = it->second % 4; p->cfg = it->second / 4; return DG_OK; }
     }
     const int variants[3] = {DG_GEMM_MODE_XCD, DG_GEMM_MODE_ROUND_ROBIN, DG_GEMM_MODE_PERSISTENT};
+    const int cfgs[2] = {DG_GEMM_TILE_128x128, DG_GEMM_TILE_256x128};
     size_t snap_bytes = 0;
     for (auto& sp : p->c_spans) snap_bytes += sp.second;
     char* snap = nullptr;
@@ -169,21 +171,22 @@ int autotune(dg_grouped_plan* p, hipStream_t st) {
     DG_HIP(hipEventCreate(&e0));
     DG_HIP(hipEventCreate(&e1));
     float best_ms = 1e30f;
-    int best = DG_GEMM_MODE_XCD;
-    for (int v : variants) {
-        float vbest = 1e30f;
-        for (int rep = 0; rep < 3; ++rep) {
-            DG_HIP(hipEventRecord(e0, st));
-            int rc = plan_launch(p, v, st);
-            if (rc) return rc;
-            DG_HIP(hipEventRecord(e1, st));
-            DG_HIP(hipEventSynchronize(e1));
-            float ms = 0.f;
-            DG_HIP(hipEventElapsedTime(&ms, e0, e1));
-            vbest = std::min(vbest, ms);
+    int best = DG_GEMM_MODE_XCD;  // encoded cfg * 4 + scheduling mode
+    for (int cfg : cfgs)
+        for (int v : variants) {
+            float vbest = 1e30f;
+            for (int rep = 0; rep < 3; ++rep) {
+                DG_HIP(hipEventRecord(e0, st));
+                int rc = plan_launch(p, v, cfg, st);
+                if (rc) return rc;
+                DG_HIP(hipEventRecord(e1, st));
+                DG_HIP(hipEventSynchronize(e1));
+                float ms = 0.f;
+                DG_HIP(hipEventElapsedTime(&ms, e0, e1));
+                vbest = std::min(vbest, ms);
+            }
+            if (vbest < best_ms) { best_ms = vbest; best = cfg * 4 + v; }
         }
-        if (vbest < best_ms) { best_ms = vbest; best = v; }
-    }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (snap) {
@@ -195,7 +198,8 @@ int autotune(dg_grouped_plan* p, hipStream_t st) {
         DG_HIP(hipStreamSynchronize(st));
         (void)hipFree(snap);
     }
-    p->variant = best;
+    p->variant = best % 4;
+    p->cfg = best / 4;
     std::lock_guard<std::mutex> lock(c.mu);
     tune_store_locked(c, p->shape_key, best);
     return DG_OK;
@@ -247,6 +251,7 @@ int dg_grouped_plan_create(const dg_gemm_desc* descs, int count, int mode, dg_gr
     p->shape_key = shape_class(descs, count);
     if (mode == DG_PLAN_NONPERSISTENT) p->variant = DG_GEMM_MODE_ROUND_ROBIN;
     if (mode == DG_PLAN_PERSISTENT) p->variant = DG_GEMM_MODE_PERSISTENT;
+    p->cfg = default_tile_cfg();
     auto cleanup = [&](int code) { dg_grouped_plan_destroy(p); return code; };
     if (hipMalloc(&p->counter, sizeof(int)) != hipSuccess) return cleanup(fail(DG_ERR_ALLOC, "plan: counter alloc"));
     for (int i = 0; i < count; ++i)
@@ -257,15 +262,16 @@ int dg_grouped_plan_create(const dg_gemm_desc* descs, int count, int mode, dg_gr
         std::vector<Seg> segs;
         build_group(descs, kv.second, probs0, segs);
         bool vec = true;
-        int tiles = 0;
-        rc = prepare_gemm(probs0, segs, probs, &vec, &tiles);
+        int tiles[2] = {0, 0};
+        rc = prepare_gemm(probs0, segs, probs, &vec, tiles);
         if (rc) return cleanup(rc);
         if (probs.empty()) continue;
         dg_grouped_plan::Group g;
         g.opA = kv.first.first;
         g.opB = kv.first.second;
         g.vec = vec;
-        g.tiles = tiles;
+        g.tiles[0] = tiles[0];
+        g.tiles[1] = tiles[1];
         g.np = (int)probs.size();
         if (hipMalloc(&g.dprobs, probs.size() * sizeof(Prob)) != hipSuccess ||
             hipMalloc(&g.dsegs, segs.size() * sizeof(Seg)) != hipSuccess) {
@@ -289,7 +295,7 @@ int dg_grouped_plan_execute(dg_grouped_plan* p, void* stream) {
         int rc = autotune(p, st);
         if (rc) return rc;
     }
-    return plan_launch(p, p->variant, st);
+    return plan_launch(p, p->variant, p->cfg, st);
 }
 
 int dg_grouped_plan_graph_launch(dg_grouped_plan* p, void* stream) {
@@ -300,13 +306,13 @@ int dg_grouped_plan_graph_launch(dg_grouped_plan* p, void* stream) {
         int rc = autotune(p, st);
         if (rc) return rc;
     }
-    if (!p->graph || p->graph_variant != p->variant) {
+    if (!p->graph || p->graph_variant != p->variant * 4 + p->cfg) {
         if (p->graph) { (void)hipGraphExecDestroy(p->graph); p->graph = nullptr; }
         // Capture on a private stream (the legacy default stream cannot be
         // captured); the graph is then launched on the caller's stream.
         if (!p->capture_stream) DG_HIP(hipStreamCreateWithFlags(&p->capture_stream, hipStreamNonBlocking));
         DG_HIP(hipStreamBeginCapture(p->capture_stream, hipStreamCaptureModeThreadLocal));
-        int rc = plan_launch(p, p->variant, p->capture_stream);
+        int rc = plan_launch(p, p->variant, p->cfg, p->capture_stream);
         hipGraph_t graph = nullptr;
         hipError_t e = hipStreamEndCapture(p->capture_stream, &graph);
         if (rc) { if (graph) (void)hipGraphDestroy(graph); return rc; }
@@ -314,7 +320,7 @@ int dg_grouped_plan_graph_launch(dg_grouped_plan* p, void* stream) {
         e = hipGraphInstantiate(&p->graph, graph, nullptr, nullptr, 0);
         (void)hipGraphDestroy(graph);
         if (e != hipSuccess) { p->graph = nullptr; return fail(DG_ERR_HIP, "plan instantiate: %s", hipGetErrorString(e)); }
-        p->graph_variant = p->variant;
+        p->graph_variant = p->variant * 4 + p->cfg;
     }
     DG_HIP(hipGraphLaunch(p->graph, st));
     return DG_OK;
