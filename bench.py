@@ -29,6 +29,9 @@ WORKLOADS = {
     "summa": "FP64 SUMMA 4x4 virtual grid, M=N=K=16384, block-cyclic nb=512, alpha=1, beta=0.5",
     "cannon": "FP64 2.5D Cannon 4x4x2 virtual grid, M=16384 N=8192 K=32768, opA=N opB=N, layer reduce in HBM",
     "dgemm": "FP64 DGEMM NN M=N=K=16384, alpha=1, beta=0",
+    "grouped": "FP64 grouped batch: 7x5 uneven tiles of M=N=K=20000 + 256 GEMMs with log-uniform M,N,K in "
+               "[64,4096], one autotuned plan replayed as a HIP graph",
+    "cannon2x2": "FP64 2D Cannon 2x2x1 virtual grid, M=N=K=2048, 1024^2 tiles, alpha=1, beta=0",
 }
 
 
@@ -103,6 +106,57 @@ def setup(workload, dev):
 
         cfg = {"workload": WORKLOADS[workload], "M": M, "N": N, "K": K, "grid": "4x4x2", "alpha": 1.0, "beta": 0.0}
         return step, (M, N, K), cfg, "cannon16384x8192x32768"
+    if workload == "grouped":
+        import numpy as np
+        n = 20000
+        drow, dcol = dgamd.virtual_grid(35)
+        ro, co = dgamd.block_splits(n, drow), dgamd.block_splits(n, dcol)
+        A = torch.empty(n * n, **f64)
+        B = torch.empty(n * n, **f64)
+        C = torch.empty(n * n, **f64)
+        dgamd.generate_t(21, 0, A)
+        dgamd.generate_t(22, 0, B)
+        descs, keep = [], [A, B, C]
+        flops = 2.0 * n ** 3
+        for i in range(drow):
+            for j in range(dcol):
+                descs.append(dgamd.gemm_desc("N", "N", ro[i + 1] - ro[i], co[j + 1] - co[j], n, 1.0,
+                                             A.data_ptr() + 8 * ro[i], n, B.data_ptr() + 8 * co[j] * n, n, 0.0,
+                                             C.data_ptr() + 8 * (ro[i] + co[j] * n), n))
+        rng = np.random.default_rng(5)
+        for t in range(256):
+            m_, n_, k_ = (int(round(np.exp(rng.uniform(np.log(64), np.log(4096))))) for _ in range(3))
+            Ae, Be, Ce = torch.empty(m_ * k_, **f64), torch.empty(k_ * n_, **f64), torch.empty(m_ * n_, **f64)
+            dgamd.generate_t(100 + 3 * t, 0, Ae)
+            dgamd.generate_t(101 + 3 * t, 0, Be)
+            keep += [Ae, Be, Ce]
+            descs.append(dgamd.gemm_desc("N", "N", m_, n_, k_, 1.0, Ae, m_, Be, k_, 0.0, Ce, m_))
+            flops += 2.0 * m_ * n_ * k_
+        plan = dgamd.GroupedPlan(descs, dgamd.DG_PLAN_AUTOTUNE)
+        plan.execute()  # autotunes once
+
+        def step():
+            plan.graph_launch()
+
+        step.keep = (keep, plan)
+        cfg = {"workload": WORKLOADS[workload], "problems": len(descs), "flop_per_step": flops}
+        return step, (flops / 2.0, 1, 1), cfg, "grouped20000"
+    if workload == "cannon2x2":
+        M = N = K = 2048
+        p = 2
+        bufs = []
+        for seed in (1, 2):
+            G = torch.empty(M * M, **f64)
+            dgamd.generate_t(seed, 0, G)
+            Pk = torch.empty_like(G)
+            dgamd.pack_block2d_t(G, M, M, M, dgamd.block_splits(M, p), dgamd.block_splits(M, p), Pk)
+            bufs.append(Pk)
+        Cp = torch.empty(M * N, **f64)
+
+        def step():
+            dgamd.cannon25d_t("N", "N", M, N, K, 1.0, bufs[0], bufs[1], 0.0, Cp, p, 1, dgamd.DG_SCHED_REFERENCE)
+
+        return step, (M, N, K), {"workload": WORKLOADS[workload], "M": M, "N": N, "K": K, "grid": "2x2x1"}, "cannon2048"
     n = 16384
     A = torch.empty(n * n, **f64)
     B = torch.empty(n * n, **f64)
