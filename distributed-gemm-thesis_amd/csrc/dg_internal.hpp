@@ -89,6 +89,7 @@ enum {
 enum {
     DG_GEMM_TILE_128x128 = 0,  // 4 waves, two workgroups per CU
     DG_GEMM_TILE_256x128 = 1,  // 8 waves, one workgroup per CU
+    DG_GEMM_TILE_128x128W8 = 2,  // 8 waves of 64x32, two workgroups per CU
     DG_GEMM_TILE_DEFAULT = DG_GEMM_TILE_128x128,
 };
 

@@ -46,17 +46,20 @@ constexpr int LDK = BK + 2;  // padded LDS row stride (doubles)
 constexpr int GROUP_M = 8;   // tile rows per raster group
 
 // Tile shapes: BM x BN output tiles computed by NTH threads, each wave owning
-// a 64 x 64 sub-tile (4 x 4 MFMA tiles of 16 x 16); OCC workgroups per CU.
-template <int BM_, int BN_, int NTH_, int OCC_>
+// a WM x WN sub-tile (WM/16 x WN/16 MFMA tiles of 16 x 16); OCC workgroups
+// per CU.
+template <int BM_, int BN_, int NTH_, int OCC_, int WM_ = 64, int WN_ = 64>
 struct TileCfg {
-    static constexpr int BM = BM_, BN = BN_, NTH = NTH_, OCC = OCC_;
-    static constexpr int WAVES_N = BN / 64;
+    static constexpr int BM = BM_, BN = BN_, NTH = NTH_, OCC = OCC_, WM = WM_, WN = WN_;
+    static constexpr int WAVES_N = BN / WN;
+    static constexpr int FM = WM / 16, FN = WN / 16;
     static constexpr int PA = BM * BK / (2 * NTH);  // 128-bit loads per thread per k-step (A)
     static constexpr int PB = BN * BK / (2 * NTH);  // (B)
-    static_assert((BM / 64) * (BN / 64) * 64 == NTH, "one wave per 64x64 sub-tile");
+    static_assert((BM / WM) * (BN / WN) * 64 == NTH, "one wave per WM x WN sub-tile");
 };
 using Cfg128 = TileCfg<128, 128, 256, 2>;  // 73.7 KB LDS, two workgroups per CU
 using Cfg256 = TileCfg<256, 128, 512, 1>;  // 110.6 KB LDS, 8 waves, half the B traffic per flop
+using Cfg128w8 = TileCfg<128, 128, 512, 2, 64, 32>;  // 8 waves of 64 x 32: four waves per SIMD
 
 // Block id -> tile id so that each XCD (blocks are dealt round-robin to the
 // 8 XCDs) receives one contiguous range of tiles, which the grouped raster
@@ -208,16 +211,17 @@ __device__ __forceinline__ void gemm_tile(const Prob* __restrict__ probs, int np
 
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    const int wm = (wave / C::WAVES_N) * 64;
-    const int wn = (wave % C::WAVES_N) * 64;
+    const int wm = (wave / C::WAVES_N) * C::WM;
+    const int wn = (wave % C::WAVES_N) * C::WN;
     const int lr = lane & 15;
     const int lg = lane >> 4;
 
-    dv4 acc[4][4];
+    constexpr int FM = C::FM, FN = C::FN;
+    dv4 acc[FM][FN];
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < FM; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = dv4{0.0, 0.0, 0.0, 0.0};
+        for (int j = 0; j < FN; ++j) acc[i][j] = dv4{0.0, 0.0, 0.0, 0.0};
 
     int s = P.seg0;
     const int s_end = P.seg0 + P.nseg;
@@ -264,19 +268,19 @@ __device__ __forceinline__ void gemm_tile(const Prob* __restrict__ probs, int np
         // reads, one per half of the k-step, to keep register pressure low.
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
-            dv2 af[4], bf[4];
+            dv2 af[FM], bf[FN];
 #pragma unroll
-            for (int i = 0; i < 4; ++i)
+            for (int i = 0; i < FM; ++i)
                 af[i] = *reinterpret_cast<const dv2*>(&as[(wm + 16 * i + lr) * LDK + 4 * lg + 2 * h]);
 #pragma unroll
-            for (int j = 0; j < 4; ++j)
+            for (int j = 0; j < FN; ++j)
                 bf[j] = *reinterpret_cast<const dv2*>(&bs[(wn + 16 * j + lr) * LDK + 4 * lg + 2 * h]);
 #pragma unroll
             for (int q = 0; q < 2; ++q)
 #pragma unroll
-                for (int i = 0; i < 4; ++i)
+                for (int i = 0; i < FM; ++i)
 #pragma unroll
-                    for (int j = 0; j < 4; ++j)
+                    for (int j = 0; j < FN; ++j)
                         acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bf[j][q], af[i][q], acc[i][j], 0, 0, 0);
         }
 
@@ -295,10 +299,10 @@ __device__ __forceinline__ void gemm_tile(const Prob* __restrict__ probs, int np
     // acc[i][j][v] = sum for C(m0 + wm + 16i + lr, n0 + wn + 16j + lg + 4v).
     const double alpha = P.alpha, beta = P.beta;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
+    for (int i = 0; i < FM; ++i) {
         const int64_t m = m0 + wm + 16 * i + lr;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
+        for (int j = 0; j < FN; ++j) {
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int64_t n = n0 + wn + 16 * j + lg + 4 * v;
@@ -432,7 +436,7 @@ int persistent_grid(int cfg) {
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
         cus = 256;
-    return cus * (cfg == DG_GEMM_TILE_256x128 ? Cfg256::OCC : Cfg128::OCC);
+    return cus * (cfg == DG_GEMM_TILE_256x128 ? Cfg256::OCC : cfg == DG_GEMM_TILE_128x128W8 ? Cfg128w8::OCC : Cfg128::OCC);
 }
 
 int launch_gemm_f64_dev(char opA, char opB, bool vec, int mode, int cfg, const Prob* dp, int np, const Seg* ds,
@@ -447,6 +451,7 @@ int launch_gemm_f64_dev(char opA, char opB, bool vec, int mode, int cfg, const P
         hipLaunchKernelGGL(zero_counter_kernel, dim3(1), dim3(1), 0, stream, counter);
     }
     if (cfg == DG_GEMM_TILE_256x128) launch_cfg<Cfg256>(opA, opB, vec, mode, dp, np, ds, t, grid, counter, stream);
+    else if (cfg == DG_GEMM_TILE_128x128W8) launch_cfg<Cfg128w8>(opA, opB, vec, mode, dp, np, ds, t, grid, counter, stream);
     else launch_cfg<Cfg128>(opA, opB, vec, mode, dp, np, ds, t, grid, counter, stream);
     DG_LAUNCH_CHECK();
     return DG_OK;
@@ -457,6 +462,7 @@ int default_tile_cfg() {
         const char* e = getenv("DG_GEMM_TILE");
         if (e && std::string(e) == "256x128") return (int)DG_GEMM_TILE_256x128;
         if (e && std::string(e) == "128x128") return (int)DG_GEMM_TILE_128x128;
+        if (e && std::string(e) == "128x128w8") return (int)DG_GEMM_TILE_128x128W8;
         return (int)DG_GEMM_TILE_DEFAULT;
     }();
     return cfg;

@@ -20,14 +20,13 @@
 //      so that the CPU oracle reproduces every bit.
 //
 // Slices are stored k-contiguous, [t][row][Kp] with rows padded to 256 and K
-// padded to one 128-byte k-step with zeros, so the GEMM kernel has no edge
-// cases until the final store.  The GEMM kernel uses 256x256 output tiles,
-// 8 waves of 128x64 (8x4 MFMA tiles of 16x16, 128 accumulator registers per
-// lane), 128-byte k-steps double buffered in LDS (144 KB) with a 144-byte row
-// stride (conflict-free 128-bit fragment reads).  The FP64 partial sums W
-// live in HBM (workspace): a flush is one coalesced read-modify-write of the
-// wave's 128x128 block, a handful per tile, so the registers hold only the
-// 32-bit accumulators.
+// padded to 128 bytes with zeros, so the product kernel has no edge cases
+// until the final store.  The product kernel uses 256x256 output tiles, 4
+// waves of 128x128 (256 accumulator registers per lane) and a 4-stage LDS
+// ring of 64-byte k-steps filled by direct global-to-LDS loads.  The FP64
+// partial sums W live in HBM (workspace): a flush is one coalesced
+// read-modify-write of the wave's 128x128 block, a handful per tile, so the
+// registers hold only the 32-bit accumulators.
 #include "dg_internal.hpp"
 
 // Every FP64 operation of the recombination is rounded on its own (no fused
@@ -43,19 +42,16 @@ typedef float fv4 __attribute__((ext_vector_type(4)));
 typedef double dv4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 
-constexpr int OT = 256;          // output tile (rows and columns)
-constexpr int WTM = 128;         // wave tile rows
-constexpr int WTN = 64;          // wave tile columns
-constexpr int FRM = WTM / 16;    // 16x16 MFMA tiles per wave, rows
-constexpr int FRN = WTN / 16;    // 16x16 MFMA tiles per wave, columns
-constexpr int NTG = 512;         // threads of the slice-product kernel (8 waves, 2 x 4)
-constexpr int KSB = 128;         // k-step in bytes
-constexpr int LDB = KSB + 16;    // LDS row stride in bytes
-constexpr int LQ = OT * (KSB / 16) / NTG;  // 16-byte chunks per thread per operand and k-step
+constexpr int OT = 256;      // output tile (rows and columns)
+constexpr int KSB = 128;     // K padding granule of the slices and residues, bytes
+constexpr int CK = 64;       // k-step in bytes
+constexpr int CSTG = 4;      // LDS stages
+constexpr int CNT = 256;     // threads of the product kernels: 4 waves of 128x128
+constexpr int FR = 8;        // 16x16 MFMA tiles per wave and dimension
+constexpr int CSTAGE = 2 * OT * CK;  // bytes per stage (A rows, then B rows)
+constexpr size_t kGemmLds = (size_t)CSTG * CSTAGE;
 constexpr int NT = 256;
 constexpr int GROUP_M = 8;
-
-typedef __attribute__((address_space(1))) const iv4 giv4;
 
 __device__ __forceinline__ int xcd_remap(int bid, int total) {
     constexpr int X = 8;
@@ -90,6 +86,7 @@ struct OzArgs {
     double* c;
     int64_t ldc;
     int32_t tiles_m, tiles_n;
+    int64_t nsteps;             // k-steps over all products
 };
 
 template <bool BF16>
@@ -102,63 +99,113 @@ __device__ __forceinline__ void mma(iv4& acc_i, fv4& acc_f, const iv4& x, const 
     }
 }
 
-// Loads one 256-row x 128-byte k-step: thread t takes 16-byte chunk t%8 of
-// rows t/8 + 64*q.
-__device__ __forceinline__ void oz_load(iv4 (&r)[LQ], const int8_t* __restrict__ base, int64_t row_bytes,
-                                        int64_t row0, int64_t kbyte0) {
-    const int t = threadIdx.x;
-    const int ch = t & 7;
-    const int rr = t >> 3;
+// The product kernels' pipeline.  256x256 output tiles, 4 waves of 128x128
+// (8x8 MFMA tiles of 16x16: 256 accumulator registers per lane), 64-byte
+// k-steps in a 4-stage LDS ring (4 x 32 KB) filled by global_load_lds_dwordx4,
+// which writes LDS without a register round trip.  One such load of a wave
+// covers 16 rows x 64 bytes; within a row, 16-byte chunk c sits at position
+// c ^ swz(row) so that the 128-bit fragment reads of 8 or 16 consecutive rows
+// hit distinct banks.  Each wave waits for its own loads of step k (vmcnt),
+// then one barrier makes every wave's loads visible and frees the stage read
+// at step k-1, which then receives step k+3.
+__host__ __device__ constexpr int swz(int r) { return (((r >> 2) & 1) << 1) | (((r >> 1) ^ (r >> 3)) & 1); }
+
+template <int N>
+__device__ __forceinline__ void wait_vmcnt() {
+    static_assert(N >= 0 && N < 64, "vmcnt field");
+    // gfx9 s_waitcnt: vmcnt in bits 3:0 and 15:14; expcnt and lgkmcnt left at their maxima
+    __builtin_amdgcn_s_waitcnt((N & 0xF) | ((N >> 4) << 14) | (0x7 << 4) | (0xF << 8));
+}
+
+// Waits until this wave's loads of the current step have landed, given the
+// number of later steps already issued (8 loads each).
+__device__ __forceinline__ void wait_step(int64_t ahead) {
+    if (ahead >= 2) wait_vmcnt<16>();
+    else if (ahead == 1) wait_vmcnt<8>();
+    else wait_vmcnt<0>();
+}
+
+typedef __attribute__((address_space(3))) void lds_void_t;
+typedef __attribute__((address_space(1))) const void gvoid_t;
+
+// Issues one k-step of both operands into stage `stage`: this wave's rows
+// 64w .. 64w+63, 16 rows per load; pa/pb point at this lane's first row and
+// chunk of the step, rows `rb` bytes apart.
+__device__ __forceinline__ void issue_step(int8_t* lds, int stage, int wave, const int8_t* pa, const int8_t* pb,
+                                           int64_t rb) {
+    int8_t* sa = lds + stage * CSTAGE + wave * 4096;
+    int8_t* sb = sa + OT * CK;
 #pragma unroll
-    for (int q = 0; q < LQ; ++q) {
-        const int8_t* p = base + (row0 + rr + (NTG / 8) * q) * row_bytes + kbyte0 + ch * 16;
-        r[q] = *(giv4*)p;
+    for (int i = 0; i < 4; ++i) {
+        __builtin_amdgcn_global_load_lds((gvoid_t*)(pa + 16 * i * rb), (lds_void_t*)(sa + 1024 * i), 16, 0, 0);
+        __builtin_amdgcn_global_load_lds((gvoid_t*)(pb + 16 * i * rb), (lds_void_t*)(sb + 1024 * i), 16, 0, 0);
     }
 }
 
-__device__ __forceinline__ void oz_store(int8_t* __restrict__ s, const iv4 (&r)[LQ]) {
-    const int t = threadIdx.x;
-    const int ch = t & 7;
-    const int rr = t >> 3;
+// 64 MFMAs of one stage: as/bs point at this lane's fragment of the wave's
+// first 16-row block of A and B.
+template <bool BF16>
+__device__ __forceinline__ void stage_mma(iv4 (&acci)[FR][FR], fv4 (&accf)[FR][FR], const int8_t* as,
+                                          const int8_t* bs) {
+    iv4 bfr[FR];
 #pragma unroll
-    for (int q = 0; q < LQ; ++q) *reinterpret_cast<iv4*>(s + (rr + (NTG / 8) * q) * LDB + ch * 16) = r[q];
+    for (int j = 0; j < FR; ++j) bfr[j] = *reinterpret_cast<const iv4*>(bs + 1024 * j);
+#pragma unroll
+    for (int i = 0; i < FR; ++i) {
+        const iv4 af = *reinterpret_cast<const iv4*>(as + 1024 * i);
+#pragma unroll
+        for (int j = 0; j < FR; ++j) mma<BF16>(acci[i][j], accf[i][j], bfr[j], af);
+    }
 }
 
-// Flush kinds: first flush writes W = x, later ones W += x, the last one
-// writes C = alpha * ldexp(W + x, e_i + f_j) + beta * C (FL_ONLY: the first
-// is also the last, C from x alone).  x = acc * 2^-(bits*level) is exact.
-enum { FL_FIRST = 0, FL_ADD = 1, FL_LAST = 2, FL_ONLY = 3 };
+// Output tile of workgroup slot `bid` among `tiles`: XCD-aware, GROUP_M tile
+// rows sweep the tile columns together.
+__device__ __forceinline__ void tile_origin(int bid, int tiles, int tiles_m, int tiles_n, int64_t& m0, int64_t& n0) {
+    const int tile = xcd_remap(bid, tiles);
+    const int per_group = GROUP_M * tiles_n;
+    const int group = tile / per_group;
+    const int first_m = group * GROUP_M;
+    const int gsize = min(tiles_m - first_m, GROUP_M);
+    const int in_group = tile - group * per_group;
+    m0 = (int64_t)(first_m + in_group % gsize) * OT;
+    n0 = (int64_t)(in_group / gsize) * OT;
+}
+
+// Flushes: the first one writes W = x, later ones W += x, the last one
+// writes C = alpha * ldexp(W + x, e_i + f_j) + beta * C (or C from x alone
+// when the first flush is also the last).  x = acc * 2^-(bits*level) is exact.
 
 // Register v of lane group lg holds C(mrow + 16i, ncol + 16j + v) with
 // mrow = m0 + wm + l%16, ncol = n0 + wn + 4*(l/16): the 16x16 i32/f32 result
 // layout D(4*(l/16) + v, l%16) of D = B^T A^T.
-template <int KIND, bool BF16, bool GUARD>
-__device__ __forceinline__ void oz_flush(const iv4 (&acci)[FRM][FRN], const fv4 (&accf)[FRM][FRN], const OzArgs& g,
-                                         double w, int64_t mrow, int64_t ncol) {
+// kind: 2 first flush (W = x), 1 later flush (W += x), 3 last flush
+// (C from W + x), 4 first and last (C from x alone); guard: the tile crosses
+// the edge of C.  Every branch is uniform across the wave.
+template <bool BF16>
+__device__ __forceinline__ void oz_flush(const iv4 (&acci)[FR][FR], const fv4 (&accf)[FR][FR], const OzArgs& g,
+                                         double w, int64_t mrow, int64_t ncol, int kind, bool guard) {
+    const bool to_c = kind >= 3;
+    const bool read_w = kind == 1 || kind == 3;
 #pragma unroll
-    for (int j = 0; j < FRN; ++j) {
+    for (int j = 0; j < FR; ++j) {
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             const int64_t n = ncol + 16 * j + v;
             double* wcol = g.w + n * g.Mp + mrow;
             double* ccol = g.c + n * g.ldc + mrow;
-            const int ef = (KIND == FL_LAST || KIND == FL_ONLY) && (!GUARD || n < g.n) ? g.eb[n] : 0;
+            const int ef = to_c && n < g.n ? g.eb[n] : 0;
 #pragma unroll
-            for (int i = 0; i < FRM; ++i) {
+            for (int i = 0; i < FR; ++i) {
                 double x;
                 if constexpr (BF16) x = (double)accf[i][j][v] * w;
                 else x = (double)acci[i][j][v] * w;
-                if constexpr (KIND == FL_FIRST) {
-                    wcol[16 * i] = x;
-                } else if constexpr (KIND == FL_ADD) {
-                    wcol[16 * i] = __dadd_rn(wcol[16 * i], x);
-                } else {
-                    if (!GUARD || (mrow + 16 * i < g.m && n < g.n)) {
-                        const double acc = KIND == FL_LAST ? __dadd_rn(wcol[16 * i], x) : x;
-                        double val = __dmul_rn(g.alpha, ldexp(acc, g.ea[mrow + 16 * i] + ef));
-                        if (g.beta != 0.0) val = __dadd_rn(val, __dmul_rn(g.beta, ccol[16 * i]));
-                        ccol[16 * i] = val;
-                    }
+                const double acc = read_w ? __dadd_rn(wcol[16 * i], x) : x;
+                if (!to_c) {
+                    wcol[16 * i] = acc;
+                } else if (!guard || (mrow + 16 * i < g.m && n < g.n)) {
+                    double val = __dmul_rn(g.alpha, ldexp(acc, g.ea[mrow + 16 * i] + ef));
+                    if (g.beta != 0.0) val = __dadd_rn(val, __dmul_rn(g.beta, ccol[16 * i]));
+                    ccol[16 * i] = val;
                 }
             }
             // one column group's read-modify-write at a time
@@ -167,122 +214,85 @@ __device__ __forceinline__ void oz_flush(const iv4 (&acci)[FRM][FRN], const fv4 
     }
 }
 
+// Slice products of one 256x256 output tile: the k-steps of every product of
+// the schedule run back to back through the LDS ring; at the end of a product
+// marked for flushing the 32-bit sums go into W (or, at the last flush, C).
 template <bool BF16>
-__global__ __launch_bounds__(NTG, 1) void oz_gemm_kernel(OzArgs g) {
-    // LDS: [A buffer 0][A buffer 1][B buffer 0][B buffer 1], OT rows of LDB bytes each.
-    extern __shared__ __attribute__((aligned(16))) int8_t lds[];
-    auto As = [&](int b) { return lds + b * (OT * LDB); };
-    auto Bs = [&](int b) { return lds + (2 + b) * (OT * LDB); };
-
-    const int total = g.tiles_m * g.tiles_n;
-    const int tile = xcd_remap(blockIdx.x, total);
-    const int per_group = GROUP_M * g.tiles_n;
-    const int group = tile / per_group;
-    const int first_m = group * GROUP_M;
-    const int gsize = min(g.tiles_m - first_m, GROUP_M);
-    const int in_group = tile - group * per_group;
-    const int64_t m0 = (int64_t)(first_m + in_group % gsize) * OT;
-    const int64_t n0 = (int64_t)(in_group / gsize) * OT;
+__global__ __launch_bounds__(CNT, 1) void oz_gemm_kernel(OzArgs g) {
+    extern __shared__ __attribute__((aligned(1024))) int8_t lds[];
+    int64_t m0, n0;
+    tile_origin(blockIdx.x, g.tiles_m * g.tiles_n, g.tiles_m, g.tiles_n, m0, n0);
 
     const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
-    const int wm = (wave >> 2) * WTM;
-    const int wn = (wave & 3) * WTN;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int wm = (wave >> 1) * 128;
+    const int wn = (wave & 1) * 128;
     const int lr = lane & 15;
     const int lg = lane >> 4;
 
     const int esz = BF16 ? 2 : 1;
-    const int64_t row_bytes = g.Kp * esz;
+    const int64_t rb = g.Kp * esz;  // bytes per slice row
+    const int lrow = lane >> 2;
+    const int lchunk = (lane & 3) ^ swz(lrow);
+    const int8_t* ga = g.a + (m0 + 64 * wave + lrow) * rb + 16 * lchunk;
+    const int8_t* gb = g.b + (n0 + 64 * wave + lrow) * rb + 16 * lchunk;
+    const int roff = lr * CK + 16 * (lg ^ swz(lr));
 
-    iv4 acci[FRM][FRN];
-    fv4 accf[FRM][FRN];
+    // issue cursor: product iseg, byte offset ikb within it
+    int iseg = 0;
+    int64_t ikb = 0;
+    OzSeg is = g.segs[0];
+    int64_t istep = 0;
+    auto issue_next = [&]() {
+        const int64_t koff = (int64_t)is.k0 * esz + ikb;
+        issue_step(lds, (int)(istep & (CSTG - 1)), wave, ga + is.t * g.a_slice + koff, gb + is.u * g.b_slice + koff,
+                   rb);
+        ++istep;
+        ikb += CK;
+        if (ikb >= (int64_t)is.klen * esz) {
+            ikb = 0;
+            if (++iseg < g.nseg) is = g.segs[iseg];
+        }
+    };
+
+    iv4 acci[FR][FR];
+    fv4 accf[FR][FR];
 #pragma unroll
-    for (int i = 0; i < FRM; ++i)
+    for (int i = 0; i < FR; ++i)
 #pragma unroll
-        for (int j = 0; j < FRN; ++j) {
+        for (int j = 0; j < FR; ++j) {
             if constexpr (BF16) accf[i][j] = fv4{0.f, 0.f, 0.f, 0.f};
             else acci[i][j] = iv4{0, 0, 0, 0};
         }
 
-    int sidx = 0;
-    OzSeg cur = g.segs[0];
-    int64_t kb = 0;  // byte offset within the current segment
-
-    iv4 ra[LQ], rb[LQ];
-    oz_load(ra, g.a + cur.t * g.a_slice, row_bytes, m0, (int64_t)cur.k0 * esz);
-    oz_load(rb, g.b + cur.u * g.b_slice, row_bytes, n0, (int64_t)cur.k0 * esz);
-    int buf = 0;
-    oz_store(As(0), ra);
-    oz_store(Bs(0), rb);
-    __syncthreads();
-
-    while (true) {
-        int64_t nkb = kb + KSB;
-        int ns = sidx;
-        OzSeg nxt = cur;
-        if (nkb >= (int64_t)cur.klen * esz) {
-            ++ns;
-            nkb = 0;
-            if (ns < g.nseg) nxt = g.segs[ns];
+    const int64_t nsteps = g.nsteps;
+    issue_next();
+    if (nsteps > 1) issue_next();
+    if (nsteps > 2) issue_next();
+    const int64_t mrow = m0 + wm + lr;
+    const int64_t ncol = n0 + wn + 4 * lg;
+    const bool full = (m0 + OT <= g.m) && (n0 + OT <= g.n);
+    int64_t kt = 0;
+    for (int cseg = 0; cseg < g.nseg; ++cseg) {
+        const OzSeg cs = g.segs[cseg];
+        const int64_t kend = kt + (int64_t)cs.klen * esz / CK;
+        for (; kt < kend; ++kt) {
+            wait_step(nsteps - 1 - kt);
+            __builtin_amdgcn_s_barrier();
+            if (kt + 3 < nsteps) issue_next();
+            const int8_t* st = lds + (int)(kt & (CSTG - 1)) * CSTAGE;
+            stage_mma<BF16>(acci, accf, st + wm * CK + roff, st + OT * CK + wn * CK + roff);
         }
-        const bool has_next = ns < g.nseg;
-        if (has_next) {
-            oz_load(ra, g.a + nxt.t * g.a_slice, row_bytes, m0, (int64_t)nxt.k0 * esz + nkb);
-            oz_load(rb, g.b + nxt.u * g.b_slice, row_bytes, n0, (int64_t)nxt.k0 * esz + nkb);
-        }
-
-        const int8_t* as = As(buf);
-        const int8_t* bs = Bs(buf);
+        if (cs.flush) {
+            oz_flush<BF16>(acci, accf, g, ldexp(1.0, -g.bits * cs.level), mrow, ncol, cs.flush, !full);
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            iv4 bfr[FRN];
+            for (int i = 0; i < FR; ++i)
 #pragma unroll
-            for (int j = 0; j < FRN; ++j)
-                bfr[j] = *reinterpret_cast<const iv4*>(bs + (wn + 16 * j + lr) * LDB + h * 64 + lg * 16);
-#pragma unroll
-            for (int i = 0; i < FRM; ++i) {
-                const iv4 af = *reinterpret_cast<const iv4*>(as + (wm + 16 * i + lr) * LDB + h * 64 + lg * 16);
-#pragma unroll
-                for (int j = 0; j < FRN; ++j) mma<BF16>(acci[i][j], accf[i][j], bfr[j], af);
-            }
-        }
-
-        // End of the current product: flush the 32-bit sums into W (or C).
-        const bool seg_end = (kb + KSB >= (int64_t)cur.klen * esz);
-        if (seg_end && cur.flush) {
-            const double w = ldexp(1.0, -g.bits * cur.level);
-            const int64_t mrow = m0 + wm + lr;
-            const int64_t ncol = n0 + wn + 4 * lg;
-            const bool full = (m0 + OT <= g.m) && (n0 + OT <= g.n);
-            switch (cur.flush) {
-                case 2: oz_flush<FL_FIRST, BF16, false>(acci, accf, g, w, mrow, ncol); break;
-                case 1: oz_flush<FL_ADD, BF16, false>(acci, accf, g, w, mrow, ncol); break;
-                case 3:
-                    if (full) oz_flush<FL_LAST, BF16, false>(acci, accf, g, w, mrow, ncol);
-                    else      oz_flush<FL_LAST, BF16, true>(acci, accf, g, w, mrow, ncol);
-                    break;
-                default:
-                    if (full) oz_flush<FL_ONLY, BF16, false>(acci, accf, g, w, mrow, ncol);
-                    else      oz_flush<FL_ONLY, BF16, true>(acci, accf, g, w, mrow, ncol);
-                    break;
-            }
-#pragma unroll
-            for (int i = 0; i < FRM; ++i)
-#pragma unroll
-                for (int j = 0; j < FRN; ++j) {
+                for (int j = 0; j < FR; ++j) {
                     if constexpr (BF16) accf[i][j] = fv4{0.f, 0.f, 0.f, 0.f};
                     else acci[i][j] = iv4{0, 0, 0, 0};
                 }
         }
-
-        if (!has_next) break;
-        buf ^= 1;
-        oz_store(As(buf), ra);
-        oz_store(Bs(buf), rb);
-        __syncthreads();
-        sidx = ns;
-        kb = nkb;
-        cur = nxt;
     }
 }
 
@@ -292,22 +302,34 @@ __global__ __launch_bounds__(NTG, 1) void oz_gemm_kernel(OzArgs g) {
 // (2025).  A' = trunc(op(A)_ik * 2^(q - e_i)) and B' likewise are integers
 // below 2^q; V = A'B' is computed modulo N pairwise coprime moduli p_t <= 256
 // (one INT8 GEMM each, residues balanced into [-128, 127]) and rebuilt
-// exactly by Garner's algorithm with balanced mixed-radix digits; q is chosen
-// so that k * 4^q <= P/4 (P = prod p_t), hence |V| < P/4 is recovered
-// without ambiguity.  C = alpha * ldexp(double(V), e_i + f_j - 2q) + beta*C
+// exactly by the Chinese remainder theorem, V = sum_t x_t (P/p_t) mod P,
+// balanced; q is chosen so that k * 4^q <= P/4 (P = prod p_t), hence
+// |V| < P/4 is recovered without ambiguity.  C = alpha * ldexp(double(V), e_i + f_j - 2q) + beta*C
 // with double(V) correctly rounded.
 
 constexpr int kMaxModuli = 16;
 // Pairwise coprime: 2^8, 3*5*17, 11*23, 13*19, 7*31 and primes.
 __constant__ int c_moduli[kMaxModuli] = {256, 255, 253, 251, 247, 241, 239, 233, 229, 227, 223, 217, 211, 199, 197, 193};
-constexpr int h_moduli[kMaxModuli] = {256, 255, 253, 251, 247, 241, 239, 233, 229, 227, 223, 217, 211, 199, 197, 193};
 
+
+// Direct CRT: V = (sum_t x_t * M_t) mod P with M_t = P / p_t and
+// x_t = r_t * (M_t^-1 mod p_t) mod p_t, then the balanced representative.
 struct CrtTables {
     int32_t n;                          // moduli in use
-    int32_t rinv[kMaxModuli];           // R_t^{-1} mod p_t
-    int16_t rmod[kMaxModuli][kMaxModuli];  // R_t mod p_u
-    uint64_t rhi[kMaxModuli], rlo[kMaxModuli];  // R_t = prod_{u<t} p_u as 128-bit
+    int32_t cinv[kMaxModuli];           // M_t^-1 mod p_t
+    uint32_t mt[kMaxModuli][4];         // M_t, 32-bit limbs, least significant first
+    uint32_t P[4];                      // P = prod p_t
+    uint32_t Ph[4];                     // floor(P / 2)
+    double Pd;                          // P rounded to double
 };
+
+// p_t as a constant expression, so that x % crt_modulus(t) in an unrolled
+// loop compiles to a multiply-shift.
+__host__ __device__ constexpr int crt_modulus(int t) {
+    return t == 0 ? 256 : t == 1 ? 255 : t == 2 ? 253 : t == 3 ? 251 : t == 4 ? 247 : t == 5 ? 241 : t == 6 ? 239
+         : t == 7 ? 233 : t == 8 ? 229 : t == 9 ? 227 : t == 10 ? 223 : t == 11 ? 217 : t == 12 ? 211
+         : t == 13 ? 199 : t == 14 ? 197 : 193;
+}
 
 // x mod p in [0, p) for an integer-valued double |x| < 2^53 (exact steps).
 __device__ __forceinline__ int mod_exact(double x, int p, double inv_p) {
@@ -341,14 +363,29 @@ __global__ void crt_residue_kernel(const double* __restrict__ x, int64_t rows, i
         }
     }
     __syncthreads();
-    for (int rr = ty; rr < 64; rr += 4) {
-        const int64_t r = r0 + rr, k = k0 + tx;
-        const double a = (r < rows) ? trunc(ldexp(tile[rr][tx], q - e[r])) : 0.0;
-        for (int t = 0; t < nmod; ++t) {
-            const int p = c_moduli[t];
-            int v = mod_exact(a, p, 1.0 / p);
-            if (2 * v >= p) v -= p;
-            out[t * slice_bytes + r * Kp + k] = (int8_t)v;
+    // thread: row t/4 of the tile, 16 consecutive k -> one 16-byte store per modulus
+    const int rr = threadIdx.x >> 2;
+    const int kq = (threadIdx.x & 3) * 16;
+    const int64_t r = r0 + rr;
+    double a[16];
+    const int sh = r < rows ? q - e[r] : 0;
+#pragma unroll
+    for (int c = 0; c < 16; ++c) a[c] = r < rows ? trunc(ldexp(tile[rr][kq + c], sh)) : 0.0;
+    int8_t* dst = out + r * Kp + k0 + kq;
+#pragma unroll
+    for (int t = 0; t < kMaxModuli; ++t) {
+        if (t < nmod) {
+            constexpr double one = 1.0;
+            const int p = crt_modulus(t);
+            const double inv_p = one / crt_modulus(t);
+            uint32_t w[4] = {0, 0, 0, 0};
+#pragma unroll
+            for (int c = 0; c < 16; ++c) {
+                int v = mod_exact(a[c], p, inv_p);
+                if (2 * v >= p) v -= p;
+                w[c >> 2] |= (uint32_t)(uint8_t)(int8_t)v << (8 * (c & 3));
+            }
+            *reinterpret_cast<iv4*>(dst + t * slice_bytes) = iv4{(int)w[0], (int)w[1], (int)w[2], (int)w[3]};
         }
     }
 }
@@ -363,153 +400,191 @@ struct CrtArgs {
 };
 
 // One INT8 GEMM per modulus in one launch: workgroup b computes tile
-// b % tiles of modulus b / tiles (256x256 tiles, 8 waves of 128x64, same
-// pipeline as oz_gemm_kernel) and stores (A'_t B'_t) mod p_t as bytes.
-__global__ __launch_bounds__(NTG, 1) void crt_gemm_kernel(CrtArgs g) {
-    extern __shared__ __attribute__((aligned(16))) int8_t lds[];
-    auto As = [&](int b) { return lds + b * (OT * LDB); };
-    auto Bs = [&](int b) { return lds + (2 + b) * (OT * LDB); };
-
+// b % tiles of modulus b / tiles through the product pipeline and stores
+// (A'_t B'_t) mod p_t as bytes.
+__global__ __launch_bounds__(CNT, 1) void crt_gemm_kernel(CrtArgs g) {
+    extern __shared__ __attribute__((aligned(1024))) int8_t lds[];
     const int tiles = g.tiles_m * g.tiles_n;
     const int tmod = blockIdx.x / tiles;
-    const int tile = xcd_remap(blockIdx.x - tmod * tiles, tiles);
-    const int per_group = GROUP_M * g.tiles_n;
-    const int group = tile / per_group;
-    const int first_m = group * GROUP_M;
-    const int gsize = min(g.tiles_m - first_m, GROUP_M);
-    const int in_group = tile - group * per_group;
-    const int64_t m0 = (int64_t)(first_m + in_group % gsize) * OT;
-    const int64_t n0 = (int64_t)(in_group / gsize) * OT;
+    int64_t m0, n0;
+    tile_origin(blockIdx.x - tmod * tiles, tiles, g.tiles_m, g.tiles_n, m0, n0);
 
     const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
-    const int wm = (wave >> 2) * WTM;
-    const int wn = (wave & 3) * WTN;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int wm = (wave >> 1) * 128;
+    const int wn = (wave & 1) * 128;
     const int lr = lane & 15;
     const int lg = lane >> 4;
 
-    const int8_t* abase = g.a + tmod * g.a_slice;
-    const int8_t* bbase = g.b + tmod * g.b_slice;
-    iv4 acc[FRM][FRN];
-    fv4 unused[FRM][FRN];
-#pragma unroll
-    for (int i = 0; i < FRM; ++i)
-#pragma unroll
-        for (int j = 0; j < FRN; ++j) acc[i][j] = iv4{0, 0, 0, 0};
+    const int lrow = lane >> 2;
+    const int lchunk = (lane & 3) ^ swz(lrow);
+    const int8_t* ga = g.a + tmod * g.a_slice + (m0 + 64 * wave + lrow) * g.Kp + 16 * lchunk;
+    const int8_t* gb = g.b + tmod * g.b_slice + (n0 + 64 * wave + lrow) * g.Kp + 16 * lchunk;
+    const int roff = lr * CK + 16 * (lg ^ swz(lr));
 
-    iv4 ra[LQ], rb[LQ];
-    oz_load(ra, abase, g.Kp, m0, 0);
-    oz_load(rb, bbase, g.Kp, n0, 0);
-    int buf = 0;
-    oz_store(As(0), ra);
-    oz_store(Bs(0), rb);
-    __syncthreads();
-    for (int64_t kb = 0; kb < g.Kp; kb += KSB) {
-        const bool has_next = kb + KSB < g.Kp;
-        if (has_next) {
-            oz_load(ra, abase, g.Kp, m0, kb + KSB);
-            oz_load(rb, bbase, g.Kp, n0, kb + KSB);
-        }
-        const int8_t* as = As(buf);
-        const int8_t* bs = Bs(buf);
+    iv4 acc[FR][FR];
+    fv4 unused[FR][FR];
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            iv4 bfr[FRN];
+    for (int i = 0; i < FR; ++i)
 #pragma unroll
-            for (int j = 0; j < FRN; ++j)
-                bfr[j] = *reinterpret_cast<const iv4*>(bs + (wn + 16 * j + lr) * LDB + h * 64 + lg * 16);
-#pragma unroll
-            for (int i = 0; i < FRM; ++i) {
-                const iv4 af = *reinterpret_cast<const iv4*>(as + (wm + 16 * i + lr) * LDB + h * 64 + lg * 16);
-#pragma unroll
-                for (int j = 0; j < FRN; ++j) mma<false>(acc[i][j], unused[i][j], bfr[j], af);
-            }
-        }
-        if (!has_next) break;
-        buf ^= 1;
-        oz_store(As(buf), ra);
-        oz_store(Bs(buf), rb);
-        __syncthreads();
+        for (int j = 0; j < FR; ++j) acc[i][j] = iv4{0, 0, 0, 0};
+
+    const int64_t nk = g.Kp / CK;
+    for (int64_t s = 0; s < 3 && s < nk; ++s) issue_step(lds, (int)s, wave, ga + s * CK, gb + s * CK, g.Kp);
+    for (int64_t kt = 0; kt < nk; ++kt) {
+        wait_step(nk - 1 - kt);
+        __builtin_amdgcn_s_barrier();
+        if (kt + 3 < nk)
+            issue_step(lds, (int)((kt + 3) & (CSTG - 1)), wave, ga + (kt + 3) * CK, gb + (kt + 3) * CK, g.Kp);
+        const int8_t* st = lds + (int)(kt & (CSTG - 1)) * CSTAGE;
+        stage_mma<false>(acc, unused, st + wm * CK + roff, st + OT * CK + wn * CK + roff);
     }
-    // acc[i][j][v] = V_t(m0 + wm + 16i + lr, n0 + wn + 16j + 4lg + v) (see oz_flush)
+    // acc[i][j][v] = V_t(m0 + wm + 16i + lr, n0 + wn + 16j + 4lg + v)
     const int p = c_moduli[tmod];
     const double inv_p = 1.0 / p;
     uint8_t* out = g.res + (int64_t)tmod * g.Mp * g.Np;
 #pragma unroll
-    for (int j = 0; j < FRN; ++j)
+    for (int j = 0; j < 8; ++j)
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             const int64_t n = n0 + wn + 16 * j + 4 * lg + v;
 #pragma unroll
-            for (int i = 0; i < FRM; ++i) {
+            for (int i = 0; i < 8; ++i) {
                 const int64_t m = m0 + wm + 16 * i + lr;
                 out[n * g.Mp + m] = (uint8_t)mod_exact((double)acc[i][j][v], p, inv_p);
             }
         }
 }
 
-// Garner's algorithm with balanced digits, then the exact 128-bit value and
-// its correctly rounded double; C = alpha * ldexp(V, e_i + f_j - 2q) + beta*C.
+// The balanced CRT value of one element from its limb sums
+// L[l] = sum_t x_t * limb_l(M_t), correctly rounded to double.
+__host__ __device__ inline double crt_value(const uint64_t (&Le)[4], const CrtTables& tb) {
+    // carry-propagate into 32-bit limbs s0..s4
+    uint32_t s0, s1, s2, s3, s4;
+    {
+        uint64_t c0 = Le[0];
+        s0 = (uint32_t)c0;
+        uint64_t c1 = Le[1] + (c0 >> 32);
+        s1 = (uint32_t)c1;
+        uint64_t c2 = Le[2] + (c1 >> 32);
+        s2 = (uint32_t)c2;
+        uint64_t c3 = Le[3] + (c2 >> 32);
+        s3 = (uint32_t)c3;
+        s4 = (uint32_t)(c3 >> 32);
+    }
+    // quotient estimate (S < 16 P, error of the estimate at most one)
+    const double Sd = (((((double)s4 * 4294967296.0 + (double)s3) * 4294967296.0) + (double)s2) * 4294967296.0 +
+                       (double)s1) * 4294967296.0 + (double)s0;
+    uint32_t qt = (uint32_t)floor(Sd / tb.Pd);
+    if (qt > 16) qt = 16;
+    // R = S - qt * P (five limbs, two's complement)
+    int64_t r0, r1, r2, r3, r4;
+    {
+        uint64_t b0 = (uint64_t)qt * tb.P[0];
+        uint64_t b1 = (uint64_t)qt * tb.P[1] + (b0 >> 32);
+        uint64_t b2 = (uint64_t)qt * tb.P[2] + (b1 >> 32);
+        uint64_t b3 = (uint64_t)qt * tb.P[3] + (b2 >> 32);
+        const uint32_t b4 = (uint32_t)(b3 >> 32);
+        int64_t d = (int64_t)s0 - (int64_t)(uint32_t)b0;
+        r0 = d & 0xffffffff;
+        d = (int64_t)s1 - (int64_t)(uint32_t)b1 + (d >> 32);
+        r1 = d & 0xffffffff;
+        d = (int64_t)s2 - (int64_t)(uint32_t)b2 + (d >> 32);
+        r2 = d & 0xffffffff;
+        d = (int64_t)s3 - (int64_t)(uint32_t)b3 + (d >> 32);
+        r3 = d & 0xffffffff;
+        r4 = (int64_t)s4 - (int64_t)b4 + (d >> 32);  // sign of R
+    }
+    auto add_limbs = [&](const uint32_t (&v)[4], int sign) {
+        int64_t d = r0 + sign * (int64_t)v[0];
+        r0 = d & 0xffffffff;
+        d = r1 + sign * (int64_t)v[1] + (d >> 32);
+        r1 = d & 0xffffffff;
+        d = r2 + sign * (int64_t)v[2] + (d >> 32);
+        r2 = d & 0xffffffff;
+        d = r3 + sign * (int64_t)v[3] + (d >> 32);
+        r3 = d & 0xffffffff;
+        r4 += d >> 32;
+    };
+    auto greater_eq = [&](const uint32_t (&v)[4]) {  // R >= v, for R >= 0
+        if (r4 != 0) return true;
+        if ((uint32_t)r3 != v[3]) return (uint32_t)r3 > v[3];
+        if ((uint32_t)r2 != v[2]) return (uint32_t)r2 > v[2];
+        if ((uint32_t)r1 != v[1]) return (uint32_t)r1 > v[1];
+        return (uint32_t)r0 >= v[0];
+    };
+    if (r4 < 0) add_limbs(tb.P, 1);
+    else if (greater_eq(tb.P)) add_limbs(tb.P, -1);
+    // balanced: R > floor(P/2) -> R - P
+    bool neg = false;
+    if (greater_eq(tb.Ph) && !((uint32_t)r3 == tb.Ph[3] && (uint32_t)r2 == tb.Ph[2] &&
+                               (uint32_t)r1 == tb.Ph[1] && (uint32_t)r0 == tb.Ph[0])) {
+        add_limbs(tb.P, -1);  // now negative: magnitude P - R
+        neg = true;
+    }
+    uint64_t lo = ((uint64_t)(uint32_t)r1 << 32) | (uint32_t)r0;
+    uint64_t hi = ((uint64_t)(uint32_t)r3 << 32) | (uint32_t)r2;
+    if (neg) {  // two's complement of the 128-bit value
+        lo = ~lo + 1;
+        hi = ~hi + (lo == 0 ? 1 : 0);
+    }
+    // correctly rounded double of the 128-bit magnitude (hi, lo)
+    double v;
+    if (hi == 0) {
+        v = (double)lo;
+    } else {
+        const int z = __builtin_clzll(hi);
+        const uint64_t top = z ? (hi << z) | (lo >> (64 - z)) : hi;
+        const uint64_t rest = z ? (lo << z) : lo;
+        v = ldexp((double)(top | (rest != 0 ? 1ull : 0ull)), 64 - z);
+    }
+    if (neg) v = -v;
+    return v;
+}
+
+// Four consecutive rows per thread (one 32-bit load of residues per modulus).
+// S = sum_t x_t M_t < n P is accumulated exactly in four 64-bit limb sums of
+// 32-bit products, reduced to R = S mod P with a floating quotient estimate
+// and one correction, V = R or R - P (balanced), then converted to the
+// correctly rounded double; C = alpha * ldexp(V, e_i + f_j - 2q) + beta*C.
 __global__ void crt_reconstruct_kernel(const uint8_t* __restrict__ res, CrtTables tb, int64_t m, int64_t n,
                                        int64_t Mp, int64_t Np, const int32_t* __restrict__ ea,
                                        const int32_t* __restrict__ eb, int q, double alpha, double beta,
                                        double* __restrict__ C, int64_t ldc) {
-    const int64_t total = m * n;
+    const int64_t qm = (m + 3) / 4;
+    const int64_t total = qm * n;
     for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x < total; x += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t j = x / m, i = x - j * m;
-        // Fully unrolled over kMaxModuli so that the partial sums stay in registers.
-        int s[kMaxModuli];
+        const int64_t j = x / qm, i0 = (x - j * qm) * 4;
+        uint64_t L[4][4];
 #pragma unroll
-        for (int u = 0; u < kMaxModuli; ++u) s[u] = 0;
-        uint64_t hi = 0, lo = 0;  // two's complement 128-bit accumulator
+        for (int e = 0; e < 4; ++e)
+#pragma unroll
+            for (int l = 0; l < 4; ++l) L[e][l] = 0;
+        const uint8_t* rp = res + j * Mp + i0;
 #pragma unroll
         for (int t = 0; t < kMaxModuli; ++t) {
             if (t < tb.n) {
-                const int p = c_moduli[t];
-                int d = ((int)res[(int64_t)t * Mp * Np + j * Mp + i] - s[t]) * tb.rinv[t];
-                d %= p;
-                if (d < 0) d += p;
-                if (2 * d >= p) d -= p;  // balanced digit
+                const uint32_t w = *reinterpret_cast<const uint32_t*>(rp + (int64_t)t * Mp * Np);
+                const uint32_t c = (uint32_t)tb.cinv[t];
 #pragma unroll
-                for (int u = t + 1; u < kMaxModuli; ++u) {
-                    if (u < tb.n) {
-                        const int w = (s[u] + d * tb.rmod[t][u]) % c_moduli[u];
-                        s[u] = w < 0 ? w + c_moduli[u] : w;
-                    }
+                for (int e = 0; e < 4; ++e) {
+                    const uint32_t xr = (((w >> (8 * e)) & 255u) * c) % (uint32_t)crt_modulus(t);
+#pragma unroll
+                    for (int l = 0; l < 4; ++l) L[e][l] += (uint64_t)xr * tb.mt[t][l];
                 }
-                // (hi, lo) += d * R_t
-                const uint64_t ad = (uint64_t)(d < 0 ? -d : d);
-                uint64_t plo = ad * tb.rlo[t];
-                uint64_t phi = __umul64hi(ad, tb.rlo[t]) + ad * tb.rhi[t];
-                if (d < 0) {  // negate the 128-bit product
-                    plo = ~plo + 1;
-                    phi = ~phi + (plo == 0 ? 1 : 0);
-                }
-                const uint64_t nlo = lo + plo;
-                hi = hi + phi + (nlo < lo ? 1 : 0);
-                lo = nlo;
             }
         }
-        // correctly rounded double of the signed 128-bit (hi, lo)
-        const bool neg = (int64_t)hi < 0;
-        if (neg) {
-            lo = ~lo + 1;
-            hi = ~hi + (lo == 0 ? 1 : 0);
+        const int64_t jf = eb[j];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int64_t i = i0 + e;
+            if (i >= m) break;
+            const double v = crt_value(L[e], tb);
+            double val = __dmul_rn(alpha, ldexp(v, ea[i] + (int)jf - 2 * q));
+            double* cp = C + i + j * ldc;
+            if (beta != 0.0) val = __dadd_rn(val, __dmul_rn(beta, *cp));
+            *cp = val;
         }
-        double v;
-        if (hi == 0) {
-            v = (double)lo;
-        } else {
-            const int z = __clzll((long long)hi);
-            const uint64_t top = z ? (hi << z) | (lo >> (64 - z)) : hi;
-            const uint64_t rest = z ? (lo << z) : lo;
-            v = ldexp((double)(top | (rest != 0 ? 1ull : 0ull)), 64 - z);
-        }
-        if (neg) v = -v;
-        double val = __dmul_rn(alpha, ldexp(v, ea[i] + eb[j] - 2 * q));
-        double* cp = C + i + j * ldc;
-        if (beta != 0.0) val = __dadd_rn(val, __dmul_rn(beta, *cp));
-        *cp = val;
     }
 }
 
@@ -585,21 +660,32 @@ __global__ void oz_slice_kernel(const double* __restrict__ x, int64_t rows, int6
     }
     __syncthreads();
     const double scale = ldexp(1.0, bits);
-    for (int rr = ty; rr < 64; rr += 4) {
-        const int64_t r = r0 + rr, k = k0 + tx;
-        double v = (r < rows) ? ldexp(tile[rr][tx], -e[r]) : 0.0;
-        for (int t = 0; t < s; ++t) {
-            v = v * scale;
-            const double d = trunc(v);
-            v = v - d;
+    // thread: row t/4 of the tile, 16 consecutive k -> 16 (INT8) or 32 (BF16) bytes per slice
+    const int rr = threadIdx.x >> 2;
+    const int kq = (threadIdx.x & 3) * 16;
+    const int64_t r = r0 + rr;
+    double v[16];
+    const int sh = r < rows ? -e[r] : 0;
+#pragma unroll
+    for (int c = 0; c < 16; ++c) v[c] = r < rows ? ldexp(tile[rr][kq + c], sh) : 0.0;
+    int8_t* dst = out + (r * Kp + k0 + kq) * (BF16 ? 2 : 1);
+    for (int t = 0; t < s; ++t) {
+        uint32_t w[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+        for (int c = 0; c < 16; ++c) {
+            v[c] = v[c] * scale;
+            const double d = trunc(v[c]);
+            v[c] = v[c] - d;
             if (BF16) {
                 const float f = (float)d;  // |d| <= 31: exact, low mantissa bits zero
-                const uint16_t hb = (uint16_t)(__float_as_uint(f) >> 16);
-                reinterpret_cast<uint16_t*>(out + t * slice_bytes)[r * Kp + k] = hb;
+                w[c >> 1] |= (__float_as_uint(f) >> 16) << (16 * (c & 1));
             } else {
-                out[t * slice_bytes + r * Kp + k] = (int8_t)(int)d;
+                w[c >> 2] |= (uint32_t)(uint8_t)(int8_t)(int)d << (8 * (c & 3));
             }
         }
+        iv4* o = reinterpret_cast<iv4*>(dst + t * slice_bytes);
+        o[0] = iv4{(int)w[0], (int)w[1], (int)w[2], (int)w[3]};
+        if (BF16) o[1] = iv4{(int)w[4], (int)w[5], (int)w[6], (int)w[7]};
     }
 }
 
@@ -612,10 +698,10 @@ struct Plan {
     size_t off_a, off_b, off_ea, off_eb, off_part, off_w, total;
 };
 
-// Host side of the CRT form: q and the Garner tables for n moduli.
+// Host side of the CRT form: q and the reconstruction tables for n moduli.
 int crt_setup(int64_t k, int nmod, CrtTables* tb, int* q_out) {
     unsigned __int128 P = 1;
-    for (int t = 0; t < nmod; ++t) P *= (unsigned)h_moduli[t];
+    for (int t = 0; t < nmod; ++t) P *= (unsigned)crt_modulus(t);
     // largest q <= 53 with k * 4^q * 4 <= P
     int q = 53;
     while (q > 0) {
@@ -626,23 +712,26 @@ int crt_setup(int64_t k, int nmod, CrtTables* tb, int* q_out) {
     if (q < 1) return fail(DG_ERR_UNSUPPORTED, "dg_ozaki_dgemm: %d moduli leave no bits for k=%lld", nmod, (long long)k);
     *q_out = q;
     tb->n = nmod;
-    unsigned __int128 R = 1;
+    auto limbs = [](unsigned __int128 v, uint32_t (&out)[4]) {
+        for (int l = 0; l < 4; ++l) out[l] = (uint32_t)(v >> (32 * l));
+    };
     for (int t = 0; t < nmod; ++t) {
-        tb->rhi[t] = (uint64_t)(R >> 64);
-        tb->rlo[t] = (uint64_t)R;
-        for (int u = 0; u < nmod; ++u) tb->rmod[t][u] = (int16_t)(R % (unsigned)h_moduli[u]);
-        // inverse of R mod p_t by extended Euclid (R and p_t are coprime)
-        const int p = h_moduli[t];
-        int64_t a = (int64_t)(R % (unsigned)p), b = p, x0 = 1, x1 = 0;
+        const int p = crt_modulus(t);
+        const unsigned __int128 M = P / (unsigned)p;
+        limbs(M, tb->mt[t]);
+        // inverse of M mod p by extended Euclid (M and p are coprime)
+        int64_t a = (int64_t)(M % (unsigned)p), b = p, x0 = 1, x1 = 0;
         while (b) {
             const int64_t qq = a / b, tmp = a - qq * b;
             a = b; b = tmp;
             const int64_t tx = x0 - qq * x1;
             x0 = x1; x1 = tx;
         }
-        tb->rinv[t] = (int32_t)(((x0 % p) + p) % p);
-        R *= (unsigned)h_moduli[t];
+        tb->cinv[t] = (int32_t)(((x0 % p) + p) % p);
     }
+    limbs(P, tb->P);
+    limbs(P / 2, tb->Ph);
+    tb->Pd = (double)P;
     return DG_OK;
 }
 
@@ -836,7 +925,6 @@ int dg_ozaki_dgemm(char opA, char opB, int64_t m, int64_t n, int64_t k, double a
                            nchunk, eb);
         DG_LAUNCH_CHECK();
     }
-    const size_t lds_bytes = (size_t)4 * OT * LDB;  // 144 KB: two buffers of A and B
     if (crt) {
         dim3 ga((unsigned)(p.Kp / 64), (unsigned)(p.Mp / 64));
         dim3 gb((unsigned)(p.Kp / 64), (unsigned)(p.Np / 64));
@@ -851,7 +939,7 @@ int dg_ozaki_dgemm(char opA, char opB, int64_t m, int64_t n, int64_t k, double a
         cg.tiles_m = (int32_t)(p.Mp / OT); cg.tiles_n = (int32_t)(p.Np / OT); cg.nmod = slices;
         const int64_t grid = (int64_t)cg.tiles_m * cg.tiles_n * slices;
         if (grid > INT32_MAX) return fail(DG_ERR_INVALID, "dg_ozaki_dgemm: too many tiles");
-        hipLaunchKernelGGL(crt_gemm_kernel, dim3((unsigned)grid), dim3(NTG), lds_bytes, st, cg);
+        hipLaunchKernelGGL(crt_gemm_kernel, dim3((unsigned)grid), dim3(CNT), kGemmLds, st, cg);
         DG_LAUNCH_CHECK();
         int64_t blocks = (m * n + 255) / 256;
         if (blocks > 65536) blocks = 65536;
@@ -886,10 +974,12 @@ int dg_ozaki_dgemm(char opA, char opB, int64_t m, int64_t n, int64_t k, double a
     g.alpha = alpha; g.beta = beta; g.c = C; g.ldc = ldc;
     g.tiles_m = (int32_t)(p.Mp / OT);
     g.tiles_n = (int32_t)(p.Np / OT);
+    g.nsteps = 0;
+    for (const OzSeg& sg : sched) g.nsteps += (int64_t)sg.klen * p.esz / CK;  // klen*esz is a multiple of KSB
     const int64_t tiles = (int64_t)g.tiles_m * g.tiles_n;
     if (tiles > INT32_MAX) return fail(DG_ERR_INVALID, "dg_ozaki_dgemm: too many tiles");
-    if (bf16) hipLaunchKernelGGL(oz_gemm_kernel<true>, dim3((unsigned)tiles), dim3(NTG), lds_bytes, st, g);
-    else      hipLaunchKernelGGL(oz_gemm_kernel<false>, dim3((unsigned)tiles), dim3(NTG), lds_bytes, st, g);
+    if (bf16) hipLaunchKernelGGL(oz_gemm_kernel<true>, dim3((unsigned)tiles), dim3(CNT), kGemmLds, st, g);
+    else      hipLaunchKernelGGL(oz_gemm_kernel<false>, dim3((unsigned)tiles), dim3(CNT), kGemmLds, st, g);
     DG_LAUNCH_CHECK();
     return DG_OK;
 }
