@@ -112,4 +112,8 @@ int default_tile_cfg();
 // C[x] = beta*C[x] over an m x n column-major matrix (beta == 0 writes 0).
 int launch_scale(int64_t m, int64_t n, double beta, double* C, int64_t ldc, hipStream_t stream);
 
+// Smallest CRT moduli count (2..16) whose scaling reaches q = 53 bits for
+// inner dimension k (16 when none does).
+int crt_moduli_for_fp64(int64_t k);
+
 }  // namespace dg

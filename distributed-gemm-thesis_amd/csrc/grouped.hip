@@ -1,6 +1,7 @@
 // Grouped batches of FP64 GEMMs: one launch for many uneven problems, plans
 // that keep their descriptor tables resident, HIP-graph replay and an
-// autotuner that picks the tile scheduling per shape class.
+// autotuner that picks the tile scheduling per shape class and, in the Ozaki
+// form, the arithmetic (FP64 MFMA or INT8 CRT emulation) per problem.
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
@@ -29,6 +30,16 @@ struct dg_grouped_plan {
     hipStream_t capture_stream = nullptr;
     int device = 0;
     std::vector<std::pair<double*, size_t>> c_spans;  // bytes of every output, for the tuning snapshot
+    // Ozaki form (DG_FORM_OZAKI_INT8): per-problem choice made on first use.
+    int form = DG_FORM_FP64;
+    int moduli_req = 0;                 // 0: per problem, FP64-level count
+    bool forms_done = true;
+    std::vector<dg_gemm_desc> descs;    // the batch, kept for the deferred build
+    std::vector<int> prob_form;         // DG_FORM_* per problem (-1 before the choice)
+    std::vector<int> prob_moduli;       // moduli per Ozaki problem (0 for FP64)
+    std::vector<int> oz_idx;            // problems run in the Ozaki form, in order
+    void* oz_ws = nullptr;
+    int64_t oz_ws_bytes = 0;
 };
 
 namespace dg {
@@ -135,11 +146,160 @@ std::map<std::pair<char, char>, std::vector<int>> by_ops(const dg_gemm_desc* d, 
     return m;
 }
 
-int plan_launch(dg_grouped_plan* p, int variant, int cfg, hipStream_t st) {
+int launch_ozaki_problem(const dg_grouped_plan* p, int i, hipStream_t st) {
+    const dg_gemm_desc& d = p->descs[i];
+    return dg_ozaki_dgemm(d.opA, d.opB, d.m, d.n, d.k, d.alpha, d.A, d.lda, d.B, d.ldb, d.beta, d.C, d.ldc,
+                          p->prob_moduli[i], DG_OZAKI_INT8_CRT, p->oz_ws, p->oz_ws_bytes, nullptr, st);
+}
+
+int plan_launch(dg_grouped_plan* p, int variant, int cfg, hipStream_t st, bool with_ozaki = true) {
     for (auto& g : p->groups) {
         int rc = launch_gemm_f64_dev(g.opA, g.opB, g.vec, variant, cfg, g.dprobs, g.np, g.dsegs, g.tiles, p->counter, st);
         if (rc) return rc;
     }
+    if (with_ozaki)
+        for (int i : p->oz_idx) {
+            int rc = launch_ozaki_problem(p, i, st);
+            if (rc) return rc;
+        }
+    return DG_OK;
+}
+
+// Device tables of the FP64 problems listed in `use` (all when empty).
+int build_groups(dg_grouped_plan* p, const dg_gemm_desc* descs, int count, const std::vector<char>* use) {
+    std::vector<dg_gemm_desc> sel;
+    sel.reserve(count);
+    for (int i = 0; i < count; ++i)
+        if (!use || (*use)[i]) sel.push_back(descs[i]);
+    for (auto& kv : by_ops(sel.data(), (int)sel.size())) {
+        std::vector<Prob> probs0, probs;
+        std::vector<Seg> segs;
+        build_group(sel.data(), kv.second, probs0, segs);
+        bool vec = true;
+        int tiles[2] = {0, 0};
+        int rc = prepare_gemm(probs0, segs, probs, &vec, tiles);
+        if (rc) return rc;
+        if (probs.empty()) continue;
+        dg_grouped_plan::Group g;
+        g.opA = kv.first.first;
+        g.opB = kv.first.second;
+        g.vec = vec;
+        g.tiles[0] = tiles[0];
+        g.tiles[1] = tiles[1];
+        g.np = (int)probs.size();
+        if (hipMalloc(&g.dprobs, probs.size() * sizeof(Prob)) != hipSuccess ||
+            hipMalloc(&g.dsegs, segs.size() * sizeof(Seg)) != hipSuccess) {
+            p->groups.push_back(g);
+            return fail(DG_ERR_ALLOC, "plan: table alloc");
+        }
+        p->groups.push_back(g);
+        if (hipMemcpy(g.dprobs, probs.data(), probs.size() * sizeof(Prob), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(g.dsegs, segs.data(), segs.size() * sizeof(Seg), hipMemcpyHostToDevice) != hipSuccess)
+            return fail(DG_ERR_HIP, "plan: table upload");
+    }
+    return DG_OK;
+}
+
+// Output bytes of problem i (its C footprint).
+size_t c_bytes(const dg_gemm_desc& d) { return sizeof(double) * (size_t)(d.ldc * (d.n - 1) + d.m); }
+
+// Times one problem alone (best of three): FP64 MFMA, or Ozaki with the
+// plan's workspace.
+int time_problem(const dg_grouped_plan* p, int i, bool ozaki, hipStream_t st, hipEvent_t e0, hipEvent_t e1,
+                 float* out) {
+    const dg_gemm_desc& d = p->descs[i];
+    float best = 1e30f;
+    for (int rep = 0; rep < 3; ++rep) {
+        DG_HIP(hipEventRecord(e0, st));
+        int rc = ozaki ? launch_ozaki_problem(p, i, st)
+                       : dg_dgemm(d.opA, d.opB, d.m, d.n, d.k, d.alpha, d.A, d.lda, d.B, d.ldb, d.beta, d.C, d.ldc, st);
+        if (rc) return rc;
+        DG_HIP(hipEventRecord(e1, st));
+        DG_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        DG_HIP(hipEventElapsedTime(&ms, e0, e1));
+        best = std::min(best, ms);
+    }
+    *out = best;
+    return DG_OK;
+}
+
+// Ozaki form: picks each problem's form, allocates the shared workspace and
+// uploads the tables of the FP64 problems.  Outputs are restored afterwards.
+int choose_forms(dg_grouped_plan* p, hipStream_t st) {
+    const int count = (int)p->descs.size();
+    p->prob_form.assign(count, DG_FORM_FP64);
+    p->prob_moduli.assign(count, 0);
+    std::vector<int> cand;
+    int64_t ws = 0;
+    for (int i = 0; i < count; ++i) {
+        const dg_gemm_desc& d = p->descs[i];
+        if (d.m <= 0 || d.n <= 0 || d.k <= 0 || d.alpha == 0.0 || d.k > 131071) continue;
+        const double flops = 2.0 * (double)d.m * (double)d.n * (double)d.k;
+        if (flops < 4294967296.0) continue;
+        const int nm = p->moduli_req > 0 ? p->moduli_req : crt_moduli_for_fp64(d.k);
+        p->prob_moduli[i] = nm;
+        cand.push_back(i);
+        ws = std::max<int64_t>(ws, dg_ozaki_workspace(d.m, d.n, d.k, nm, DG_OZAKI_INT8_CRT));
+    }
+    if (!cand.empty()) {
+        if (hipMalloc(&p->oz_ws, (size_t)ws) != hipSuccess) return fail(DG_ERR_ALLOC, "plan: Ozaki workspace of %lld bytes", (long long)ws);
+        p->oz_ws_bytes = ws;
+        TuneCache& c = tune_cache();
+        hipEvent_t e0, e1;
+        DG_HIP(hipEventCreate(&e0));
+        DG_HIP(hipEventCreate(&e1));
+        for (int i : cand) {
+            const dg_gemm_desc& d = p->descs[i];
+            char key[160];
+            snprintf(key, sizeof(key), "form:%c%c.%lld.%lld.%lld.%d", norm_op(d.opA), norm_op(d.opB), (long long)d.m,
+                     (long long)d.n, (long long)d.k, p->prob_moduli[i]);
+            int choice = -1;
+            {
+                std::lock_guard<std::mutex> lock(c.mu);
+                tune_load_locked(c);
+                auto it = c.best.find(key);
+                if (it != c.best.end()) choice = it->second;
+            }
+            if (choice < 0) {
+                // snapshot C, time both forms, restore C
+                double* snap = nullptr;
+                const size_t cb = c_bytes(d);
+                if (hipMalloc(&snap, cb) != hipSuccess) return fail(DG_ERR_ALLOC, "autotune: snapshot of %zu bytes", cb);
+                DG_HIP(hipMemcpyAsync(snap, d.C, cb, hipMemcpyDeviceToDevice, st));
+                float t64 = 0.f, toz = 0.f;
+                int rc = time_problem(p, i, false, st, e0, e1, &t64);
+                if (!rc) rc = time_problem(p, i, true, st, e0, e1, &toz);
+                DG_HIP(hipMemcpyAsync(d.C, snap, cb, hipMemcpyDeviceToDevice, st));
+                DG_HIP(hipStreamSynchronize(st));
+                (void)hipFree(snap);
+                if (rc) return rc;
+                choice = toz < t64 ? DG_FORM_OZAKI_INT8 : DG_FORM_FP64;
+                std::lock_guard<std::mutex> lock(c.mu);
+                tune_store_locked(c, key, choice);
+            }
+            p->prob_form[i] = choice;
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+    }
+    std::vector<char> use(count, 1);
+    p->oz_idx.clear();
+    for (int i = 0; i < count; ++i)
+        if (p->prob_form[i] == DG_FORM_OZAKI_INT8) {
+            use[i] = 0;
+            p->oz_idx.push_back(i);
+        } else {
+            p->prob_moduli[i] = 0;
+        }
+    if (p->oz_idx.empty() && p->oz_ws) {
+        (void)hipFree(p->oz_ws);
+        p->oz_ws = nullptr;
+        p->oz_ws_bytes = 0;
+    }
+    int rc = build_groups(p, p->descs.data(), count, &use);
+    if (rc) return rc;
+    p->forms_done = true;
     return DG_OK;
 }
 
@@ -177,7 +337,7 @@ int autotune(dg_grouped_plan* p, hipStream_t st) {
             float vbest = 1e30f;
             for (int rep = 0; rep < 3; ++rep) {
                 DG_HIP(hipEventRecord(e0, st));
-                int rc = plan_launch(p, v, cfg, st);
+                int rc = plan_launch(p, v, cfg, st, false);
                 if (rc) return rc;
                 DG_HIP(hipEventRecord(e1, st));
                 DG_HIP(hipEventSynchronize(e1));
@@ -233,11 +393,20 @@ int dg_grouped_dgemm(const dg_gemm_desc* descs, int count, void* stream) {
 }
 
 int dg_grouped_plan_create(const dg_gemm_desc* descs, int count, int mode, dg_grouped_plan** out) {
+    return dg_grouped_plan_create_form(descs, count, mode, DG_FORM_FP64, 0, out);
+}
+
+int dg_grouped_plan_create_form(const dg_gemm_desc* descs, int count, int mode, int form, int moduli,
+                                dg_grouped_plan** out) {
     using namespace dg;
     if (!out) return fail(DG_ERR_INVALID, "dg_grouped_plan_create: null output");
     *out = nullptr;
     if (mode != DG_PLAN_AUTOTUNE && mode != DG_PLAN_NONPERSISTENT && mode != DG_PLAN_PERSISTENT)
         return fail(DG_ERR_INVALID, "dg_grouped_plan_create: mode %d", mode);
+    if (form != DG_FORM_FP64 && form != DG_FORM_OZAKI_INT8)
+        return fail(DG_ERR_INVALID, "dg_grouped_plan_create: form %d", form);
+    if (form == DG_FORM_OZAKI_INT8 && moduli != 0 && (moduli < 2 || moduli > 16))
+        return fail(DG_ERR_INVALID, "dg_grouped_plan_create: moduli %d (0 or 2..16)", moduli);
     int rc = validate_descs(descs, count);
     if (rc) return rc;
     for (int i = 0; i < count; ++i)
@@ -247,41 +416,27 @@ int dg_grouped_plan_create(const dg_gemm_desc* descs, int count, int mode, dg_gr
     if (rc) return rc;
     dg_grouped_plan* p = new dg_grouped_plan();
     p->mode = mode;
+    p->form = form;
+    p->moduli_req = moduli;
     (void)hipGetDevice(&p->device);
     p->shape_key = shape_class(descs, count);
+    if (form == DG_FORM_OZAKI_INT8) p->shape_key = "oz" + std::to_string(moduli) + ":" + p->shape_key;
     if (mode == DG_PLAN_NONPERSISTENT) p->variant = DG_GEMM_MODE_ROUND_ROBIN;
     if (mode == DG_PLAN_PERSISTENT) p->variant = DG_GEMM_MODE_PERSISTENT;
     p->cfg = default_tile_cfg();
     auto cleanup = [&](int code) { dg_grouped_plan_destroy(p); return code; };
     if (hipMalloc(&p->counter, sizeof(int)) != hipSuccess) return cleanup(fail(DG_ERR_ALLOC, "plan: counter alloc"));
     for (int i = 0; i < count; ++i)
-        if (descs[i].m > 0 && descs[i].n > 0)
-            p->c_spans.push_back({descs[i].C, sizeof(double) * (size_t)(descs[i].ldc * (descs[i].n - 1) + descs[i].m)});
-    for (auto& kv : by_ops(descs, count)) {
-        std::vector<Prob> probs0, probs;
-        std::vector<Seg> segs;
-        build_group(descs, kv.second, probs0, segs);
-        bool vec = true;
-        int tiles[2] = {0, 0};
-        rc = prepare_gemm(probs0, segs, probs, &vec, tiles);
+        if (descs[i].m > 0 && descs[i].n > 0) p->c_spans.push_back({descs[i].C, c_bytes(descs[i])});
+    if (form == DG_FORM_OZAKI_INT8) {
+        // the per-problem forms (and with them the FP64 tables) are chosen on first use
+        p->descs.assign(descs, descs + count);
+        p->forms_done = false;
+    } else {
+        rc = build_groups(p, descs, count, nullptr);
         if (rc) return cleanup(rc);
-        if (probs.empty()) continue;
-        dg_grouped_plan::Group g;
-        g.opA = kv.first.first;
-        g.opB = kv.first.second;
-        g.vec = vec;
-        g.tiles[0] = tiles[0];
-        g.tiles[1] = tiles[1];
-        g.np = (int)probs.size();
-        if (hipMalloc(&g.dprobs, probs.size() * sizeof(Prob)) != hipSuccess ||
-            hipMalloc(&g.dsegs, segs.size() * sizeof(Seg)) != hipSuccess) {
-            p->groups.push_back(g);
-            return cleanup(fail(DG_ERR_ALLOC, "plan: table alloc"));
-        }
-        p->groups.push_back(g);
-        if (hipMemcpy(g.dprobs, probs.data(), probs.size() * sizeof(Prob), hipMemcpyHostToDevice) != hipSuccess ||
-            hipMemcpy(g.dsegs, segs.data(), segs.size() * sizeof(Seg), hipMemcpyHostToDevice) != hipSuccess)
-            return cleanup(fail(DG_ERR_HIP, "plan: table upload"));
+        p->prob_form.assign(count, DG_FORM_FP64);
+        p->prob_moduli.assign(count, 0);
     }
     *out = p;
     return DG_OK;
@@ -291,6 +446,10 @@ int dg_grouped_plan_execute(dg_grouped_plan* p, void* stream) {
     using namespace dg;
     if (!p) return fail(DG_ERR_INVALID, "dg_grouped_plan_execute: null plan");
     hipStream_t st = as_stream(stream);
+    if (!p->forms_done) {
+        int rc = choose_forms(p, st);
+        if (rc) return rc;
+    }
     if (p->variant < 0) {
         int rc = autotune(p, st);
         if (rc) return rc;
@@ -302,6 +461,10 @@ int dg_grouped_plan_graph_launch(dg_grouped_plan* p, void* stream) {
     using namespace dg;
     if (!p) return fail(DG_ERR_INVALID, "dg_grouped_plan_graph_launch: null plan");
     hipStream_t st = as_stream(stream);
+    if (!p->forms_done) {
+        int rc = choose_forms(p, st);
+        if (rc) return rc;
+    }
     if (p->variant < 0) {
         int rc = autotune(p, st);
         if (rc) return rc;
@@ -333,8 +496,16 @@ int dg_grouped_plan_variant(const dg_grouped_plan* p) {
     return DG_PLAN_NONPERSISTENT;
 }
 
+int dg_grouped_plan_problem_form(const dg_grouped_plan* p, int index, int* moduli) {
+    if (moduli) *moduli = 0;
+    if (!p || !p->forms_done || index < 0 || index >= (int)p->prob_form.size()) return -1;
+    if (moduli) *moduli = p->prob_moduli[index];
+    return p->prob_form[index];
+}
+
 int dg_grouped_plan_destroy(dg_grouped_plan* p) {
     if (!p) return DG_OK;
+    if (p->oz_ws) (void)hipFree(p->oz_ws);
     if (p->graph) (void)hipGraphExecDestroy(p->graph);
     if (p->capture_stream) (void)hipStreamDestroy(p->capture_stream);
     for (auto& g : p->groups) {

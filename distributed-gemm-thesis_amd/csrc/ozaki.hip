@@ -42,14 +42,24 @@ typedef float fv4 __attribute__((ext_vector_type(4)));
 typedef double dv4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 
-constexpr int OT = 256;      // output tile (rows and columns)
+constexpr int PAD = 256;     // row padding of the slices and residues (a multiple of BM and BN)
 constexpr int KSB = 128;     // K padding granule of the slices and residues, bytes
 constexpr int CK = 64;       // k-step in bytes
-constexpr int CSTG = 4;      // LDS stages
-constexpr int CNT = 256;     // threads of the product kernels: 4 waves of 128x128
-constexpr int FR = 8;        // 16x16 MFMA tiles per wave and dimension
-constexpr int CSTAGE = 2 * OT * CK;  // bytes per stage (A rows, then B rows)
-constexpr size_t kGemmLds = (size_t)CSTG * CSTAGE;
+// Product kernels: BM x BN output tiles, waves of WM x WN, STG-stage LDS ring,
+// OCC workgroups per CU.
+constexpr int BM = 256, BN = 256;
+constexpr int WM = 128, WN = 64;
+constexpr int STG = 4;
+constexpr int OCC = 1;
+constexpr int NW = (BM / WM) * (BN / WN);  // waves per workgroup
+constexpr int CNT = NW * 64;               // threads per workgroup
+constexpr int FM = WM / 16, FN = WN / 16;  // 16x16 MFMA tiles per wave
+constexpr int LA = BM / 16 / NW, LB = BN / 16 / NW;  // 16-row loads per wave and step
+constexpr int LPS = LA + LB;
+constexpr int SSZ = (BM + BN) * CK;        // bytes per stage (A rows, then B rows)
+constexpr size_t kGemmLds = (size_t)STG * SSZ;
+static_assert(LA * 16 * NW == BM && LB * 16 * NW == BN, "loader split");
+static_assert(PAD % BM == 0 && PAD % BN == 0 && STG >= 2 && STG <= 5, "tile configuration");
 constexpr int NT = 256;
 constexpr int GROUP_M = 8;
 
@@ -99,15 +109,15 @@ __device__ __forceinline__ void mma(iv4& acc_i, fv4& acc_f, const iv4& x, const 
     }
 }
 
-// The product kernels' pipeline.  256x256 output tiles, 4 waves of 128x128
-// (8x8 MFMA tiles of 16x16: 256 accumulator registers per lane), 64-byte
-// k-steps in a 4-stage LDS ring (4 x 32 KB) filled by global_load_lds_dwordx4,
-// which writes LDS without a register round trip.  One such load of a wave
-// covers 16 rows x 64 bytes; within a row, 16-byte chunk c sits at position
-// c ^ swz(row) so that the 128-bit fragment reads of 8 or 16 consecutive rows
-// hit distinct banks.  Each wave waits for its own loads of step k (vmcnt),
-// then one barrier makes every wave's loads visible and frees the stage read
-// at step k-1, which then receives step k+3.
+// The product kernels' pipeline.  BM x BN output tiles, waves of WM x WN
+// (FM x FN MFMA tiles of 16x16), 64-byte k-steps in an STG-stage LDS ring
+// filled by global_load_lds_dwordx4, which writes LDS without a register
+// round trip.  One such load of a wave covers 16 rows x 64 bytes; within a
+// row, 16-byte chunk c sits at position c ^ swz(row) so that the 128-bit
+// fragment reads of 8 or 16 consecutive rows hit distinct banks.  Each wave
+// waits for its own loads of step k (vmcnt), then one barrier makes every
+// wave's loads visible and frees the stage read at step k-1, which then
+// receives step k+STG-1.  All fragments of a step are read before its MFMAs.
 __host__ __device__ constexpr int swz(int r) { return (((r >> 2) & 1) << 1) | (((r >> 1) ^ (r >> 3)) & 1); }
 
 template <int N>
@@ -118,44 +128,48 @@ __device__ __forceinline__ void wait_vmcnt() {
 }
 
 // Waits until this wave's loads of the current step have landed, given the
-// number of later steps already issued (8 loads each).
+// number of later steps already issued (LPS loads each, at most STG-2 steps).
 __device__ __forceinline__ void wait_step(int64_t ahead) {
-    if (ahead >= 2) wait_vmcnt<16>();
-    else if (ahead == 1) wait_vmcnt<8>();
+    if (ahead >= 3) wait_vmcnt<3 * LPS>();
+    else if (ahead == 2) wait_vmcnt<2 * LPS>();
+    else if (ahead == 1) wait_vmcnt<LPS>();
     else wait_vmcnt<0>();
 }
 
 typedef __attribute__((address_space(3))) void lds_void_t;
 typedef __attribute__((address_space(1))) const void gvoid_t;
 
-// Issues one k-step of both operands into stage `stage`: this wave's rows
-// 64w .. 64w+63, 16 rows per load; pa/pb point at this lane's first row and
-// chunk of the step, rows `rb` bytes apart.
+// Issues one k-step of both operands into stage `stage`: this wave's A rows
+// 16*LA*w .. +16*LA and B rows 16*LB*w .. +16*LB, 16 rows per load; pa/pb
+// point at this lane's first row and chunk of the step, rows `rb` bytes apart.
 __device__ __forceinline__ void issue_step(int8_t* lds, int stage, int wave, const int8_t* pa, const int8_t* pb,
                                            int64_t rb) {
-    int8_t* sa = lds + stage * CSTAGE + wave * 4096;
-    int8_t* sb = sa + OT * CK;
+    int8_t* sa = lds + stage * SSZ + wave * (1024 * LA);
+    int8_t* sb = lds + stage * SSZ + BM * CK + wave * (1024 * LB);
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
+    for (int i = 0; i < LA; ++i)
         __builtin_amdgcn_global_load_lds((gvoid_t*)(pa + 16 * i * rb), (lds_void_t*)(sa + 1024 * i), 16, 0, 0);
+#pragma unroll
+    for (int i = 0; i < LB; ++i)
         __builtin_amdgcn_global_load_lds((gvoid_t*)(pb + 16 * i * rb), (lds_void_t*)(sb + 1024 * i), 16, 0, 0);
-    }
 }
 
-// 64 MFMAs of one stage: as/bs point at this lane's fragment of the wave's
-// first 16-row block of A and B.
+// FM x FN MFMAs of one stage: as/bs point at this lane's fragment of the
+// wave's first 16-row block of A and B.
 template <bool BF16>
-__device__ __forceinline__ void stage_mma(iv4 (&acci)[FR][FR], fv4 (&accf)[FR][FR], const int8_t* as,
+__device__ __forceinline__ void stage_mma(iv4 (&acci)[FM][FN], fv4 (&accf)[FM][FN], const int8_t* as,
                                           const int8_t* bs) {
-    iv4 bfr[FR];
+    iv4 fb[FN], fa[FM];
+    fa[0] = *reinterpret_cast<const iv4*>(as);
 #pragma unroll
-    for (int j = 0; j < FR; ++j) bfr[j] = *reinterpret_cast<const iv4*>(bs + 1024 * j);
+    for (int j = 0; j < FN; ++j) fb[j] = *reinterpret_cast<const iv4*>(bs + 1024 * j);
 #pragma unroll
-    for (int i = 0; i < FR; ++i) {
-        const iv4 af = *reinterpret_cast<const iv4*>(as + 1024 * i);
+    for (int i = 1; i < FM; ++i) fa[i] = *reinterpret_cast<const iv4*>(as + 1024 * i);
+    __builtin_amdgcn_sched_barrier(0);  // every fragment read ahead of the MFMAs
 #pragma unroll
-        for (int j = 0; j < FR; ++j) mma<BF16>(acci[i][j], accf[i][j], bfr[j], af);
-    }
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j) mma<BF16>(acci[i][j], accf[i][j], fb[j], fa[i]);
 }
 
 // Output tile of workgroup slot `bid` among `tiles`: XCD-aware, GROUP_M tile
@@ -167,8 +181,8 @@ __device__ __forceinline__ void tile_origin(int bid, int tiles, int tiles_m, int
     const int first_m = group * GROUP_M;
     const int gsize = min(tiles_m - first_m, GROUP_M);
     const int in_group = tile - group * per_group;
-    m0 = (int64_t)(first_m + in_group % gsize) * OT;
-    n0 = (int64_t)(in_group / gsize) * OT;
+    m0 = (int64_t)(first_m + in_group % gsize) * BM;
+    n0 = (int64_t)(in_group / gsize) * BN;
 }
 
 // Flushes: the first one writes W = x, later ones W += x, the last one
@@ -182,12 +196,12 @@ __device__ __forceinline__ void tile_origin(int bid, int tiles, int tiles_m, int
 // (C from W + x), 4 first and last (C from x alone); guard: the tile crosses
 // the edge of C.  Every branch is uniform across the wave.
 template <bool BF16>
-__device__ __forceinline__ void oz_flush(const iv4 (&acci)[FR][FR], const fv4 (&accf)[FR][FR], const OzArgs& g,
+__device__ __forceinline__ void oz_flush(const iv4 (&acci)[FM][FN], const fv4 (&accf)[FM][FN], const OzArgs& g,
                                          double w, int64_t mrow, int64_t ncol, int kind, bool guard) {
     const bool to_c = kind >= 3;
     const bool read_w = kind == 1 || kind == 3;
 #pragma unroll
-    for (int j = 0; j < FR; ++j) {
+    for (int j = 0; j < FN; ++j) {
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             const int64_t n = ncol + 16 * j + v;
@@ -195,7 +209,7 @@ __device__ __forceinline__ void oz_flush(const iv4 (&acci)[FR][FR], const fv4 (&
             double* ccol = g.c + n * g.ldc + mrow;
             const int ef = to_c && n < g.n ? g.eb[n] : 0;
 #pragma unroll
-            for (int i = 0; i < FR; ++i) {
+            for (int i = 0; i < FM; ++i) {
                 double x;
                 if constexpr (BF16) x = (double)accf[i][j][v] * w;
                 else x = (double)acci[i][j][v] * w;
@@ -218,15 +232,15 @@ __device__ __forceinline__ void oz_flush(const iv4 (&acci)[FR][FR], const fv4 (&
 // the schedule run back to back through the LDS ring; at the end of a product
 // marked for flushing the 32-bit sums go into W (or, at the last flush, C).
 template <bool BF16>
-__global__ __launch_bounds__(CNT, 1) void oz_gemm_kernel(OzArgs g) {
+__global__ __launch_bounds__(CNT, OCC) void oz_gemm_kernel(OzArgs g) {
     extern __shared__ __attribute__((aligned(1024))) int8_t lds[];
     int64_t m0, n0;
     tile_origin(blockIdx.x, g.tiles_m * g.tiles_n, g.tiles_m, g.tiles_n, m0, n0);
 
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int wm = (wave >> 1) * 128;
-    const int wn = (wave & 1) * 128;
+    const int wm = (wave / (BN / WN)) * WM;
+    const int wn = (wave % (BN / WN)) * WN;
     const int lr = lane & 15;
     const int lg = lane >> 4;
 
@@ -234,20 +248,19 @@ __global__ __launch_bounds__(CNT, 1) void oz_gemm_kernel(OzArgs g) {
     const int64_t rb = g.Kp * esz;  // bytes per slice row
     const int lrow = lane >> 2;
     const int lchunk = (lane & 3) ^ swz(lrow);
-    const int8_t* ga = g.a + (m0 + 64 * wave + lrow) * rb + 16 * lchunk;
-    const int8_t* gb = g.b + (n0 + 64 * wave + lrow) * rb + 16 * lchunk;
+    const int8_t* ga = g.a + (m0 + 16 * LA * wave + lrow) * rb + 16 * lchunk;
+    const int8_t* gb = g.b + (n0 + 16 * LB * wave + lrow) * rb + 16 * lchunk;
     const int roff = lr * CK + 16 * (lg ^ swz(lr));
 
     // issue cursor: product iseg, byte offset ikb within it
     int iseg = 0;
     int64_t ikb = 0;
     OzSeg is = g.segs[0];
-    int64_t istep = 0;
+    int ist = 0;  // stage of the next issue
     auto issue_next = [&]() {
         const int64_t koff = (int64_t)is.k0 * esz + ikb;
-        issue_step(lds, (int)(istep & (CSTG - 1)), wave, ga + is.t * g.a_slice + koff, gb + is.u * g.b_slice + koff,
-                   rb);
-        ++istep;
+        issue_step(lds, ist, wave, ga + is.t * g.a_slice + koff, gb + is.u * g.b_slice + koff, rb);
+        ist = ist + 1 == STG ? 0 : ist + 1;
         ikb += CK;
         if (ikb >= (int64_t)is.klen * esz) {
             ikb = 0;
@@ -255,40 +268,41 @@ __global__ __launch_bounds__(CNT, 1) void oz_gemm_kernel(OzArgs g) {
         }
     };
 
-    iv4 acci[FR][FR];
-    fv4 accf[FR][FR];
+    iv4 acci[FM][FN];
+    fv4 accf[FM][FN];
 #pragma unroll
-    for (int i = 0; i < FR; ++i)
+    for (int i = 0; i < FM; ++i)
 #pragma unroll
-        for (int j = 0; j < FR; ++j) {
+        for (int j = 0; j < FN; ++j) {
             if constexpr (BF16) accf[i][j] = fv4{0.f, 0.f, 0.f, 0.f};
             else acci[i][j] = iv4{0, 0, 0, 0};
         }
 
     const int64_t nsteps = g.nsteps;
-    issue_next();
-    if (nsteps > 1) issue_next();
-    if (nsteps > 2) issue_next();
+    for (int64_t s = 0; s < STG - 1 && s < nsteps; ++s) issue_next();
+    int cst = 0;  // stage of step kt
     const int64_t mrow = m0 + wm + lr;
     const int64_t ncol = n0 + wn + 4 * lg;
-    const bool full = (m0 + OT <= g.m) && (n0 + OT <= g.n);
+    const bool full = (m0 + BM <= g.m) && (n0 + BN <= g.n);
     int64_t kt = 0;
     for (int cseg = 0; cseg < g.nseg; ++cseg) {
         const OzSeg cs = g.segs[cseg];
         const int64_t kend = kt + (int64_t)cs.klen * esz / CK;
         for (; kt < kend; ++kt) {
-            wait_step(nsteps - 1 - kt);
+            const int64_t rem = nsteps - 1 - kt;
+            wait_step(rem < STG - 2 ? rem : STG - 2);
             __builtin_amdgcn_s_barrier();
-            if (kt + 3 < nsteps) issue_next();
-            const int8_t* st = lds + (int)(kt & (CSTG - 1)) * CSTAGE;
-            stage_mma<BF16>(acci, accf, st + wm * CK + roff, st + OT * CK + wn * CK + roff);
+            if (kt + STG - 1 < nsteps) issue_next();
+            const int8_t* st = lds + cst * SSZ;
+            stage_mma<BF16>(acci, accf, st + wm * CK + roff, st + BM * CK + wn * CK + roff);
+            cst = cst + 1 == STG ? 0 : cst + 1;
         }
         if (cs.flush) {
             oz_flush<BF16>(acci, accf, g, ldexp(1.0, -g.bits * cs.level), mrow, ncol, cs.flush, !full);
 #pragma unroll
-            for (int i = 0; i < FR; ++i)
+            for (int i = 0; i < FM; ++i)
 #pragma unroll
-                for (int j = 0; j < FR; ++j) {
+                for (int j = 0; j < FN; ++j) {
                     if constexpr (BF16) accf[i][j] = fv4{0.f, 0.f, 0.f, 0.f};
                     else acci[i][j] = iv4{0, 0, 0, 0};
                 }
@@ -299,13 +313,15 @@ __global__ __launch_bounds__(CNT, 1) void oz_gemm_kernel(OzArgs g) {
 // ------------------------------------------------------------ Ozaki scheme II
 // Ozaki, Uchino, Imamura, "Ozaki Scheme II: A GEMM-oriented emulation of
 // floating-point matrix multiplication using an integer modular technique"
-// (2025).  A' = trunc(op(A)_ik * 2^(q - e_i)) and B' likewise are integers
-// below 2^q; V = A'B' is computed modulo N pairwise coprime moduli p_t <= 256
-// (one INT8 GEMM each, residues balanced into [-128, 127]) and rebuilt
-// exactly by the Chinese remainder theorem, V = sum_t x_t (P/p_t) mod P,
-// balanced; q is chosen so that k * 4^q <= P/4 (P = prod p_t), hence
-// |V| < P/4 is recovered without ambiguity.  C = alpha * ldexp(double(V), e_i + f_j - 2q) + beta*C
-// with double(V) correctly rounded.
+// (2025).  A' = rint(op(A)_ik * 2^(q - e_i)) (rounded to nearest, which
+// halves the RMS scaling error of truncation) and B' likewise are integers
+// of magnitude <= 2^q; V = A'B' is computed modulo N pairwise coprime moduli
+// p_t <= 256 (one INT8 GEMM each, residues balanced into [-128, 127]) and
+// rebuilt exactly by the Chinese remainder theorem, V = sum_t x_t (P/p_t)
+// mod P, balanced; q is chosen so that k * 4^q <= P/4 (P = prod p_t), hence
+// |V| <= P/4 is recovered without ambiguity.
+// C = alpha * ldexp(double(V), e_i + f_j - 2q) + beta*C with double(V)
+// correctly rounded.
 
 constexpr int kMaxModuli = 16;
 // Pairwise coprime: 2^8, 3*5*17, 11*23, 13*19, 7*31 and primes.
@@ -317,10 +333,11 @@ __constant__ int c_moduli[kMaxModuli] = {256, 255, 253, 251, 247, 241, 239, 233,
 struct CrtTables {
     int32_t n;                          // moduli in use
     int32_t cinv[kMaxModuli];           // M_t^-1 mod p_t
-    uint32_t mt[kMaxModuli][4];         // M_t, 32-bit limbs, least significant first
+    double mt[kMaxModuli][4];           // M_t, 32-bit limbs as doubles, least significant first
     uint32_t P[4];                      // P = prod p_t
     uint32_t Ph[4];                     // floor(P / 2)
     double Pd;                          // P rounded to double
+    double invPd;                       // 1 / Pd rounded to double
 };
 
 // p_t as a constant expression, so that x % crt_modulus(t) in an unrolled
@@ -340,52 +357,60 @@ __device__ __forceinline__ int mod_exact(double x, int p, double inv_p) {
     return (int)r;
 }
 
-// Balanced residues of A' = trunc(op(X)(row, k) * 2^(q - e_row)) for every
-// modulus, [t][row][Kp] int8 (zeros in the padding); 64x64 tiles through LDS
-// as oz_slice_kernel.
-__global__ void crt_residue_kernel(const double* __restrict__ x, int64_t rows, int64_t K, int64_t rs, int64_t ks,
-                                   const int32_t* __restrict__ e, int q, int nmod, int8_t* __restrict__ out,
-                                   int64_t Kp, int64_t slice_bytes) {
-    __shared__ double tile[64][65];
-    const int64_t r0 = (int64_t)blockIdx.y * 64;
-    const int64_t k0 = (int64_t)blockIdx.x * 64;
-    const int tx = threadIdx.x & 63;
-    const int ty = threadIdx.x >> 6;
+// Balanced residues of A' = rint(op(X)(row, k) * 2^(q - e_row)) for every
+// modulus, [t][row][Kp] int8 (zeros in the padding).  A workgroup stages a
+// 32-row x 128-k tile of op(X) through LDS (loaded along whichever direction
+// is contiguous in memory); each thread then turns 16 consecutive k of one
+// row into one 16-byte store per modulus, so a wave writes whole 128-byte
+// lines.  Residue: q = rint(a/p) from a*fl(1/p), r = a - q*p exact by FMA
+// (|r| <= p/2 + p), then one step into [-floor(p/2), ceil(p/2) - 1].
+constexpr int RT_R = 32, RT_K = 128;
+__global__ __launch_bounds__(256) void crt_residue_kernel(const double* __restrict__ x, int64_t rows, int64_t K,
+                                                          int64_t rs, int64_t ks, const int32_t* __restrict__ e, int q,
+                                                          int nmod, int8_t* __restrict__ out, int64_t Kp,
+                                                          int64_t slice_bytes) {
+    __shared__ double tile[RT_R][RT_K + 1];
+    const int64_t r0 = (int64_t)blockIdx.y * RT_R;
+    const int64_t k0 = (int64_t)blockIdx.x * RT_K;
+    const int t = threadIdx.x;
     if (ks == 1) {
-        for (int rr = ty; rr < 64; rr += 4) {
-            const int64_t r = r0 + rr, k = k0 + tx;
-            tile[rr][tx] = (r < rows && k < K) ? x[r * rs + k] : 0.0;
+#pragma unroll 4
+        for (int rr = t >> 7; rr < RT_R; rr += 2) {
+            const int64_t r = r0 + rr, k = k0 + (t & 127);
+            tile[rr][t & 127] = (r < rows && k < K) ? x[r * rs + k] : 0.0;
         }
     } else {
-        for (int kk = ty; kk < 64; kk += 4) {
-            const int64_t r = r0 + tx, k = k0 + kk;
-            tile[tx][kk] = (r < rows && k < K) ? x[r * rs + k * ks] : 0.0;
+#pragma unroll 4
+        for (int kk = t >> 5; kk < RT_K; kk += 8) {
+            const int64_t r = r0 + (t & 31), k = k0 + kk;
+            tile[t & 31][kk] = (r < rows && k < K) ? x[r * rs + k * ks] : 0.0;
         }
     }
     __syncthreads();
-    // thread: row t/4 of the tile, 16 consecutive k -> one 16-byte store per modulus
-    const int rr = threadIdx.x >> 2;
-    const int kq = (threadIdx.x & 3) * 16;
+    const int rr = t >> 3;
+    const int kq = (t & 7) * 16;
     const int64_t r = r0 + rr;
     double a[16];
     const int sh = r < rows ? q - e[r] : 0;
 #pragma unroll
-    for (int c = 0; c < 16; ++c) a[c] = r < rows ? trunc(ldexp(tile[rr][kq + c], sh)) : 0.0;
+    for (int c = 0; c < 16; ++c) a[c] = r < rows ? rint(ldexp(tile[rr][kq + c], sh)) : 0.0;
     int8_t* dst = out + r * Kp + k0 + kq;
 #pragma unroll
-    for (int t = 0; t < kMaxModuli; ++t) {
-        if (t < nmod) {
+    for (int m = 0; m < kMaxModuli; ++m) {
+        if (m < nmod) {
             constexpr double one = 1.0;
-            const int p = crt_modulus(t);
-            const double inv_p = one / crt_modulus(t);
+            const int p = crt_modulus(m);
+            const double pd = p, inv_p = one / crt_modulus(m);
             uint32_t w[4] = {0, 0, 0, 0};
 #pragma unroll
             for (int c = 0; c < 16; ++c) {
-                int v = mod_exact(a[c], p, inv_p);
+                const double qd = rint(a[c] * inv_p);
+                int v = (int)fma(-qd, pd, a[c]);
                 if (2 * v >= p) v -= p;
-                w[c >> 2] |= (uint32_t)(uint8_t)(int8_t)v << (8 * (c & 3));
+                if (2 * v < -p) v += p;
+                w[c >> 2] |= (uint32_t)(v & 255) << (8 * (c & 3));
             }
-            *reinterpret_cast<iv4*>(dst + t * slice_bytes) = iv4{(int)w[0], (int)w[1], (int)w[2], (int)w[3]};
+            *reinterpret_cast<iv4*>(dst + m * slice_bytes) = iv4{(int)w[0], (int)w[1], (int)w[2], (int)w[3]};
         }
     }
 }
@@ -402,7 +427,7 @@ struct CrtArgs {
 // One INT8 GEMM per modulus in one launch: workgroup b computes tile
 // b % tiles of modulus b / tiles through the product pipeline and stores
 // (A'_t B'_t) mod p_t as bytes.
-__global__ __launch_bounds__(CNT, 1) void crt_gemm_kernel(CrtArgs g) {
+__global__ __launch_bounds__(CNT, OCC) void crt_gemm_kernel(CrtArgs g) {
     extern __shared__ __attribute__((aligned(1024))) int8_t lds[];
     const int tiles = g.tiles_m * g.tiles_n;
     const int tmod = blockIdx.x / tiles;
@@ -411,49 +436,65 @@ __global__ __launch_bounds__(CNT, 1) void crt_gemm_kernel(CrtArgs g) {
 
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int wm = (wave >> 1) * 128;
-    const int wn = (wave & 1) * 128;
+    const int wm = (wave / (BN / WN)) * WM;
+    const int wn = (wave % (BN / WN)) * WN;
     const int lr = lane & 15;
     const int lg = lane >> 4;
 
     const int lrow = lane >> 2;
     const int lchunk = (lane & 3) ^ swz(lrow);
-    const int8_t* ga = g.a + tmod * g.a_slice + (m0 + 64 * wave + lrow) * g.Kp + 16 * lchunk;
-    const int8_t* gb = g.b + tmod * g.b_slice + (n0 + 64 * wave + lrow) * g.Kp + 16 * lchunk;
+    const int8_t* ga = g.a + tmod * g.a_slice + (m0 + 16 * LA * wave + lrow) * g.Kp + 16 * lchunk;
+    const int8_t* gb = g.b + tmod * g.b_slice + (n0 + 16 * LB * wave + lrow) * g.Kp + 16 * lchunk;
     const int roff = lr * CK + 16 * (lg ^ swz(lr));
 
-    iv4 acc[FR][FR];
-    fv4 unused[FR][FR];
+    iv4 acc[FM][FN];
+    fv4 unused[FM][FN];
 #pragma unroll
-    for (int i = 0; i < FR; ++i)
+    for (int i = 0; i < FM; ++i)
 #pragma unroll
-        for (int j = 0; j < FR; ++j) acc[i][j] = iv4{0, 0, 0, 0};
+        for (int j = 0; j < FN; ++j) acc[i][j] = iv4{0, 0, 0, 0};
 
     const int64_t nk = g.Kp / CK;
-    for (int64_t s = 0; s < 3 && s < nk; ++s) issue_step(lds, (int)s, wave, ga + s * CK, gb + s * CK, g.Kp);
+    for (int64_t s = 0; s < STG - 1 && s < nk; ++s) issue_step(lds, (int)s, wave, ga + s * CK, gb + s * CK, g.Kp);
+    int cst = 0, ist = STG - 1;  // stages of steps kt and kt + STG - 1
     for (int64_t kt = 0; kt < nk; ++kt) {
-        wait_step(nk - 1 - kt);
+        const int64_t rem = nk - 1 - kt;
+        wait_step(rem < STG - 2 ? rem : STG - 2);
         __builtin_amdgcn_s_barrier();
-        if (kt + 3 < nk)
-            issue_step(lds, (int)((kt + 3) & (CSTG - 1)), wave, ga + (kt + 3) * CK, gb + (kt + 3) * CK, g.Kp);
-        const int8_t* st = lds + (int)(kt & (CSTG - 1)) * CSTAGE;
-        stage_mma<false>(acc, unused, st + wm * CK + roff, st + OT * CK + wn * CK + roff);
+        if (kt + STG - 1 < nk)
+            issue_step(lds, ist, wave, ga + (kt + STG - 1) * CK, gb + (kt + STG - 1) * CK, g.Kp);
+        const int8_t* st = lds + cst * SSZ;
+        stage_mma<false>(acc, unused, st + wm * CK + roff, st + BM * CK + wn * CK + roff);
+        cst = cst + 1 == STG ? 0 : cst + 1;
+        ist = ist + 1 == STG ? 0 : ist + 1;
     }
     // acc[i][j][v] = V_t(m0 + wm + 16i + lr, n0 + wn + 16j + 4lg + v)
     const int p = c_moduli[tmod];
     const double inv_p = 1.0 / p;
     uint8_t* out = g.res + (int64_t)tmod * g.Mp * g.Np;
 #pragma unroll
-    for (int j = 0; j < 8; ++j)
+    for (int j = 0; j < FN; ++j)
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             const int64_t n = n0 + wn + 16 * j + 4 * lg + v;
 #pragma unroll
-            for (int i = 0; i < 8; ++i) {
+            for (int i = 0; i < FM; ++i) {
                 const int64_t m = m0 + wm + 16 * i + lr;
                 out[n * g.Mp + m] = (uint8_t)mod_exact((double)acc[i][j][v], p, inv_p);
             }
         }
+}
+
+// x mod p for x < 2^16 with q = floor(x / p) = (2x * ceil(2^31 / p)) >> 32,
+// exact for p <= 256 (the product's excess over 2x * 2^31 / p stays below
+// 2^-15 < 1/p); both factors fit 24 bits, so it compiles to 24-bit multiplies.
+__host__ __device__ constexpr uint32_t crt_magic(int t) {
+    return (uint32_t)((((uint64_t)1 << 31) + (uint64_t)crt_modulus(t) - 1) / (uint64_t)crt_modulus(t));
+}
+__device__ __forceinline__ uint32_t mod_u16(uint32_t x, uint32_t p, uint32_t magic) {
+    x &= 0xFFFFu;
+    const uint32_t q = (uint32_t)(((uint64_t)(x << 1) * magic) >> 32);
+    return x - q * p;
 }
 
 // The balanced CRT value of one element from its limb sums
@@ -472,10 +513,10 @@ __host__ __device__ inline double crt_value(const uint64_t (&Le)[4], const CrtTa
         s3 = (uint32_t)c3;
         s4 = (uint32_t)(c3 >> 32);
     }
-    // quotient estimate (S < 16 P, error of the estimate at most one)
+    // quotient estimate (S < 16 P; with S and 1/P rounded the estimate is off by at most one)
     const double Sd = (((((double)s4 * 4294967296.0 + (double)s3) * 4294967296.0) + (double)s2) * 4294967296.0 +
                        (double)s1) * 4294967296.0 + (double)s0;
-    uint32_t qt = (uint32_t)floor(Sd / tb.Pd);
+    uint32_t qt = (uint32_t)floor(Sd * tb.invPd);
     if (qt > 16) qt = 16;
     // R = S - qt * P (five limbs, two's complement)
     int64_t r0, r1, r2, r3, r4;
@@ -543,23 +584,27 @@ __host__ __device__ inline double crt_value(const uint64_t (&Le)[4], const CrtTa
 }
 
 // Four consecutive rows per thread (one 32-bit load of residues per modulus).
-// S = sum_t x_t M_t < n P is accumulated exactly in four 64-bit limb sums of
-// 32-bit products, reduced to R = S mod P with a floating quotient estimate
-// and one correction, V = R or R - P (balanced), then converted to the
-// correctly rounded double; C = alpha * ldexp(V, e_i + f_j - 2q) + beta*C.
+// The limb sums L_l = sum_t x_t * limb_l(M_t) are exact FP64 FMAs (x_t < 256,
+// limbs < 2^32, at most 16 terms: below 2^44); crt_value reduces S = sum_l
+// L_l 2^(32 l) < 16 P modulo P with a floating quotient estimate and one
+// correction, balances it and rounds it to double once.
+// C = alpha * ldexp(V, e_i + f_j - 2q) + beta*C, four rows per 32-byte access
+// when C allows it.
 __global__ void crt_reconstruct_kernel(const uint8_t* __restrict__ res, CrtTables tb, int64_t m, int64_t n,
                                        int64_t Mp, int64_t Np, const int32_t* __restrict__ ea,
                                        const int32_t* __restrict__ eb, int q, double alpha, double beta,
                                        double* __restrict__ C, int64_t ldc) {
     const int64_t qm = (m + 3) / 4;
     const int64_t total = qm * n;
-    for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x < total; x += (int64_t)gridDim.x * blockDim.x) {
+    {
+        const int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;  // one thread per 4 rows
+        if (x >= total) return;
         const int64_t j = x / qm, i0 = (x - j * qm) * 4;
-        uint64_t L[4][4];
+        double L[4][4];
 #pragma unroll
         for (int e = 0; e < 4; ++e)
 #pragma unroll
-            for (int l = 0; l < 4; ++l) L[e][l] = 0;
+            for (int l = 0; l < 4; ++l) L[e][l] = 0.0;
         const uint8_t* rp = res + j * Mp + i0;
 #pragma unroll
         for (int t = 0; t < kMaxModuli; ++t) {
@@ -568,22 +613,38 @@ __global__ void crt_reconstruct_kernel(const uint8_t* __restrict__ res, CrtTable
                 const uint32_t c = (uint32_t)tb.cinv[t];
 #pragma unroll
                 for (int e = 0; e < 4; ++e) {
-                    const uint32_t xr = (((w >> (8 * e)) & 255u) * c) % (uint32_t)crt_modulus(t);
+                    const uint32_t xr = mod_u16(__umul24((w >> (8 * e)) & 255u, c), crt_modulus(t), crt_magic(t));
+                    const double xd = (double)xr;
 #pragma unroll
-                    for (int l = 0; l < 4; ++l) L[e][l] += (uint64_t)xr * tb.mt[t][l];
+                    for (int l = 0; l < 4; ++l) L[e][l] = fma(xd, tb.mt[t][l], L[e][l]);
                 }
             }
         }
-        const int64_t jf = eb[j];
+        const int jf = eb[j];
+        double val[4];
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
-            const int64_t i = i0 + e;
-            if (i >= m) break;
-            const double v = crt_value(L[e], tb);
-            double val = __dmul_rn(alpha, ldexp(v, ea[i] + (int)jf - 2 * q));
-            double* cp = C + i + j * ldc;
-            if (beta != 0.0) val = __dadd_rn(val, __dmul_rn(beta, *cp));
-            *cp = val;
+            const uint64_t Le[4] = {(uint64_t)L[e][0], (uint64_t)L[e][1], (uint64_t)L[e][2], (uint64_t)L[e][3]};
+            const int64_t i = i0 + e < m ? i0 + e : m - 1;  // rows past m are computed but never stored
+            val[e] = __dmul_rn(alpha, ldexp(crt_value(Le, tb), ea[i] + jf - 2 * q));
+        }
+        double* cp = C + i0 + j * ldc;
+        if (i0 + 3 < m && (reinterpret_cast<uintptr_t>(cp) & 31) == 0) {
+            dv4* c4 = reinterpret_cast<dv4*>(cp);
+            if (beta != 0.0) {
+                const dv4 old = *c4;
+#pragma unroll
+                for (int e = 0; e < 4; ++e) val[e] = __dadd_rn(val[e], __dmul_rn(beta, old[e]));
+            }
+            *c4 = dv4{val[0], val[1], val[2], val[3]};
+        } else {
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                if (i0 + e >= m) break;
+                double v = val[e];
+                if (beta != 0.0) v = __dadd_rn(v, __dmul_rn(beta, cp[e]));
+                cp[e] = v;
+            }
         }
     }
 }
@@ -718,7 +779,9 @@ int crt_setup(int64_t k, int nmod, CrtTables* tb, int* q_out) {
     for (int t = 0; t < nmod; ++t) {
         const int p = crt_modulus(t);
         const unsigned __int128 M = P / (unsigned)p;
-        limbs(M, tb->mt[t]);
+        uint32_t ml[4];
+        limbs(M, ml);
+        for (int l = 0; l < 4; ++l) tb->mt[t][l] = (double)ml[l];
         // inverse of M mod p by extended Euclid (M and p are coprime)
         int64_t a = (int64_t)(M % (unsigned)p), b = p, x0 = 1, x1 = 0;
         while (b) {
@@ -732,14 +795,30 @@ int crt_setup(int64_t k, int nmod, CrtTables* tb, int* q_out) {
     limbs(P, tb->P);
     limbs(P / 2, tb->Ph);
     tb->Pd = (double)P;
+    tb->invPd = 1.0 / tb->Pd;
     return DG_OK;
 }
+
+}  // namespace
+
+int crt_moduli_for_fp64(int64_t k) {
+    if (k >= ((int64_t)1 << 19)) return kMaxModuli;
+    unsigned __int128 P = 1;
+    const unsigned __int128 need = (unsigned __int128)(k > 0 ? k : 1) << (2 * 53 + 2);
+    for (int t = 0; t < kMaxModuli; ++t) {
+        P *= (unsigned)crt_modulus(t);
+        if (t >= 1 && need <= P) return t + 1;
+    }
+    return kMaxModuli;
+}
+
+namespace {
 
 Plan make_crt_plan(int64_t m, int64_t n, int64_t k, int nmod) {
     Plan p{};
     p.esz = 1;
-    p.Mp = round_up(m > 0 ? m : 1, OT);
-    p.Np = round_up(n > 0 ? n : 1, OT);
+    p.Mp = round_up(m > 0 ? m : 1, PAD);
+    p.Np = round_up(n > 0 ? n : 1, PAD);
     p.Kp = round_up(k > 0 ? k : 1, KSB);
     p.a_slice = p.Mp * p.Kp;
     p.b_slice = p.Np * p.Kp;
@@ -763,8 +842,8 @@ Plan make_crt_plan(int64_t m, int64_t n, int64_t k, int nmod) {
 Plan make_plan(int64_t m, int64_t n, int64_t k, int s, bool bf16) {
     Plan p{};
     p.esz = bf16 ? 2 : 1;
-    p.Mp = round_up(m > 0 ? m : 1, OT);
-    p.Np = round_up(n > 0 ? n : 1, OT);
+    p.Mp = round_up(m > 0 ? m : 1, PAD);
+    p.Np = round_up(n > 0 ? n : 1, PAD);
     p.Kp = round_up(k > 0 ? k : 1, KSB / p.esz);
     p.a_slice = p.Mp * p.Kp * p.esz;
     p.b_slice = p.Np * p.Kp * p.esz;
@@ -883,7 +962,7 @@ int dg_ozaki_dgemm(char opA, char opB, int64_t m, int64_t n, int64_t k, double a
     if (!A || !B || !C) return fail(DG_ERR_INVALID, "dg_ozaki_dgemm: null pointer");
     if (k > INT32_MAX / 2) return fail(DG_ERR_INVALID, "dg_ozaki_dgemm: k too large");
     const Plan p = crt ? make_crt_plan(m, n, k, slices) : make_plan(m, n, k, slices, bf16);
-    if (p.Mp / 64 > 65535 || p.Np / 64 > 65535 || p.Kp / 64 > INT32_MAX)
+    if (p.Mp / 32 > 65535 || p.Np / 32 > 65535 || p.Kp / 64 > INT32_MAX)
         return fail(DG_ERR_UNSUPPORTED, "dg_ozaki_dgemm: matrix too large for one slicing launch");
     if (!workspace || workspace_bytes < (int64_t)p.total)
         return fail(DG_ERR_WORKSPACE, "dg_ozaki_dgemm: workspace %lld < %zu bytes", (long long)workspace_bytes, p.total);
@@ -926,23 +1005,23 @@ int dg_ozaki_dgemm(char opA, char opB, int64_t m, int64_t n, int64_t k, double a
         DG_LAUNCH_CHECK();
     }
     if (crt) {
-        dim3 ga((unsigned)(p.Kp / 64), (unsigned)(p.Mp / 64));
-        dim3 gb((unsigned)(p.Kp / 64), (unsigned)(p.Np / 64));
-        hipLaunchKernelGGL(crt_residue_kernel, ga, dim3(NT), 0, st, A, m, k, a_rs, a_ks, ea, q, slices, sa, p.Kp,
+        dim3 ga((unsigned)(p.Kp / RT_K), (unsigned)(p.Mp / RT_R));
+        dim3 gb((unsigned)(p.Kp / RT_K), (unsigned)(p.Np / RT_R));
+        hipLaunchKernelGGL(crt_residue_kernel, ga, dim3(256), 0, st, A, m, k, a_rs, a_ks, ea, q, slices, sa, p.Kp,
                            p.a_slice);
-        hipLaunchKernelGGL(crt_residue_kernel, gb, dim3(NT), 0, st, B, n, k, b_rs, b_ks, eb, q, slices, sb, p.Kp,
+        hipLaunchKernelGGL(crt_residue_kernel, gb, dim3(256), 0, st, B, n, k, b_rs, b_ks, eb, q, slices, sb, p.Kp,
                            p.b_slice);
         DG_LAUNCH_CHECK();
         CrtArgs cg{};
         cg.a = sa; cg.b = sb; cg.res = reinterpret_cast<uint8_t*>(ws + p.off_w);
         cg.Mp = p.Mp; cg.Np = p.Np; cg.Kp = p.Kp; cg.a_slice = p.a_slice; cg.b_slice = p.b_slice;
-        cg.tiles_m = (int32_t)(p.Mp / OT); cg.tiles_n = (int32_t)(p.Np / OT); cg.nmod = slices;
+        cg.tiles_m = (int32_t)(p.Mp / BM); cg.tiles_n = (int32_t)(p.Np / BN); cg.nmod = slices;
         const int64_t grid = (int64_t)cg.tiles_m * cg.tiles_n * slices;
         if (grid > INT32_MAX) return fail(DG_ERR_INVALID, "dg_ozaki_dgemm: too many tiles");
         hipLaunchKernelGGL(crt_gemm_kernel, dim3((unsigned)grid), dim3(CNT), kGemmLds, st, cg);
         DG_LAUNCH_CHECK();
-        int64_t blocks = (m * n + 255) / 256;
-        if (blocks > 65536) blocks = 65536;
+        const int64_t blocks = (((m + 3) / 4) * n + 255) / 256;
+        if (blocks > INT32_MAX) return fail(DG_ERR_INVALID, "dg_ozaki_dgemm: too many output elements");
         hipLaunchKernelGGL(crt_reconstruct_kernel, dim3((unsigned)blocks), dim3(256), 0, st, cg.res, tables, m, n, p.Mp,
                            p.Np, ea, eb, q, alpha, beta, C, ldc);
         DG_LAUNCH_CHECK();
@@ -972,8 +1051,8 @@ int dg_ozaki_dgemm(char opA, char opB, int64_t m, int64_t n, int64_t k, double a
     g.ea = ea; g.eb = eb; g.segs = dsched; g.nseg = (int32_t)sched.size(); g.bits = bits;
     g.m = m; g.n = n; g.Mp = p.Mp; g.Np = p.Np; g.Kp = p.Kp; g.a_slice = p.a_slice; g.b_slice = p.b_slice;
     g.alpha = alpha; g.beta = beta; g.c = C; g.ldc = ldc;
-    g.tiles_m = (int32_t)(p.Mp / OT);
-    g.tiles_n = (int32_t)(p.Np / OT);
+    g.tiles_m = (int32_t)(p.Mp / BM);
+    g.tiles_n = (int32_t)(p.Np / BN);
     g.nsteps = 0;
     for (const OzSeg& sg : sched) g.nsteps += (int64_t)sg.klen * p.esz / CK;  // klen*esz is a multiple of KSB
     const int64_t tiles = (int64_t)g.tiles_m * g.tiles_n;

@@ -43,6 +43,8 @@ _i64p = ctypes.POINTER(ctypes.c_int64)
 DG_PLAN_AUTOTUNE = 0
 DG_PLAN_NONPERSISTENT = 1
 DG_PLAN_PERSISTENT = 2
+DG_FORM_FP64 = 0
+DG_FORM_OZAKI_INT8 = 1
 
 
 class GemmDesc(ctypes.Structure):
@@ -82,6 +84,8 @@ SIGNATURES = {
     "dg_generate": (_int, [_u64, _int, _ptr, _i64, _ptr]),
     "dg_grouped_dgemm": (_int, [_descp, _int, _ptr]),
     "dg_grouped_plan_create": (_int, [_descp, _int, _int, ctypes.POINTER(ctypes.c_void_p)]),
+    "dg_grouped_plan_create_form": (_int, [_descp, _int, _int, _int, _int, ctypes.POINTER(ctypes.c_void_p)]),
+    "dg_grouped_plan_problem_form": (_int, [ctypes.c_void_p, _int, ctypes.POINTER(_int)]),
     "dg_grouped_plan_execute": (_int, [_ptr, _ptr]),
     "dg_grouped_plan_graph_launch": (_int, [_ptr, _ptr]),
     "dg_grouped_plan_variant": (_int, [_ptr]),
@@ -272,11 +276,21 @@ def grouped_dgemm_t(descs):
 class GroupedPlan:
     """dg_grouped_plan over a list of GemmDesc; destroyed with close()."""
 
-    def __init__(self, descs, mode=DG_PLAN_AUTOTUNE):
+    def __init__(self, descs, mode=DG_PLAN_AUTOTUNE, form=DG_FORM_FP64, moduli=0):
         arr = _desc_array(descs)
         h = ctypes.c_void_p()
-        check("dg_grouped_plan_create", lib().dg_grouped_plan_create(arr, len(descs), mode, ctypes.byref(h)))
+        if form == DG_FORM_FP64 and moduli == 0:
+            check("dg_grouped_plan_create", lib().dg_grouped_plan_create(arr, len(descs), mode, ctypes.byref(h)))
+        else:
+            check("dg_grouped_plan_create_form",
+                  lib().dg_grouped_plan_create_form(arr, len(descs), mode, form, moduli, ctypes.byref(h)))
         self.handle = h
+
+    def problem_form(self, index):
+        """(DG_FORM_*, moduli) chosen for problem `index`; (-1, 0) before the first execution."""
+        mod = ctypes.c_int(0)
+        f = lib().dg_grouped_plan_problem_form(self.handle, index, ctypes.byref(mod))
+        return f, mod.value
 
     def execute(self):
         check("dg_grouped_plan_execute", lib().dg_grouped_plan_execute(self.handle, _stream()))

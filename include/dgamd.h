@@ -251,6 +251,29 @@ int dg_ozaki_dgemm(char opA, char opB, int64_t m, int64_t n, int64_t k, double a
                    double beta, double* C, int64_t ldc, int slices, int kind,
                    void* workspace, int64_t workspace_bytes, double* err_bound, void* stream);
 
+/* ------------------------------------------- grouped batch, Ozaki form -- */
+
+#define DG_FORM_FP64 0        /* FP64 MFMA for every problem                           */
+#define DG_FORM_OZAKI_INT8 1  /* per problem, Ozaki scheme II (DG_OZAKI_INT8_CRT) where it
+                                 is measured faster than the FP64 MFMA path, else FP64  */
+
+/* A grouped plan in the given arithmetic form.  DG_FORM_FP64 is
+ * dg_grouped_plan_create.  With DG_FORM_OZAKI_INT8, `moduli` is the CRT
+ * moduli count (2..16), or 0 to take per problem the smallest count whose
+ * scaling reaches q = 53 bits (the FP64-level bound k*2^-52, 16 when none
+ * does).  On the first execution the autotuner times each problem of at
+ * least 2^32 flops both ways on its own operands (outputs restored
+ * afterwards; choices cached per shape and moduli count) and keeps the
+ * faster; smaller problems stay FP64.  The FP64 problems still run as one
+ * grouped launch; the Ozaki problems follow on the same stream, sharing one
+ * workspace the plan owns; dg_grouped_plan_graph_launch captures all of it
+ * in one HIP graph.  New relative to the reference. */
+int dg_grouped_plan_create_form(const dg_gemm_desc* descs, int count, int mode, int form, int moduli,
+                                dg_grouped_plan** plan);
+/* Form chosen for problem `index` (DG_FORM_*; -1 before the first execution
+ * or on a bad index); *moduli receives its moduli count (0 for FP64). */
+int dg_grouped_plan_problem_form(const dg_grouped_plan* plan, int index, int* moduli);
+
 /* ------------------------------------------------------------ generation -- */
 
 #define DG_DIST_UNIFORM_PM1 0  /* u in [-1, 1): 2*(x>>11)*2^-53 - 1                 */

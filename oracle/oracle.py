@@ -324,9 +324,10 @@ def ozaki_dgemm(a, b, s, alpha=1.0, beta=0.0, c=None, bits=OZ_INT8_BITS, kchunk=
 # ---------------------------------------------------------------- Ozaki scheme II (CRT)
 # Ozaki, Uchino, Imamura, "Ozaki Scheme II: A GEMM-oriented emulation of
 # floating-point matrix multiplication using an integer modular technique"
-# (2025).  Rows of A / columns of B are scaled to integers of q bits; their
-# exact integer product is computed modulo N pairwise coprime moduli <= 256
-# (one INT8 GEMM each) and rebuilt by the Chinese remainder theorem.
+# (2025).  Rows of A / columns of B are scaled to integers of q bits (rounded
+# to nearest); their exact integer product is computed modulo N pairwise
+# coprime moduli <= 256 (one INT8 GEMM each) and rebuilt by the Chinese
+# remainder theorem.
 
 CRT_MODULI = (256, 255, 253, 251, 247, 241, 239, 233, 229, 227, 223, 217, 211, 199, 197, 193, 191, 181, 179, 173)
 
@@ -352,10 +353,12 @@ def crt_bal(x, p):
 
 
 def crt_scale(X, e, axis, q):
-    """Integers trunc(x * 2^(q - e)) (exact in FP64 for q <= 53) as int64."""
+    """Integers rint(x * 2^(q - e)), rounded to nearest (ties to even), as
+    int64: the scaling is exact in FP64 for q <= 53 and |x| < 2^e gives
+    |result| <= 2^q."""
     shape = [1, 1]
     shape[1 - axis] = -1
-    return np.trunc(np.ldexp(X, (q - e).reshape(shape))).astype(np.int64)
+    return np.rint(np.ldexp(X, (q - e).reshape(shape))).astype(np.int64)
 
 
 def crt_tables(nmod):
@@ -381,9 +384,6 @@ def crt_reconstruct(res, nmod):
         digits.append(v)
         for u in range(t + 1, nmod):
             s[u] = np.mod(s[u] + v * rmod[t][u], ps[u])
-    out = np.zeros(shape, dtype=object)
-    for t in range(nmod - 1, -1, -1):
-        out = out * (ps[t - 1] if t else 1) if False else out
     total = np.zeros(shape, dtype=object)
     for t in range(nmod):
         total = total + digits[t].astype(object) * R[t]

@@ -100,6 +100,14 @@ def test_argument_errors_need_no_device():
     assert st == dgamd.DG_ERR_UNSUPPORTED
     st = L.dg_generate(1, 7, None, 10, None)
     assert st == dgamd.DG_ERR_INVALID
+    import ctypes
+    h = ctypes.c_void_p()
+    st = L.dg_grouped_plan_create_form(None, 0, dgamd.DG_PLAN_AUTOTUNE, 7, 0, ctypes.byref(h))
+    assert st == dgamd.DG_ERR_INVALID and "form" in dgamd.last_error()
+    st = L.dg_grouped_plan_create_form(None, 0, dgamd.DG_PLAN_AUTOTUNE, dgamd.DG_FORM_OZAKI_INT8, 1, ctypes.byref(h))
+    assert st == dgamd.DG_ERR_INVALID and "moduli" in dgamd.last_error()
+    mod = ctypes.c_int(5)
+    assert L.dg_grouped_plan_problem_form(None, 0, ctypes.byref(mod)) == -1 and mod.value == 0
 
 
 def test_device_calls_fail_loudly_without_gpu():

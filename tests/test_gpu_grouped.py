@@ -123,3 +123,73 @@ def test_plan_rejects_bad_input(dev):
     bad = dgamd.gemm_desc("N", "N", 10, 10, 10, 1.0, 0, 5, 0, 10, 0.0, 0, 10)
     arr = (dgamd.GemmDesc * 1)(bad)
     assert L.dg_grouped_dgemm(arr, 1, None) == dgamd.DG_ERR_INVALID
+
+
+def test_ozaki_form_plan(dev):
+    """DG_FORM_OZAKI_INT8: the autotuner runs the 8192^3 problem as INT8 CRT
+    emulation (16 moduli, FP64-level scaling), keeps problems under 2^32
+    flops in FP64, and every output stays within 1e-13 of the FP64 MFMA
+    result; a graph replay reproduces the eager outputs bit for bit."""
+    shapes = [(8192, 8192, 8192), (3000, 2000, 5000), (300, 200, 100), (64, 64, 64)]
+    probs = []
+    for t, (m, n, k) in enumerate(shapes):
+        A = torch.empty(m * k, dtype=torch.float64, device=dev)
+        B = torch.empty(k * n, dtype=torch.float64, device=dev)
+        C = torch.empty(m * n, dtype=torch.float64, device=dev)
+        dgamd.generate_t(500 + 3 * t, dgamd.DG_DIST_WIDE, A)
+        dgamd.generate_t(501 + 3 * t, dgamd.DG_DIST_WIDE, B)
+        dgamd.generate_t(502 + 3 * t, dgamd.DG_DIST_UNIFORM_PM1, C)
+        ref = C.clone()
+        dgamd.dgemm_t("N", "N", m, n, k, 1.0, A, m, B, k, 0.5, ref, m)
+        probs.append(dict(m=m, n=n, k=k, A=A, B=B, C=C, C0=C.clone(), ref=ref))
+    descs = [dgamd.gemm_desc("N", "N", p["m"], p["n"], p["k"], 1.0, p["A"], p["m"], p["B"], p["k"], 0.5, p["C"],
+                             p["m"]) for p in probs]
+    plan = dgamd.GroupedPlan(descs, dgamd.DG_PLAN_AUTOTUNE, form=dgamd.DG_FORM_OZAKI_INT8)
+    assert plan.problem_form(0) == (-1, 0)  # chosen on first use
+    plan.execute()
+    torch.cuda.synchronize()
+    forms = [plan.problem_form(i) for i in range(len(probs))]
+    assert forms[0] == (dgamd.DG_FORM_OZAKI_INT8, 16), forms
+    assert forms[2] == (dgamd.DG_FORM_FP64, 0) and forms[3] == (dgamd.DG_FORM_FP64, 0), forms
+    assert forms[1][0] in (dgamd.DG_FORM_FP64, dgamd.DG_FORM_OZAKI_INT8)
+    eager = []
+    for p in probs:
+        err = (torch.linalg.vector_norm(p["C"] - p["ref"]) / torch.linalg.vector_norm(p["ref"])).item()
+        assert err <= TOL, (p["m"], p["n"], p["k"], err)
+        eager.append(p["C"].clone())
+    for p in probs:
+        p["C"].copy_(p["C0"])
+    for _ in range(2):
+        for p in probs:
+            p["C"].copy_(p["C0"])
+        torch.cuda.synchronize()
+        plan.graph_launch()
+        torch.cuda.synchronize()
+        for p, e in zip(probs, eager):
+            assert torch.equal(p["C"], e)
+    plan.close()
+
+
+def test_ozaki_form_fixed_moduli(dev):
+    """An explicit moduli count: the large problem uses it and stays within
+    the scheme's a-priori bound (13 moduli: q = 44 bits at k = 4096)."""
+    m = n = k = 4096
+    A = torch.empty(m * k, dtype=torch.float64, device=dev)
+    B = torch.empty(k * n, dtype=torch.float64, device=dev)
+    C = torch.zeros(m * n, dtype=torch.float64, device=dev)
+    dgamd.generate_t(61, dgamd.DG_DIST_UNIFORM_PM1, A)
+    dgamd.generate_t(62, dgamd.DG_DIST_UNIFORM_PM1, B)
+    ref = torch.empty_like(C)
+    dgamd.dgemm_t("N", "N", m, n, k, 1.0, A, m, B, k, 0.0, ref, m)
+    plan = dgamd.GroupedPlan([dgamd.gemm_desc("N", "N", m, n, k, 1.0, A, m, B, k, 0.0, C, m)],
+                             dgamd.DG_PLAN_NONPERSISTENT, form=dgamd.DG_FORM_OZAKI_INT8, moduli=13)
+    plan.execute()
+    torch.cuda.synchronize()
+    f, mod = plan.problem_form(0)
+    err = (torch.linalg.vector_norm(C - ref) / torch.linalg.vector_norm(ref)).item()
+    if f == dgamd.DG_FORM_OZAKI_INT8:
+        assert mod == 13
+        assert err <= 1e-9, err
+    else:
+        assert (f, mod) == (dgamd.DG_FORM_FP64, 0) and err <= TOL
+    plan.close()

@@ -140,6 +140,17 @@ def test_crt_q_bound():
                 assert q == 53 or K * 4 ** (q + 1) * 4 > P
 
 
+def test_crt_scale_rounds_to_nearest_even():
+    # row exponent 0 and q = 3: the scaled value is the entry itself
+    X = np.array([[0.7, -0.7, 0.5, 1.5, 2.5, -2.5, 0.49, 7.5]]) * 2.0 ** -3
+    out = oracle.crt_scale(X, np.array([0]), 1, 3)
+    assert out.tolist() == [[1, -1, 0, 2, 2, -2, 0, 8]]
+    # column scaling (axis 0) uses one exponent per column
+    Y = np.array([[0.3, 3.0], [-0.3, -1.75]])
+    out = oracle.crt_scale(Y, np.array([-1, 2]), 0, 2)  # column 0 by 2^3, column 1 by 2^0
+    assert out.tolist() == [[2, 3], [-2, -2]]
+
+
 def test_crt_exact_for_short_integers():
     rng = np.random.default_rng(11)
     a = rng.integers(-1000, 1000, (7, 50)).astype(float)
