@@ -59,7 +59,7 @@ constexpr int LPS = LA + LB;
 constexpr int SSZ = (BM + BN) * CK;        // bytes per stage (A rows, then B rows)
 constexpr size_t kGemmLds = (size_t)STG * SSZ;
 static_assert(LA * 16 * NW == BM && LB * 16 * NW == BN, "loader split");
-static_assert(PAD % BM == 0 && PAD % BN == 0 && STG >= 2 && STG <= 5, "tile configuration");
+static_assert(PAD % BM == 0 && PAD % BN == 0 && STG == 4, "tile configuration (the ring index is k & 3)");
 constexpr int NT = 256;
 constexpr int GROUP_M = 8;
 
@@ -114,10 +114,16 @@ __device__ __forceinline__ void mma(iv4& acc_i, fv4& acc_f, const iv4& x, const 
 // filled by global_load_lds_dwordx4, which writes LDS without a register
 // round trip.  One such load of a wave covers 16 rows x 64 bytes; within a
 // row, 16-byte chunk c sits at position c ^ swz(row) so that the 128-bit
-// fragment reads of 8 or 16 consecutive rows hit distinct banks.  Each wave
-// waits for its own loads of step k (vmcnt), then one barrier makes every
-// wave's loads visible and frees the stage read at step k-1, which then
-// receives step k+STG-1.  All fragments of a step are read before its MFMAs.
+// fragment reads of 8 or 16 consecutive rows hit distinct banks.
+// In the scheme II kernel the barrier runs one step ahead: at the end of
+// step k each wave waits for its own loads of step k+2, and one barrier
+// publishes step k+2 and frees the stage of step k, which then receives step
+// k+STG.  The fragments of step k+1 (published a step earlier) are read while
+// the MFMAs of step k run: each A fragment right after its last use, the B
+// fragments into a second register set, so LDS latency hides behind the
+// matrix cores.  The scheme I kernel, whose product boundaries and flushes
+// break that rhythm, publishes step k at its start and reads every fragment
+// before the MFMAs.
 __host__ __device__ constexpr int swz(int r) { return (((r >> 2) & 1) << 1) | (((r >> 1) ^ (r >> 3)) & 1); }
 
 template <int N>
@@ -127,8 +133,8 @@ __device__ __forceinline__ void wait_vmcnt() {
     __builtin_amdgcn_s_waitcnt((N & 0xF) | ((N >> 4) << 14) | (0x7 << 4) | (0xF << 8));
 }
 
-// Waits until this wave's loads of the current step have landed, given the
-// number of later steps already issued (LPS loads each, at most STG-2 steps).
+// Waits until at most `ahead` steps (LPS loads each, ahead <= 3) of this
+// wave's loads are still pending.
 __device__ __forceinline__ void wait_step(int64_t ahead) {
     if (ahead >= 3) wait_vmcnt<3 * LPS>();
     else if (ahead == 2) wait_vmcnt<2 * LPS>();
@@ -154,17 +160,46 @@ __device__ __forceinline__ void issue_step(int8_t* lds, int stage, int wave, con
         __builtin_amdgcn_global_load_lds((gvoid_t*)(pb + 16 * i * rb), (lds_void_t*)(sb + 1024 * i), 16, 0, 0);
 }
 
-// FM x FN MFMAs of one stage: as/bs point at this lane's fragment of the
-// wave's first 16-row block of A and B.
-template <bool BF16>
-__device__ __forceinline__ void stage_mma(iv4 (&acci)[FM][FN], fv4 (&accf)[FM][FN], const int8_t* as,
-                                          const int8_t* bs) {
-    iv4 fb[FN], fa[FM];
-    fa[0] = *reinterpret_cast<const iv4*>(as);
+// Fragments of one step: fa (A) and fb (B) from this lane's place in the
+// wave's first 16-row block of A (as) and B (bs).
+__device__ __forceinline__ void read_frags(iv4 (&fa)[FM], iv4 (&fb)[FN], const int8_t* as, const int8_t* bs) {
 #pragma unroll
     for (int j = 0; j < FN; ++j) fb[j] = *reinterpret_cast<const iv4*>(bs + 1024 * j);
 #pragma unroll
-    for (int i = 1; i < FM; ++i) fa[i] = *reinterpret_cast<const iv4*>(as + 1024 * i);
+    for (int i = 0; i < FM; ++i) fa[i] = *reinterpret_cast<const iv4*>(as + 1024 * i);
+}
+
+// The FM x FN MFMAs of one step on fa / cb, with the next step's fragments
+// read meanwhile: fa[i] right after its last use, the B set into nb.
+template <bool BF16>
+__device__ __forceinline__ void step_mma(iv4 (&acci)[FM][FN], fv4 (&accf)[FM][FN], iv4 (&fa)[FM], const iv4 (&cb)[FN],
+                                         iv4 (&nb)[FN], const int8_t* nas, const int8_t* nbs) {
+#pragma unroll
+    for (int i = 0; i < FM; ++i) {
+#pragma unroll
+        for (int j = 0; j < FN; ++j) mma<BF16>(acci[i][j], accf[i][j], cb[j], fa[i]);
+        if (i == 0) {
+#pragma unroll
+            for (int j = 0; j < FN; ++j) nb[j] = *reinterpret_cast<const iv4*>(nbs + 1024 * j);
+        }
+        fa[i] = *reinterpret_cast<const iv4*>(nas + 1024 * i);
+    }
+    // interleave: FN MFMAs, then the reads they free
+    __builtin_amdgcn_sched_group_barrier(0x008, FN, 0);
+    __builtin_amdgcn_sched_group_barrier(0x100, FN + 1, 0);
+#pragma unroll
+    for (int g = 1; g < FM; ++g) {
+        __builtin_amdgcn_sched_group_barrier(0x008, FN, 0);
+        __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+    }
+}
+
+// The FM x FN MFMAs of one step with every fragment read first (the scheme I
+// kernel, whose product boundaries and flushes break the step-ahead rhythm).
+template <bool BF16>
+__device__ __forceinline__ void stage_mma(iv4 (&acci)[FM][FN], fv4 (&accf)[FM][FN], const int8_t* as, const int8_t* bs) {
+    iv4 fa[FM], fb[FN];
+    read_frags(fa, fb, as, bs);
     __builtin_amdgcn_sched_barrier(0);  // every fragment read ahead of the MFMAs
 #pragma unroll
     for (int i = 0; i < FM; ++i)
@@ -278,9 +313,11 @@ __global__ __launch_bounds__(CNT, OCC) void oz_gemm_kernel(OzArgs g) {
             else acci[i][j] = iv4{0, 0, 0, 0};
         }
 
+    // Scheme I keeps the barrier at the start of each step: wait for this
+    // wave's loads of step k, publish it, refill the stage of step k-1 with
+    // step k+STG-1, read every fragment, then the MFMAs.
     const int64_t nsteps = g.nsteps;
     for (int64_t s = 0; s < STG - 1 && s < nsteps; ++s) issue_next();
-    int cst = 0;  // stage of step kt
     const int64_t mrow = m0 + wm + lr;
     const int64_t ncol = n0 + wn + 4 * lg;
     const bool full = (m0 + BM <= g.m) && (n0 + BN <= g.n);
@@ -293,9 +330,8 @@ __global__ __launch_bounds__(CNT, OCC) void oz_gemm_kernel(OzArgs g) {
             wait_step(rem < STG - 2 ? rem : STG - 2);
             __builtin_amdgcn_s_barrier();
             if (kt + STG - 1 < nsteps) issue_next();
-            const int8_t* st = lds + cst * SSZ;
+            const int8_t* st = lds + (int)(kt & (STG - 1)) * SSZ;
             stage_mma<BF16>(acci, accf, st + wm * CK + roff, st + BM * CK + wn * CK + roff);
-            cst = cst + 1 == STG ? 0 : cst + 1;
         }
         if (cs.flush) {
             oz_flush<BF16>(acci, accf, g, ldexp(1.0, -g.bits * cs.level), mrow, ncol, cs.flush, !full);
@@ -454,20 +490,32 @@ __global__ __launch_bounds__(CNT, OCC) void crt_gemm_kernel(CrtArgs g) {
 #pragma unroll
         for (int j = 0; j < FN; ++j) acc[i][j] = iv4{0, 0, 0, 0};
 
-    const int64_t nk = g.Kp / CK;
-    for (int64_t s = 0; s < STG - 1 && s < nk; ++s) issue_step(lds, (int)s, wave, ga + s * CK, gb + s * CK, g.Kp);
-    int cst = 0, ist = STG - 1;  // stages of steps kt and kt + STG - 1
-    for (int64_t kt = 0; kt < nk; ++kt) {
-        const int64_t rem = nk - 1 - kt;
-        wait_step(rem < STG - 2 ? rem : STG - 2);
-        __builtin_amdgcn_s_barrier();
-        if (kt + STG - 1 < nk)
-            issue_step(lds, ist, wave, ga + (kt + STG - 1) * CK, gb + (kt + STG - 1) * CK, g.Kp);
-        const int8_t* st = lds + cst * SSZ;
-        stage_mma<false>(acc, unused, st + wm * CK + roff, st + BM * CK + wn * CK + roff);
-        cst = cst + 1 == STG ? 0 : cst + 1;
-        ist = ist + 1 == STG ? 0 : ist + 1;
+    const int64_t nk = g.Kp / CK;  // even, >= 2
+    for (int64_t s = 0; s < STG && s < nk; ++s) issue_step(lds, (int)s, wave, ga + s * CK, gb + s * CK, g.Kp);
+    // steps 0 and 1 published
+    wait_step((nk < STG ? nk : STG) - 2);
+    __builtin_amdgcn_s_barrier();
+    iv4 fa[FM], fb0[FN], fb1[FN];
+    read_frags(fa, fb0, lds + wm * CK + roff, lds + BM * CK + wn * CK + roff);
+    int64_t kt = 0;
+#define CRT_STEP(CB, NB)                                                                           \
+    {                                                                                              \
+        const int8_t* nst = lds + (int)((kt + 1) & (STG - 1)) * SSZ;                                \
+        step_mma<false>(acc, unused, fa, CB, NB, nst + wm * CK + roff, nst + BM * CK + wn * CK + roff); \
+        if (kt + 2 < nk) {                                                                         \
+            const int64_t later = nk - 3 - kt;                                                     \
+            wait_step(later < STG - 3 ? later : STG - 3);                                          \
+            __builtin_amdgcn_s_barrier();                                                          \
+            if (kt + STG < nk)                                                                     \
+                issue_step(lds, (int)(kt & (STG - 1)), wave, ga + (kt + STG) * CK, gb + (kt + STG) * CK, g.Kp); \
+        }                                                                                          \
+        ++kt;                                                                                      \
     }
+    while (kt < nk) {
+        CRT_STEP(fb0, fb1);
+        CRT_STEP(fb1, fb0);
+    }
+#undef CRT_STEP
     // acc[i][j][v] = V_t(m0 + wm + 16i + lr, n0 + wn + 16j + 4lg + v)
     const int p = c_moduli[tmod];
     const double inv_p = 1.0 / p;
