@@ -99,13 +99,17 @@ int launch_gemm_f64(char opA, char opB, const std::vector<Prob>& probs,
                     const std::vector<Seg>& segs, hipStream_t stream, int mode);
 
 // Host half of a launch: drops empty problems, assigns tile ranges and tells
-// whether every segment allows 128-bit loads.
+// whether every segment allows 128-bit loads (`vec`) and whether every tile
+// is interior for the direct-to-LDS kernel (`dma`: m and n multiples of 128,
+// every segment's K a multiple of 8, and `vec`).
 int prepare_gemm(const std::vector<Prob>& probs_in, const std::vector<Seg>& segs, std::vector<Prob>& probs,
-                 bool* vec, int tiles[2]);
+                 bool* vec, int tiles[2], bool* dma = nullptr);
 // Device half: tables already resident.  The persistent mode needs a device
 // int `counter`, which it zeroes with a kernel before the launch (graph safe).
+// With `dma` (every tile interior, see prepare_gemm) the 128x128 non-persistent
+// launches use the direct-to-LDS kernel.
 int launch_gemm_f64_dev(char opA, char opB, bool vec, int mode, int cfg, const Prob* dp, int np, const Seg* ds,
-                        const int tiles[2], int* counter, hipStream_t stream);
+                        const int tiles[2], int* counter, hipStream_t stream, bool dma = false);
 // Tile shape used by the one-shot entry points (DG_GEMM_TILE env override).
 int default_tile_cfg();
 

@@ -9,6 +9,9 @@
 // would have brought in; a SUMMA rank is one problem whose segments are the
 // owners' panels.
 //
+// Two kernels share the problem/segment tables and the tile order.  Batches
+// whose tiles are all interior run gemm_f64_dma_kernel (direct-to-LDS loads,
+// described with it below); the rest run gemm_f64_kernel:
 // Tiling (256 threads = 4 waves of 64 lanes):
 //   block tile 128 x 128, k-step 16, LDS double buffered (2 blocks per CU);
 //   each wave owns a 64 x 64 sub-tile = 4 x 4 MFMA tiles of 16 x 16.
@@ -345,6 +348,235 @@ __global__ __launch_bounds__(C::NTH, C::OCC) void gemm_f64_kernel(const Prob* __
     }
 }
 
+// ---------------------------------------------------------------------------
+// Direct-to-LDS form of the same GEMM for batches whose tiles are all
+// interior (m, n multiples of 128, every segment's K a multiple of 8 and an
+// even number of k-steps per problem, 16-byte aligned operands with even
+// leading dimensions).  128x128 tiles, 4 waves of
+// 64x64, two workgroups per CU, k-steps of 8 in a 4-stage LDS ring filled by
+// global_load_lds_dwordx4 (no register staging).  Each operand is staged in
+// the layout its memory order gives:
+//  * contiguous along the tile's rows (A for opA = N, B for opB = T): one load
+//    of a wave moves one k-row of 128 rows (1 KB); LDS [k][row] with a
+//    1088-byte row stride, so the 64-bit fragment reads of four k-rows that
+//    one MFMA needs fall in different bank halves;
+//  * contiguous along k (A for opA = T, B for opB = N): one load moves 16
+//    rows x 8 k (64 B each); LDS [row][k] with 16-byte chunk c of row r at
+//    position c ^ swz(r), so the 128-bit reads of 8 or 16 rows are
+//    conflict-free.
+// Lane l of an MFMA supplies k = 2*(l/16) + s for sub-step s of a k-step:
+// A and B use the same k assignment.  The barrier runs one step ahead: at the
+// end of step k each wave waits for its own loads of step k+2, one barrier
+// publishes them and frees the stage of step k, which receives step k+4; the
+// fragments of step k+1 are read while the MFMAs of step k run.
+namespace dmak {
+constexpr int DBM = 128, DBN = 128, DBK = 8, DSTG = 4;
+constexpr int RS = 1088;                   // k-row stride of a row-contiguous operand (bytes)
+constexpr int RPART = DBK * RS;             // 8704 bytes
+constexpr int KPART = 128 * DBK * 8;        // 8192 bytes
+template <bool KC> constexpr int part() { return KC ? KPART : RPART; }
+template <bool AK, bool BKC> constexpr int stage() { return part<AK>() + part<BKC>(); }
+constexpr size_t lds_bytes(bool ak, bool bkc) { return (size_t)DSTG * ((ak ? KPART : RPART) + (bkc ? KPART : RPART)); }
+
+__host__ __device__ constexpr int swz(int r) { return (((r >> 2) & 1) << 1) | (((r >> 1) ^ (r >> 3)) & 1); }
+
+template <int N>
+__device__ __forceinline__ void wait_vmcnt() {
+    static_assert(N >= 0 && N < 64, "vmcnt field");
+    // gfx9 s_waitcnt: vmcnt in bits 3:0 and 15:14; expcnt and lgkmcnt left at their maxima
+    __builtin_amdgcn_s_waitcnt((N & 0xF) | ((N >> 4) << 14) | (0x7 << 4) | (0xF << 8));
+}
+
+typedef __attribute__((address_space(3))) void lds_void_t;
+typedef __attribute__((address_space(1))) const void gvoid_t;
+
+// One k-step of one operand into `dst` (its part of a stage).  `base` is the
+// segment's operand pointer, `r0` the tile's first row, `kk` the step's k
+// within the segment, `ld` the leading dimension.
+template <bool KC>
+__device__ __forceinline__ void issue_operand(char* dst, const double* base, int64_t ld, int64_t r0, int64_t kk,
+                                              int wave, int lane) {
+    if constexpr (KC) {
+        // rows 32w .. 32w+31 in two loads of 16 rows; lane -> row lane/4, position lane%4
+        const int row = lane >> 2;
+        const int chunk = (lane & 3) ^ swz(row);
+#pragma unroll
+        for (int r = 0; r < 2; ++r) {
+            const double* src = base + (r0 + 32 * wave + 16 * r + row) * ld + kk + 2 * chunk;
+            __builtin_amdgcn_global_load_lds((gvoid_t*)src, (lds_void_t*)(dst + (32 * wave + 16 * r) * 64), 16, 0, 0);
+        }
+    } else {
+        // k-rows 2w, 2w+1 of 128 rows; lane -> rows 2*lane, 2*lane+1
+#pragma unroll
+        for (int r = 0; r < 2; ++r) {
+            const int kr = 2 * wave + r;
+            const double* src = base + r0 + 2 * lane + (kk + kr) * ld;
+            __builtin_amdgcn_global_load_lds((gvoid_t*)src, (lds_void_t*)(dst + kr * RS), 16, 0, 0);
+        }
+    }
+}
+
+// Fragments of one operand for one k-step: f[i][s] = X(row w0 + 16i + lr, k = 2lg + s).
+template <bool KC>
+__device__ __forceinline__ void read_operand(double (&f)[4][2], const char* part, int w0, int lr, int lg) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        if constexpr (KC) {
+            const dv2 v = *reinterpret_cast<const dv2*>(part + (w0 + 16 * i + lr) * 64 + 16 * (lg ^ swz(lr)));
+            f[i][0] = v.x;
+            f[i][1] = v.y;
+        } else {
+#pragma unroll
+            for (int s = 0; s < 2; ++s)
+                f[i][s] = *reinterpret_cast<const double*>(part + (2 * lg + s) * RS + (w0 + 16 * i + lr) * 8);
+        }
+    }
+}
+}  // namespace dmak
+
+template <bool AK, bool BKC>
+__device__ __forceinline__ void dma_tile(const Prob* __restrict__ probs, int nprob, const Seg* __restrict__ segs,
+                                         int tile, char* lds) {
+    using dmak::DBM; using dmak::DBN; using dmak::DBK; using dmak::DSTG;
+    using dmak::issue_operand; using dmak::read_operand; using dmak::wait_vmcnt;
+    constexpr int SSZ = dmak::stage<AK, BKC>(), APART = dmak::part<AK>();
+    int lo = 0, hi = nprob - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (probs[mid].tile0[0] <= tile) lo = mid; else hi = mid - 1;
+    }
+    const Prob P = probs[lo];
+    const int local = tile - P.tile0[0];
+    const int per_group = GROUP_M * P.tiles_n[0];
+    const int group = local / per_group;
+    const int first_m = group * GROUP_M;
+    const int gsize = min(P.tiles_m[0] - first_m, GROUP_M);
+    const int in_group = local - group * per_group;
+    const int64_t m0 = (int64_t)(first_m + in_group % gsize) * DBM;
+    const int64_t n0 = (int64_t)(in_group / gsize) * DBN;
+
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
+    const int lr = lane & 15, lg = lane >> 4;
+
+    // issue cursor over the problem's segments, one k-step (8) at a time
+    int iseg = P.seg0;
+    int64_t ik = 0;
+    Seg is = segs[iseg];
+    int ist = 0;
+    auto issue_next = [&]() {
+        char* st = lds + ist * SSZ;
+        issue_operand<AK>(st, is.a, is.lda, m0, ik, wave, lane);
+        issue_operand<BKC>(st + APART, is.b, is.ldb, n0, ik, wave, lane);
+        ist = (ist + 1) & (DSTG - 1);
+        ik += DBK;
+        if (ik >= is.klen) {
+            ik = 0;
+            if (++iseg < P.seg0 + P.nseg) is = segs[iseg];
+        }
+    };
+    int64_t nk = 0;  // k-steps over all segments: even (checked on the host), >= 2
+    for (int sg = P.seg0; sg < P.seg0 + P.nseg; ++sg) nk += segs[sg].klen / DBK;
+
+    dv4 acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = dv4{0.0, 0.0, 0.0, 0.0};
+
+    for (int64_t s = 0; s < DSTG && s < nk; ++s) issue_next();
+    // steps 0 and 1 published (4 loads per wave and step)
+    if (nk >= 4) wait_vmcnt<8>(); else if (nk == 3) wait_vmcnt<4>(); else wait_vmcnt<0>();
+    __builtin_amdgcn_s_barrier();
+    double fa0[4][2], fb0[4][2], fa1[4][2], fb1[4][2];
+    read_operand<AK>(fa0, lds, wm, lr, lg);
+    read_operand<BKC>(fb0, lds + APART, wn, lr, lg);
+    int64_t kt = 0;
+    // one step on (CA, CB) while the next step's fragments are read into (NA, NB)
+#define DMA_STEP(CA, CB, NA, NB)                                                                    \
+    {                                                                                               \
+        const char* nst = lds + (int)((kt + 1) & (DSTG - 1)) * SSZ;  /* harmless past the last step */ \
+        read_operand<AK>(NA, nst, wm, lr, lg);                                                      \
+        read_operand<BKC>(NB, nst + APART, wn, lr, lg);                                             \
+        _Pragma("unroll") for (int q = 0; q < 2; ++q)                                               \
+        _Pragma("unroll") for (int i = 0; i < 4; ++i)                                               \
+        _Pragma("unroll") for (int j = 0; j < 4; ++j)                                               \
+            acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(CB[j][q], CA[i][q], acc[i][j], 0, 0, 0); \
+        __builtin_amdgcn_sched_group_barrier(0x008, 2, 0);                                          \
+        _Pragma("unroll") for (int g_ = 0; g_ < 12; ++g_) {                                         \
+            __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);                                      \
+            __builtin_amdgcn_sched_group_barrier(0x008, 2, 0);                                      \
+        }                                                                                           \
+        __builtin_amdgcn_sched_group_barrier(0x008, 6, 0);                                          \
+        if (kt + 2 < nk) {                                                                          \
+            if (kt + 3 < nk) wait_vmcnt<4>(); else wait_vmcnt<0>();                                 \
+            __builtin_amdgcn_s_barrier();                                                           \
+            if (kt + DSTG < nk) issue_next();                                                        \
+        }                                                                                           \
+        ++kt;                                                                                       \
+    }
+    while (kt < nk) {
+        DMA_STEP(fa0, fb0, fa1, fb1);
+        DMA_STEP(fa1, fb1, fa0, fb0);
+    }
+#undef DMA_STEP
+
+    // acc[i][j][v] = sum for C(m0 + wm + 16i + lr, n0 + wn + 16j + lg + 4v); the tile is interior
+    const double alpha = P.alpha, beta = P.beta;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int64_t m = m0 + wm + 16 * i + lr;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int64_t n = n0 + wn + 16 * j + lg + 4 * v;
+                __attribute__((address_space(1))) double* cp =
+                    (__attribute__((address_space(1))) double*)(P.c + m + n * P.ldc);
+                double val = alpha * acc[i][j][v];
+                if (beta != 0.0) val = fma(beta, *cp, val);
+                *cp = val;
+            }
+        }
+    }
+}
+
+// MODE as gemm_f64_kernel: 0 tile = blockIdx.x, 1 XCD-contiguous ranges,
+// 2 persistent (tiles taken from `counter`, zeroed before the launch).
+template <bool AK, bool BKC, int MODE>
+__global__ __launch_bounds__(256, 2) void gemm_f64_dma_kernel(const Prob* __restrict__ probs, int nprob,
+                                                               const Seg* __restrict__ segs, int total_tiles,
+                                                               int* __restrict__ counter) {
+    extern __shared__ __attribute__((aligned(1024))) char lds[];
+    if (MODE == 0) {
+        dma_tile<AK, BKC>(probs, nprob, segs, blockIdx.x, lds);
+    } else if (MODE == 1) {
+        dma_tile<AK, BKC>(probs, nprob, segs, xcd_remap(blockIdx.x, total_tiles), lds);
+    } else {
+        __shared__ int next;
+        while (true) {
+            __syncthreads();  // the previous tile's LDS reads and `next` reads are complete
+            if (threadIdx.x == 0) next = atomicAdd(counter, 1);
+            __syncthreads();
+            const int tile = next;
+            if (tile >= total_tiles) break;
+            dma_tile<AK, BKC>(probs, nprob, segs, tile, lds);
+        }
+    }
+}
+
+template <bool AK, bool BKC>
+void launch_dma(int mode, const Prob* dp, int np, const Seg* ds, int tiles, int grid, int* counter, hipStream_t st) {
+    const size_t lb = dmak::lds_bytes(AK, BKC);
+    if (mode == DG_GEMM_MODE_ROUND_ROBIN)
+        hipLaunchKernelGGL((gemm_f64_dma_kernel<AK, BKC, 0>), dim3(tiles), dim3(256), lb, st, dp, np, ds, tiles, counter);
+    else if (mode == DG_GEMM_MODE_XCD)
+        hipLaunchKernelGGL((gemm_f64_dma_kernel<AK, BKC, 1>), dim3(tiles), dim3(256), lb, st, dp, np, ds, tiles, counter);
+    else
+        hipLaunchKernelGGL((gemm_f64_dma_kernel<AK, BKC, 2>), dim3(grid), dim3(256), lb, st, dp, np, ds, tiles, counter);
+}
+
 __global__ void zero_counter_kernel(int* counter) { *counter = 0; }
 
 __global__ void scale_kernel(int64_t m, int64_t n, double beta, double* __restrict__ C, int64_t ldc) {
@@ -401,15 +633,16 @@ int launch_scale(int64_t m, int64_t n, double beta, double* C, int64_t ldc, hipS
 }
 
 int prepare_gemm(const std::vector<Prob>& probs_in, const std::vector<Seg>& segs, std::vector<Prob>& probs,
-                 bool* vec, int tiles_out[2]) {
+                 bool* vec, int tiles_out[2], bool* dma) {
     probs.clear();
     probs.reserve(probs_in.size());
     int64_t tiles[2] = {0, 0};
     const int bm[2] = {Cfg128::BM, Cfg256::BM}, bn[2] = {Cfg128::BN, Cfg256::BN};
-    bool v = true;
+    bool v = true, d = true;
     for (const Prob& p0 : probs_in) {
         if (p0.m <= 0 || p0.n <= 0 || p0.nseg <= 0) continue;
         Prob p = p0;
+        if (p.m % dmak::DBM || p.n % dmak::DBN) d = false;
         for (int c = 0; c < 2; ++c) {
             p.tiles_m[c] = (int32_t)((p.m + bm[c] - 1) / bm[c]);
             p.tiles_n[c] = (int32_t)((p.n + bn[c] - 1) / bn[c]);
@@ -418,14 +651,19 @@ int prepare_gemm(const std::vector<Prob>& probs_in, const std::vector<Seg>& segs
         }
         if (tiles[0] > INT32_MAX) return fail(DG_ERR_INVALID, "gemm: %lld output tiles exceed one launch", (long long)tiles[0]);
         probs.push_back(p);
+        int64_t steps = 0;
         for (int s = p.seg0; s < p.seg0 + p.nseg; ++s) {
             const Seg& g = segs[s];
+            steps += g.klen / dmak::DBK;
             if ((reinterpret_cast<uintptr_t>(g.a) & 15) || (reinterpret_cast<uintptr_t>(g.b) & 15) || (g.lda & 1) ||
                 (g.ldb & 1))
                 v = false;
+            if (g.klen % dmak::DBK) d = false;
         }
+        if (steps % 2) d = false;  // the direct-to-LDS loop runs k-steps in pairs
     }
     *vec = v;
+    if (dma) *dma = v && d && !probs.empty();
     tiles_out[0] = (int)tiles[0];
     tiles_out[1] = (int)tiles[1];
     return DG_OK;
@@ -440,9 +678,30 @@ int persistent_grid(int cfg) {
 }
 
 int launch_gemm_f64_dev(char opA, char opB, bool vec, int mode, int cfg, const Prob* dp, int np, const Seg* ds,
-                        const int tiles[2], int* counter, hipStream_t stream) {
+                        const int tiles[2], int* counter, hipStream_t stream, bool dma) {
     const int t = tiles[cfg == DG_GEMM_TILE_256x128 ? 1 : 0];
     if (t <= 0) return DG_OK;
+    // Batches whose tiles are all interior run the direct-to-LDS kernel in every
+    // scheduling mode and tile setting, so every variant sums in the same order.
+    // DG_GEMM_NO_DMA=1 keeps them on the register-staged kernel (A/B comparisons).
+    static const bool no_dma = getenv("DG_GEMM_NO_DMA") != nullptr;
+    if (dma && !no_dma) {
+        const int t0 = tiles[0];
+        int grid = 0;
+        if (mode == DG_GEMM_MODE_PERSISTENT) {
+            if (!counter) return fail(DG_ERR_INVALID, "gemm: persistent launch without a work counter");
+            grid = persistent_grid(DG_GEMM_TILE_128x128);
+            if (grid > t0) grid = t0;
+            hipLaunchKernelGGL(zero_counter_kernel, dim3(1), dim3(1), 0, stream, counter);
+        }
+        const bool ak = opA == 'T', bkc = opB != 'T';
+        if (ak && bkc) launch_dma<true, true>(mode, dp, np, ds, t0, grid, counter, stream);
+        else if (ak) launch_dma<true, false>(mode, dp, np, ds, t0, grid, counter, stream);
+        else if (bkc) launch_dma<false, true>(mode, dp, np, ds, t0, grid, counter, stream);
+        else launch_dma<false, false>(mode, dp, np, ds, t0, grid, counter, stream);
+        DG_LAUNCH_CHECK();
+        return DG_OK;
+    }
     int grid = 0;
     if (mode == DG_GEMM_MODE_PERSISTENT) {
         if (!counter) return fail(DG_ERR_INVALID, "gemm: persistent launch without a work counter");
@@ -471,9 +730,9 @@ int default_tile_cfg() {
 int launch_gemm_f64(char opA, char opB, const std::vector<Prob>& probs_in, const std::vector<Seg>& segs,
                     hipStream_t stream, int mode) {
     std::vector<Prob> probs;
-    bool vec = true;
+    bool vec = true, dma = false;
     int tiles[2] = {0, 0};
-    int rc = prepare_gemm(probs_in, segs, probs, &vec, tiles);
+    int rc = prepare_gemm(probs_in, segs, probs, &vec, tiles, &dma);
     if (rc) return rc;
     if (probs.empty()) return DG_OK;
     const Prob* dp = static_cast<const Prob*>(cached_table(probs.data(), probs.size() * sizeof(Prob), stream));
@@ -482,7 +741,7 @@ int launch_gemm_f64(char opA, char opB, const std::vector<Prob>& probs_in, const
     if (!ds) return DG_ERR_ALLOC;
     if (mode == DG_GEMM_MODE_PERSISTENT) mode = DG_GEMM_MODE_XCD;  // needs a counter: plans only
     return launch_gemm_f64_dev(opA, opB, vec, mode, default_tile_cfg(), dp, (int)probs.size(), ds, tiles, nullptr,
-                               stream);
+                               stream, dma);
 }
 
 }  // namespace dg

@@ -14,7 +14,7 @@
 struct dg_grouped_plan {
     struct Group {
         char opA, opB;
-        bool vec;
+        bool vec, dma = false;
         int tiles[2], np;
         dg::Prob* dprobs = nullptr;
         dg::Seg* dsegs = nullptr;
@@ -154,7 +154,8 @@ int launch_ozaki_problem(const dg_grouped_plan* p, int i, hipStream_t st) {
 
 int plan_launch(dg_grouped_plan* p, int variant, int cfg, hipStream_t st, bool with_ozaki = true) {
     for (auto& g : p->groups) {
-        int rc = launch_gemm_f64_dev(g.opA, g.opB, g.vec, variant, cfg, g.dprobs, g.np, g.dsegs, g.tiles, p->counter, st);
+        int rc = launch_gemm_f64_dev(g.opA, g.opB, g.vec, variant, cfg, g.dprobs, g.np, g.dsegs, g.tiles, p->counter, st,
+                                     g.dma);
         if (rc) return rc;
     }
     if (with_ozaki)
@@ -175,15 +176,16 @@ int build_groups(dg_grouped_plan* p, const dg_gemm_desc* descs, int count, const
         std::vector<Prob> probs0, probs;
         std::vector<Seg> segs;
         build_group(sel.data(), kv.second, probs0, segs);
-        bool vec = true;
+        bool vec = true, dma = false;
         int tiles[2] = {0, 0};
-        int rc = prepare_gemm(probs0, segs, probs, &vec, tiles);
+        int rc = prepare_gemm(probs0, segs, probs, &vec, tiles, &dma);
         if (rc) return rc;
         if (probs.empty()) continue;
         dg_grouped_plan::Group g;
         g.opA = kv.first.first;
         g.opB = kv.first.second;
         g.vec = vec;
+        g.dma = dma;
         g.tiles[0] = tiles[0];
         g.tiles[1] = tiles[1];
         g.np = (int)probs.size();

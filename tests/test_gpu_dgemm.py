@@ -77,6 +77,16 @@ def test_beta_zero_nan_c(dev):
     assert oracle.rel_frobenius(got, ref) <= TOL
 
 
+@pytest.mark.parametrize("opA,opB", [("N", "N"), ("N", "T"), ("T", "N"), ("T", "T")])
+def test_all_interior_tiles_exact_integer(dev, opA, opB):
+    # m, n multiples of 128 and an even number of 8-deep k-steps: the
+    # direct-to-LDS kernel, every operand layout, exact
+    got, ref = run(dev, opA, opB, 384, 256, 208, 1.0, 0.0, integer=True, seed=4)
+    assert np.array_equal(got, ref)
+    got, ref = run(dev, opA, opB, 128, 128, 16, -2.0, 0.5, integer=True, seed=5)
+    assert np.array_equal(got, ref)
+
+
 @pytest.mark.parametrize("pad", [(1, 0, 0), (0, 1, 0), (1, 1, 5)])
 def test_odd_leading_dims(dev, pad):
     # odd lda (257) or ldb (131) forces the 64-bit load path; ldc padding must stay untouched
