@@ -390,30 +390,37 @@ __device__ __forceinline__ void wait_vmcnt() {
 typedef __attribute__((address_space(3))) void lds_void_t;
 typedef __attribute__((address_space(1))) const void gvoid_t;
 
-// One k-step of one operand into `dst` (its part of a stage).  `base` is the
-// segment's operand pointer, `r0` the tile's first row, `kk` the step's k
-// within the segment, `ld` the leading dimension.
+// Per-lane source of one operand's loads within a segment: `p` points at
+// this lane's first element of the current k-step, `second` is the offset of
+// the wave's second load and `step` the advance per k-step (all in doubles).
+struct OperandCursor {
+    const double* p;
+    int64_t second, step;
+};
+
+// Cursor at k = 0 of a segment: `base` is the segment's operand pointer,
+// `r0` the tile's first row, `ld` the leading dimension.
 template <bool KC>
-__device__ __forceinline__ void issue_operand(char* dst, const double* base, int64_t ld, int64_t r0, int64_t kk,
-                                              int wave, int lane) {
+__device__ __forceinline__ OperandCursor operand_cursor(const double* base, int64_t ld, int64_t r0, int wave, int lane) {
     if constexpr (KC) {
         // rows 32w .. 32w+31 in two loads of 16 rows; lane -> row lane/4, position lane%4
         const int row = lane >> 2;
         const int chunk = (lane & 3) ^ swz(row);
-#pragma unroll
-        for (int r = 0; r < 2; ++r) {
-            const double* src = base + (r0 + 32 * wave + 16 * r + row) * ld + kk + 2 * chunk;
-            __builtin_amdgcn_global_load_lds((gvoid_t*)src, (lds_void_t*)(dst + (32 * wave + 16 * r) * 64), 16, 0, 0);
-        }
+        return OperandCursor{base + (r0 + 32 * wave + row) * ld + 2 * chunk, 16 * ld, DBK};
     } else {
         // k-rows 2w, 2w+1 of 128 rows; lane -> rows 2*lane, 2*lane+1
-#pragma unroll
-        for (int r = 0; r < 2; ++r) {
-            const int kr = 2 * wave + r;
-            const double* src = base + r0 + 2 * lane + (kk + kr) * ld;
-            __builtin_amdgcn_global_load_lds((gvoid_t*)src, (lds_void_t*)(dst + kr * RS), 16, 0, 0);
-        }
+        return OperandCursor{base + r0 + 2 * lane + (int64_t)(2 * wave) * ld, ld, DBK * ld};
     }
+}
+
+// One k-step of one operand into `dst` (its part of a stage), then advance.
+template <bool KC>
+__device__ __forceinline__ void issue_operand(char* dst, OperandCursor& c, int wave) {
+    char* d0 = KC ? dst + (32 * wave) * 64 : dst + (2 * wave) * RS;
+    char* d1 = KC ? d0 + 16 * 64 : d0 + RS;
+    __builtin_amdgcn_global_load_lds((gvoid_t*)c.p, (lds_void_t*)d0, 16, 0, 0);
+    __builtin_amdgcn_global_load_lds((gvoid_t*)(c.p + c.second), (lds_void_t*)d1, 16, 0, 0);
+    c.p += c.step;
 }
 
 // Fragments of one operand for one k-step: f[i][s] = X(row w0 + 16i + lr, k = 2lg + s).
@@ -438,7 +445,7 @@ template <bool AK, bool BKC>
 __device__ __forceinline__ void dma_tile(const Prob* __restrict__ probs, int nprob, const Seg* __restrict__ segs,
                                          int tile, char* lds) {
     using dmak::DBM; using dmak::DBN; using dmak::DBK; using dmak::DSTG;
-    using dmak::issue_operand; using dmak::read_operand; using dmak::wait_vmcnt;
+    using dmak::read_operand; using dmak::wait_vmcnt;
     constexpr int SSZ = dmak::stage<AK, BKC>(), APART = dmak::part<AK>();
     int lo = 0, hi = nprob - 1;
     while (lo < hi) {
@@ -462,18 +469,21 @@ __device__ __forceinline__ void dma_tile(const Prob* __restrict__ probs, int npr
 
     // issue cursor over the problem's segments, one k-step (8) at a time
     int iseg = P.seg0;
-    int64_t ik = 0;
     Seg is = segs[iseg];
+    int64_t left = is.klen / DBK;  // k-steps left in the current segment
+    dmak::OperandCursor ca = dmak::operand_cursor<AK>(is.a, is.lda, m0, wave, lane);
+    dmak::OperandCursor cb = dmak::operand_cursor<BKC>(is.b, is.ldb, n0, wave, lane);
     int ist = 0;
     auto issue_next = [&]() {
         char* st = lds + ist * SSZ;
-        issue_operand<AK>(st, is.a, is.lda, m0, ik, wave, lane);
-        issue_operand<BKC>(st + APART, is.b, is.ldb, n0, ik, wave, lane);
+        dmak::issue_operand<AK>(st, ca, wave);
+        dmak::issue_operand<BKC>(st + APART, cb, wave);
         ist = (ist + 1) & (DSTG - 1);
-        ik += DBK;
-        if (ik >= is.klen) {
-            ik = 0;
-            if (++iseg < P.seg0 + P.nseg) is = segs[iseg];
+        if (--left == 0 && ++iseg < P.seg0 + P.nseg) {
+            is = segs[iseg];
+            left = is.klen / DBK;
+            ca = dmak::operand_cursor<AK>(is.a, is.lda, m0, wave, lane);
+            cb = dmak::operand_cursor<BKC>(is.b, is.ldb, n0, wave, lane);
         }
     };
     int64_t nk = 0;  // k-steps over all segments: even (checked on the host), >= 2

@@ -369,7 +369,7 @@ __constant__ int c_moduli[kMaxModuli] = {256, 255, 253, 251, 247, 241, 239, 233,
 struct CrtTables {
     int32_t n;                          // moduli in use
     int32_t cinv[kMaxModuli];           // M_t^-1 mod p_t
-    double mt[kMaxModuli][4];           // M_t, 32-bit limbs as doubles, least significant first
+    double mt[kMaxModuli][3];           // M_t in 40-bit limbs (as doubles), least significant first
     uint32_t P[4];                      // P = prod p_t
     uint32_t Ph[4];                     // floor(P / 2)
     double Pd;                          // P rounded to double
@@ -545,8 +545,9 @@ __device__ __forceinline__ uint32_t mod_u16(uint32_t x, uint32_t p, uint32_t mag
     return x - q * p;
 }
 
-// The balanced CRT value of one element from its limb sums
-// L[l] = sum_t x_t * limb_l(M_t), correctly rounded to double.
+// The balanced CRT value of one element, correctly rounded to double, from
+// Le[l] = sum over 32-bit limb positions l of S = sum_t x_t M_t (each Le[l]
+// below 2^40, so the carries fit 64 bits).
 __host__ __device__ inline double crt_value(const uint64_t (&Le)[4], const CrtTables& tb) {
     // carry-propagate into 32-bit limbs s0..s4
     uint32_t s0, s1, s2, s3, s4;
@@ -632,9 +633,10 @@ __host__ __device__ inline double crt_value(const uint64_t (&Le)[4], const CrtTa
 }
 
 // Four consecutive rows per thread (one 32-bit load of residues per modulus).
-// The limb sums L_l = sum_t x_t * limb_l(M_t) are exact FP64 FMAs (x_t < 256,
-// limbs < 2^32, at most 16 terms: below 2^44); crt_value reduces S = sum_l
-// L_l 2^(32 l) < 16 P modulo P with a floating quotient estimate and one
+// The limb sums L_l = sum_t x_t * limb_l(M_t) over 40-bit limbs of M_t are
+// exact FP64 FMAs (x_t < 256, limbs < 2^40, at most 16 terms: below 2^52);
+// they are regrouped into 32-bit limb positions and crt_value reduces
+// S = sum_t x_t M_t < 16 P modulo P with a floating quotient estimate and one
 // correction, balances it and rounds it to double once.
 // C = alpha * ldexp(V, e_i + f_j - 2q) + beta*C, four rows per 32-byte access
 // when C allows it.
@@ -648,11 +650,11 @@ __global__ void crt_reconstruct_kernel(const uint8_t* __restrict__ res, CrtTable
         const int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;  // one thread per 4 rows
         if (x >= total) return;
         const int64_t j = x / qm, i0 = (x - j * qm) * 4;
-        double L[4][4];
+        double L[4][3];
 #pragma unroll
         for (int e = 0; e < 4; ++e)
 #pragma unroll
-            for (int l = 0; l < 4; ++l) L[e][l] = 0.0;
+            for (int l = 0; l < 3; ++l) L[e][l] = 0.0;
         const uint8_t* rp = res + j * Mp + i0;
 #pragma unroll
         for (int t = 0; t < kMaxModuli; ++t) {
@@ -664,7 +666,7 @@ __global__ void crt_reconstruct_kernel(const uint8_t* __restrict__ res, CrtTable
                     const uint32_t xr = mod_u16(__umul24((w >> (8 * e)) & 255u, c), crt_modulus(t), crt_magic(t));
                     const double xd = (double)xr;
 #pragma unroll
-                    for (int l = 0; l < 4; ++l) L[e][l] = fma(xd, tb.mt[t][l], L[e][l]);
+                    for (int l = 0; l < 3; ++l) L[e][l] = fma(xd, tb.mt[t][l], L[e][l]);
                 }
             }
         }
@@ -672,7 +674,10 @@ __global__ void crt_reconstruct_kernel(const uint8_t* __restrict__ res, CrtTable
         double val[4];
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
-            const uint64_t Le[4] = {(uint64_t)L[e][0], (uint64_t)L[e][1], (uint64_t)L[e][2], (uint64_t)L[e][3]};
+            // S = a0 + a1 2^40 + a2 2^80 regrouped at 32-bit positions
+            const uint64_t a0 = (uint64_t)L[e][0], a1 = (uint64_t)L[e][1], a2 = (uint64_t)L[e][2];
+            const uint64_t Le[4] = {a0 & 0xffffffffull, (a0 >> 32) + ((a1 << 8) & 0xffffffffull),
+                                    (a1 >> 24) + ((a2 << 16) & 0xffffffffull), a2 >> 16};
             const int64_t i = i0 + e < m ? i0 + e : m - 1;  // rows past m are computed but never stored
             val[e] = __dmul_rn(alpha, ldexp(crt_value(Le, tb), ea[i] + jf - 2 * q));
         }
@@ -701,29 +706,36 @@ __global__ void crt_reconstruct_kernel(const uint8_t* __restrict__ res, CrtTable
 // Element (row, k) of op(X) is at x[row*rs + k*ks].
 __global__ void oz_partial_max_kernel(const double* __restrict__ x, int64_t rows, int64_t K, int64_t rs, int64_t ks,
                                       int64_t kchunk, double* __restrict__ part) {
-    // blockIdx.y: k-chunk; threads stride over rows, or over k when rows are
-    // contiguous in k (ks == 1), with a block reduction.
+    // blockIdx.y: k-chunk.  Rows strided in memory (ks != 1): a thread per
+    // row, consecutive lanes on consecutive rows, four loads in flight.
+    // Rows contiguous in k (ks == 1): a wave per row, lanes along k, then a
+    // wave reduction.
     const int64_t k0 = blockIdx.y * kchunk;
     const int64_t k1 = min(K, k0 + kchunk);
     if (ks != 1) {
         for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < rows; r += (int64_t)gridDim.x * blockDim.x) {
-            double mx = 0.0;
-            for (int64_t k = k0; k < k1; ++k) mx = fmax(mx, fabs(x[r * rs + k * ks]));
-            part[blockIdx.y * rows + r] = mx;
+            const double* xr = x + r * rs;
+            double m0 = 0.0, m1 = 0.0, m2 = 0.0, m3 = 0.0;
+            int64_t k = k0;
+            for (; k + 4 <= k1; k += 4) {
+                m0 = fmax(m0, fabs(xr[k * ks]));
+                m1 = fmax(m1, fabs(xr[(k + 1) * ks]));
+                m2 = fmax(m2, fabs(xr[(k + 2) * ks]));
+                m3 = fmax(m3, fabs(xr[(k + 3) * ks]));
+            }
+            for (; k < k1; ++k) m0 = fmax(m0, fabs(xr[k * ks]));
+            part[blockIdx.y * rows + r] = fmax(fmax(m0, m1), fmax(m2, m3));
         }
     } else {
-        __shared__ double red[NT];
-        for (int64_t r = blockIdx.x; r < rows; r += gridDim.x) {
+        const int lane = threadIdx.x & 63;
+        const int64_t waves = (int64_t)gridDim.x * (blockDim.x >> 6);
+        for (int64_t r = blockIdx.x * (int64_t)(blockDim.x >> 6) + (threadIdx.x >> 6); r < rows; r += waves) {
+            const double* xr = x + r * rs;
             double mx = 0.0;
-            for (int64_t k = k0 + threadIdx.x; k < k1; k += blockDim.x) mx = fmax(mx, fabs(x[r * rs + k]));
-            red[threadIdx.x] = mx;
-            __syncthreads();
-            for (int w = NT / 2; w > 0; w >>= 1) {
-                if ((int)threadIdx.x < w) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
-                __syncthreads();
-            }
-            if (threadIdx.x == 0) part[blockIdx.y * rows + r] = red[0];
-            __syncthreads();
+            for (int64_t k = k0 + lane; k < k1; k += 64) mx = fmax(mx, fabs(xr[k]));
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off));
+            if (lane == 0) part[blockIdx.y * rows + r] = mx;
         }
     }
 }
@@ -798,7 +810,7 @@ __global__ void oz_slice_kernel(const double* __restrict__ x, int64_t rows, int6
     }
 }
 
-constexpr int64_t kMaxChunk = 1024;  // k-chunk of the max reduction
+constexpr int64_t kMaxChunk = 256;  // k-chunk of the max reduction
 
 int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
 
@@ -827,9 +839,7 @@ int crt_setup(int64_t k, int nmod, CrtTables* tb, int* q_out) {
     for (int t = 0; t < nmod; ++t) {
         const int p = crt_modulus(t);
         const unsigned __int128 M = P / (unsigned)p;
-        uint32_t ml[4];
-        limbs(M, ml);
-        for (int l = 0; l < 4; ++l) tb->mt[t][l] = (double)ml[l];
+        for (int l = 0; l < 3; ++l) tb->mt[t][l] = (double)(uint64_t)((M >> (40 * l)) & (((unsigned __int128)1 << 40) - 1));
         // inverse of M mod p by extended Euclid (M and p are coprime)
         int64_t a = (int64_t)(M % (unsigned)p), b = p, x0 = 1, x1 = 0;
         while (b) {
@@ -1039,13 +1049,19 @@ int dg_ozaki_dgemm(char opA, char opB, int64_t m, int64_t n, int64_t k, double a
 
     // Row exponents of op(A), column exponents of op(B).
     {
-        unsigned gx = a_ks == 1 ? (unsigned)(m < 65535 ? m : 65535) : (unsigned)((m + NT - 1) / NT < 65535 ? (m + NT - 1) / NT : 65535);
+        // ks == 1: a wave per row (NT/64 rows per block); otherwise a thread per row
+        auto pm_grid = [](int64_t rows, int64_t ks) {
+            const int64_t per = ks == 1 ? NT / 64 : NT;
+            const int64_t g = (rows + per - 1) / per;
+            return (unsigned)(g < 65535 ? g : 65535);
+        };
+        unsigned gx = pm_grid(m, a_ks);
         hipLaunchKernelGGL(oz_partial_max_kernel, dim3(gx, gy), dim3(NT), 0, st, A, m, k, a_rs, a_ks, p.kchunk_max, part);
         DG_LAUNCH_CHECK();
         hipLaunchKernelGGL(oz_exponent_kernel, dim3((unsigned)((p.Mp + NT - 1) / NT)), dim3(NT), 0, st, part, m, p.Mp,
                            nchunk, ea);
         DG_LAUNCH_CHECK();
-        gx = b_ks == 1 ? (unsigned)(n < 65535 ? n : 65535) : (unsigned)((n + NT - 1) / NT < 65535 ? (n + NT - 1) / NT : 65535);
+        gx = pm_grid(n, b_ks);
         hipLaunchKernelGGL(oz_partial_max_kernel, dim3(gx, gy), dim3(NT), 0, st, B, n, k, b_rs, b_ks, p.kchunk_max, part);
         DG_LAUNCH_CHECK();
         hipLaunchKernelGGL(oz_exponent_kernel, dim3((unsigned)((p.Np + NT - 1) / NT)), dim3(NT), 0, st, part, n, p.Np,
