@@ -362,8 +362,8 @@ __global__ __launch_bounds__(C::NTH, C::OCC) void gemm_f64_kernel(const Prob* __
 //    one MFMA needs fall in different bank halves;
 //  * contiguous along k (A for opA = T, B for opB = N): one load moves 16
 //    rows x 8 k (64 B each); LDS [row][k] with 16-byte chunk c of row r at
-//    position c ^ swz(r), so the 128-bit reads of 8 or 16 rows are
-//    conflict-free.
+//    position c ^ ((r >> 1) & 3), the placement at which a wave's 128-bit
+//    reads of 16 rows x 4 chunks run at the full LDS rate.
 // Lane l of an MFMA supplies k = 2*(l/16) + s for sub-step s of a k-step:
 // A and B use the same k assignment.  The barrier runs one step ahead: at the
 // end of step k each wave waits for its own loads of step k+2, one barrier
@@ -378,7 +378,7 @@ template <bool KC> constexpr int part() { return KC ? KPART : RPART; }
 template <bool AK, bool BKC> constexpr int stage() { return part<AK>() + part<BKC>(); }
 constexpr size_t lds_bytes(bool ak, bool bkc) { return (size_t)DSTG * ((ak ? KPART : RPART) + (bkc ? KPART : RPART)); }
 
-__host__ __device__ constexpr int swz(int r) { return (((r >> 2) & 1) << 1) | (((r >> 1) ^ (r >> 3)) & 1); }
+__host__ __device__ constexpr int swz(int r) { return (r >> 1) & 3; }
 
 template <int N>
 __device__ __forceinline__ void wait_vmcnt() {

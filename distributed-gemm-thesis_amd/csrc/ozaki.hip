@@ -49,7 +49,7 @@ constexpr int CK = 64;       // k-step in bytes
 // OCC workgroups per CU.
 constexpr int BM = 256, BN = 256;
 constexpr int WM = 128, WN = 64;
-constexpr int STG = 4;
+constexpr int STG = 5;
 constexpr int OCC = 1;
 constexpr int NW = (BM / WM) * (BN / WN);  // waves per workgroup
 constexpr int CNT = NW * 64;               // threads per workgroup
@@ -59,7 +59,7 @@ constexpr int LPS = LA + LB;
 constexpr int SSZ = (BM + BN) * CK;        // bytes per stage (A rows, then B rows)
 constexpr size_t kGemmLds = (size_t)STG * SSZ;
 static_assert(LA * 16 * NW == BM && LB * 16 * NW == BN, "loader split");
-static_assert(PAD % BM == 0 && PAD % BN == 0 && STG == 4, "tile configuration (the ring index is k & 3)");
+static_assert(PAD % BM == 0 && PAD % BN == 0 && STG >= 4 && STG <= 5, "tile configuration (waits reach 3 steps ahead)");
 constexpr int NT = 256;
 constexpr int GROUP_M = 8;
 
@@ -113,8 +113,11 @@ __device__ __forceinline__ void mma(iv4& acc_i, fv4& acc_f, const iv4& x, const 
 // (FM x FN MFMA tiles of 16x16), 64-byte k-steps in an STG-stage LDS ring
 // filled by global_load_lds_dwordx4, which writes LDS without a register
 // round trip.  One such load of a wave covers 16 rows x 64 bytes; within a
-// row, 16-byte chunk c sits at position c ^ swz(row) so that the 128-bit
-// fragment reads of 8 or 16 consecutive rows hit distinct banks.
+// row, 16-byte chunk c sits at position c ^ ((row >> 1) & 3).  With that
+// placement a wave's 128-bit fragment reads (lane l: row l % 16, chunk l / 16)
+// run at the LDS's full 256 bytes per clock on gfx950; without it, or with
+// the chunk flipped for rows 8..15, they run at half that rate (measured with
+// a read-only microbenchmark of the five placements).
 // In the scheme II kernel the barrier runs one step ahead: at the end of
 // step k each wave waits for its own loads of step k+2, and one barrier
 // publishes step k+2 and frees the stage of step k, which then receives step
@@ -124,7 +127,7 @@ __device__ __forceinline__ void mma(iv4& acc_i, fv4& acc_f, const iv4& x, const 
 // matrix cores.  The scheme I kernel, whose product boundaries and flushes
 // break that rhythm, publishes step k at its start and reads every fragment
 // before the MFMAs.
-__host__ __device__ constexpr int swz(int r) { return (((r >> 2) & 1) << 1) | (((r >> 1) ^ (r >> 3)) & 1); }
+__host__ __device__ constexpr int swz(int r) { return (r >> 1) & 3; }
 
 template <int N>
 __device__ __forceinline__ void wait_vmcnt() {
@@ -322,6 +325,7 @@ __global__ __launch_bounds__(CNT, OCC) void oz_gemm_kernel(OzArgs g) {
     const int64_t ncol = n0 + wn + 4 * lg;
     const bool full = (m0 + BM <= g.m) && (n0 + BN <= g.n);
     int64_t kt = 0;
+    int cst = 0;  // stage of step kt
     for (int cseg = 0; cseg < g.nseg; ++cseg) {
         const OzSeg cs = g.segs[cseg];
         const int64_t kend = kt + (int64_t)cs.klen * esz / CK;
@@ -330,8 +334,9 @@ __global__ __launch_bounds__(CNT, OCC) void oz_gemm_kernel(OzArgs g) {
             wait_step(rem < STG - 2 ? rem : STG - 2);
             __builtin_amdgcn_s_barrier();
             if (kt + STG - 1 < nsteps) issue_next();
-            const int8_t* st = lds + (int)(kt & (STG - 1)) * SSZ;
+            const int8_t* st = lds + cst * SSZ;
             stage_mma<BF16>(acci, accf, st + wm * CK + roff, st + BM * CK + wn * CK + roff);
+            cst = cst + 1 == STG ? 0 : cst + 1;
         }
         if (cs.flush) {
             oz_flush<BF16>(acci, accf, g, ldexp(1.0, -g.bits * cs.level), mrow, ncol, cs.flush, !full);
@@ -460,9 +465,63 @@ struct CrtArgs {
     int32_t tiles_m, tiles_n, nmod;
 };
 
+// The scheme II product pipeline moves 128-byte k-steps: one
+// global_load_lds_dwordx4 of a wave covers 8 rows x 128 bytes, so every L2
+// request is a whole 128-byte line (64-byte half lines cost the same request
+// slot and halve the operand stream).  A step's A rows and B rows each fill
+// a 32 KB slot; five slots (160 KB) form a ring in which A_0 B_0 A_1 B_1 ...
+// take consecutive slots.  Within a row, 16-byte chunk c sits at position
+// c ^ (row % 8): the fragment reads (lane l: row l % 16, chunk 4h + l / 16 of
+// half-step h) then run at the full LDS rate.  A step is two MFMA half-steps
+// of 64 bytes; the barrier sits between them: it publishes step s+1 (A_{s+1}
+// loaded two steps earlier, B_{s+1} one step earlier) and releases the slots
+// of step s, whose fragments are all in registers by then, to B_{s+2} and
+// A_{s+3}.  The second half-step's MFMAs run while the fragments of the
+// first half of step s+1 are read, and the first half-step's while those of
+// the second half are read.
+constexpr int CK2 = 128;                  // bytes per scheme II k-step
+constexpr int CSLOT = BM * CK2;           // one operand's rows of one step
+constexpr int NSLOT = 5;
+constexpr size_t kCrtLds = (size_t)NSLOT * CSLOT;
+static_assert(BM == BN && BM / NW == 32, "scheme II loader: 32 rows per wave and operand");
+
+__device__ __forceinline__ int next_slot(int x) { return x + 1 == NSLOT ? 0 : x + 1; }
+
+// This wave's 32 rows of one operand for k-step s into `slot`; g points at
+// the lane's row and chunk, rows `ld` bytes apart.
+__device__ __forceinline__ void crt_issue(int8_t* lds, int slot, int wave, const int8_t* g, int64_t s, int64_t ld) {
+    int8_t* d = lds + slot * CSLOT + wave * 4096;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+        __builtin_amdgcn_global_load_lds((gvoid_t*)(g + s * CK2 + (int64_t)(8 * i) * ld), (lds_void_t*)(d + 1024 * i), 16, 0,
+                                         0);
+}
+
+// One MFMA half-step on fa / cb, reading the fragments of the following
+// half-step meanwhile: fa[i] right after its last use, B into nb.
+__device__ __forceinline__ void crt_half(iv4 (&acc)[FM][FN], iv4 (&fa)[FM], const iv4 (&cb)[FN], iv4 (&nb)[FN],
+                                         const int8_t* nas, const int8_t* nbs) {
+#pragma unroll
+    for (int i = 0; i < FM; ++i) {
+#pragma unroll
+        for (int j = 0; j < FN; ++j) acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(cb[j], fa[i], acc[i][j], 0, 0, 0);
+        if (i == 0) {
+#pragma unroll
+            for (int j = 0; j < FN; ++j) nb[j] = *reinterpret_cast<const iv4*>(nbs + 2048 * j);
+        }
+        fa[i] = *reinterpret_cast<const iv4*>(nas + 2048 * i);
+    }
+    __builtin_amdgcn_sched_group_barrier(0x008, FN, 0);
+    __builtin_amdgcn_sched_group_barrier(0x100, FN + 1, 0);
+#pragma unroll
+    for (int g = 1; g < FM; ++g) {
+        __builtin_amdgcn_sched_group_barrier(0x008, FN, 0);
+        __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+    }
+}
+
 // One INT8 GEMM per modulus in one launch: workgroup b computes tile
-// b % tiles of modulus b / tiles through the product pipeline and stores
-// (A'_t B'_t) mod p_t as bytes.
+// b % tiles of modulus b / tiles and stores (A'_t B'_t) mod p_t as bytes.
 __global__ __launch_bounds__(CNT, OCC) void crt_gemm_kernel(CrtArgs g) {
     extern __shared__ __attribute__((aligned(1024))) int8_t lds[];
     const int tiles = g.tiles_m * g.tiles_n;
@@ -477,45 +536,63 @@ __global__ __launch_bounds__(CNT, OCC) void crt_gemm_kernel(CrtArgs g) {
     const int lr = lane & 15;
     const int lg = lane >> 4;
 
-    const int lrow = lane >> 2;
-    const int lchunk = (lane & 3) ^ swz(lrow);
-    const int8_t* ga = g.a + tmod * g.a_slice + (m0 + 16 * LA * wave + lrow) * g.Kp + 16 * lchunk;
-    const int8_t* gb = g.b + tmod * g.b_slice + (n0 + 16 * LB * wave + lrow) * g.Kp + 16 * lchunk;
-    const int roff = lr * CK + 16 * (lg ^ swz(lr));
+    // loader: lane -> row lane / 8 of an 8-row group, position lane % 8
+    // holding global chunk (lane % 8) ^ (row % 8)
+    const int lrow = lane >> 3;
+    const int lchunk = (lane & 7) ^ lrow;
+    const int8_t* ga = g.a + tmod * g.a_slice + (m0 + 32 * wave + lrow) * g.Kp + 16 * lchunk;
+    const int8_t* gb = g.b + tmod * g.b_slice + (n0 + 32 * wave + lrow) * g.Kp + 16 * lchunk;
+    const int roff0 = lr * CK2 + 16 * (lg ^ (lr & 7));
+    const int roff1 = lr * CK2 + 16 * ((4 + lg) ^ (lr & 7));
 
     iv4 acc[FM][FN];
-    fv4 unused[FM][FN];
 #pragma unroll
     for (int i = 0; i < FM; ++i)
 #pragma unroll
         for (int j = 0; j < FN; ++j) acc[i][j] = iv4{0, 0, 0, 0};
 
-    const int64_t nk = g.Kp / CK;  // even, >= 2
-    for (int64_t s = 0; s < STG && s < nk; ++s) issue_step(lds, (int)s, wave, ga + s * CK, gb + s * CK, g.Kp);
-    // steps 0 and 1 published
-    wait_step((nk < STG ? nk : STG) - 2);
+    const int64_t nk = g.Kp / CK2;  // >= 1
+    // prologue: A_0 B_0 A_1 B_1 A_2 into slots 0..4
+    int isl = 0;
+    crt_issue(lds, isl, wave, ga, 0, g.Kp); isl = next_slot(isl);
+    crt_issue(lds, isl, wave, gb, 0, g.Kp); isl = next_slot(isl);
+    if (nk > 1) {
+        crt_issue(lds, isl, wave, ga, 1, g.Kp); isl = next_slot(isl);
+        crt_issue(lds, isl, wave, gb, 1, g.Kp); isl = next_slot(isl);
+    }
+    if (nk > 2) { crt_issue(lds, isl, wave, ga, 2, g.Kp); isl = next_slot(isl); }
+    // step 0 landed: at most A_1 B_1 A_2 (4 loads each) pending
+    if (nk > 2) wait_vmcnt<12>(); else if (nk > 1) wait_vmcnt<8>(); else wait_vmcnt<0>();
     __builtin_amdgcn_s_barrier();
-    iv4 fa[FM], fb0[FN], fb1[FN];
-    read_frags(fa, fb0, lds + wm * CK + roff, lds + BM * CK + wn * CK + roff);
-    int64_t kt = 0;
-#define CRT_STEP(CB, NB)                                                                           \
-    {                                                                                              \
-        const int8_t* nst = lds + (int)((kt + 1) & (STG - 1)) * SSZ;                                \
-        step_mma<false>(acc, unused, fa, CB, NB, nst + wm * CK + roff, nst + BM * CK + wn * CK + roff); \
-        if (kt + 2 < nk) {                                                                         \
-            const int64_t later = nk - 3 - kt;                                                     \
-            wait_step(later < STG - 3 ? later : STG - 3);                                          \
-            __builtin_amdgcn_s_barrier();                                                          \
-            if (kt + STG < nk)                                                                     \
-                issue_step(lds, (int)(kt & (STG - 1)), wave, ga + (kt + STG) * CK, gb + (kt + STG) * CK, g.Kp); \
-        }                                                                                          \
-        ++kt;                                                                                      \
+    int csl = 0;  // slot of A_s; B_s in the next one
+    iv4 fa[FM], cb[FN], nb[FN];
+    {
+        const int8_t* as = lds + wm * CK2 + roff0;
+        const int8_t* bs = lds + CSLOT + wn * CK2 + roff0;
+#pragma unroll
+        for (int j = 0; j < FN; ++j) cb[j] = *reinterpret_cast<const iv4*>(bs + 2048 * j);
+#pragma unroll
+        for (int i = 0; i < FM; ++i) fa[i] = *reinterpret_cast<const iv4*>(as + 2048 * i);
     }
-    while (kt < nk) {
-        CRT_STEP(fb0, fb1);
-        CRT_STEP(fb1, fb0);
+    for (int64_t s = 0; s < nk; ++s) {
+        const int bsl = next_slot(csl), asl1 = next_slot(bsl), bsl1 = next_slot(asl1);
+        // first half of step s; fragments of its second half read meanwhile
+        crt_half(acc, fa, cb, nb, lds + csl * CSLOT + wm * CK2 + roff1, lds + bsl * CSLOT + wn * CK2 + roff1);
+        if (s + 1 < nk) {
+            // every fragment of step s is in registers once this wave's LDS
+            // reads are done; A_{s+1} and B_{s+1} landed with at most A_{s+2}
+            // pending
+            __builtin_amdgcn_s_waitcnt((0xF) | (0x3 << 14) | (0x7 << 4));  // lgkmcnt(0)
+            if (s + 2 < nk) wait_vmcnt<4>(); else wait_vmcnt<0>();
+            __builtin_amdgcn_s_barrier();
+            if (s + 2 < nk) { crt_issue(lds, isl, wave, gb, s + 2, g.Kp); isl = next_slot(isl); }
+            if (s + 3 < nk) { crt_issue(lds, isl, wave, ga, s + 3, g.Kp); isl = next_slot(isl); }
+        }
+        // second half of step s; fragments of the first half of step s+1
+        // read meanwhile (in-bounds and unused after the last step)
+        crt_half(acc, fa, nb, cb, lds + asl1 * CSLOT + wm * CK2 + roff0, lds + bsl1 * CSLOT + wn * CK2 + roff0);
+        csl = asl1;
     }
-#undef CRT_STEP
     // acc[i][j][v] = V_t(m0 + wm + 16i + lr, n0 + wn + 16j + 4lg + v)
     const int p = c_moduli[tmod];
     const double inv_p = 1.0 / p;
@@ -1082,7 +1159,7 @@ int dg_ozaki_dgemm(char opA, char opB, int64_t m, int64_t n, int64_t k, double a
         cg.tiles_m = (int32_t)(p.Mp / BM); cg.tiles_n = (int32_t)(p.Np / BN); cg.nmod = slices;
         const int64_t grid = (int64_t)cg.tiles_m * cg.tiles_n * slices;
         if (grid > INT32_MAX) return fail(DG_ERR_INVALID, "dg_ozaki_dgemm: too many tiles");
-        hipLaunchKernelGGL(crt_gemm_kernel, dim3((unsigned)grid), dim3(CNT), kGemmLds, st, cg);
+        hipLaunchKernelGGL(crt_gemm_kernel, dim3((unsigned)grid), dim3(CNT), kCrtLds, st, cg);
         DG_LAUNCH_CHECK();
         const int64_t blocks = (((m + 3) / 4) * n + 255) / 256;
         if (blocks > INT32_MAX) return fail(DG_ERR_INVALID, "dg_ozaki_dgemm: too many output elements");
