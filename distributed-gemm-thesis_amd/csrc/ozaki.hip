@@ -459,10 +459,11 @@ __global__ __launch_bounds__(256) void crt_residue_kernel(const double* __restri
 struct CrtArgs {
     const int8_t* a;     // residues of A', [N][Mp][Kp]
     const int8_t* b;     // residues of B', [N][Np][Kp]
-    uint8_t* res;        // residues of V, [N][Np][Mp] (column-major per modulus)
+    uint8_t* res;        // CRT digits x_t of V, [N][Np][Mp] (column-major per modulus)
     int64_t Mp, Np, Kp;
     int64_t a_slice, b_slice;
     int32_t tiles_m, tiles_n, nmod;
+    int32_t cinv[kMaxModuli];  // (P / p_t)^-1 mod p_t
 };
 
 // The scheme II product pipeline moves 128-byte k-steps: one
@@ -521,7 +522,8 @@ __device__ __forceinline__ void crt_half(iv4 (&acc)[FM][FN], iv4 (&fa)[FM], cons
 }
 
 // One INT8 GEMM per modulus in one launch: workgroup b computes tile
-// b % tiles of modulus b / tiles and stores (A'_t B'_t) mod p_t as bytes.
+// b % tiles of modulus b / tiles and stores the CRT digit
+// x_t = ((A'_t B'_t) mod p_t) * ((P / p_t)^-1 mod p_t) mod p_t as a byte.
 __global__ __launch_bounds__(CNT, OCC) void crt_gemm_kernel(CrtArgs g) {
     extern __shared__ __attribute__((aligned(1024))) int8_t lds[];
     const int tiles = g.tiles_m * g.tiles_n;
@@ -595,6 +597,7 @@ __global__ __launch_bounds__(CNT, OCC) void crt_gemm_kernel(CrtArgs g) {
     }
     // acc[i][j][v] = V_t(m0 + wm + 16i + lr, n0 + wn + 16j + 4lg + v)
     const int p = c_moduli[tmod];
+    const int cinv = g.cinv[tmod];
     const double inv_p = 1.0 / p;
     uint8_t* out = g.res + (int64_t)tmod * g.Mp * g.Np;
 #pragma unroll
@@ -605,21 +608,10 @@ __global__ __launch_bounds__(CNT, OCC) void crt_gemm_kernel(CrtArgs g) {
 #pragma unroll
             for (int i = 0; i < FM; ++i) {
                 const int64_t m = m0 + wm + 16 * i + lr;
-                out[n * g.Mp + m] = (uint8_t)mod_exact((double)acc[i][j][v], p, inv_p);
+                const int r = mod_exact((double)acc[i][j][v], p, inv_p);
+                out[n * g.Mp + m] = (uint8_t)mod_exact((double)(r * cinv), p, inv_p);
             }
         }
-}
-
-// x mod p for x < 2^16 with q = floor(x / p) = (2x * ceil(2^31 / p)) >> 32,
-// exact for p <= 256 (the product's excess over 2x * 2^31 / p stays below
-// 2^-15 < 1/p); both factors fit 24 bits, so it compiles to 24-bit multiplies.
-__host__ __device__ constexpr uint32_t crt_magic(int t) {
-    return (uint32_t)((((uint64_t)1 << 31) + (uint64_t)crt_modulus(t) - 1) / (uint64_t)crt_modulus(t));
-}
-__device__ __forceinline__ uint32_t mod_u16(uint32_t x, uint32_t p, uint32_t magic) {
-    x &= 0xFFFFu;
-    const uint32_t q = (uint32_t)(((uint64_t)(x << 1) * magic) >> 32);
-    return x - q * p;
 }
 
 // The balanced CRT value of one element, correctly rounded to double, from
@@ -709,7 +701,7 @@ __host__ __device__ inline double crt_value(const uint64_t (&Le)[4], const CrtTa
     return v;
 }
 
-// Four consecutive rows per thread (one 32-bit load of residues per modulus).
+// Four consecutive rows per thread (one 32-bit load of CRT digits per modulus).
 // The limb sums L_l = sum_t x_t * limb_l(M_t) over 40-bit limbs of M_t are
 // exact FP64 FMAs (x_t < 256, limbs < 2^40, at most 16 terms: below 2^52);
 // they are regrouped into 32-bit limb positions and crt_value reduces
@@ -737,11 +729,9 @@ __global__ void crt_reconstruct_kernel(const uint8_t* __restrict__ res, CrtTable
         for (int t = 0; t < kMaxModuli; ++t) {
             if (t < tb.n) {
                 const uint32_t w = *reinterpret_cast<const uint32_t*>(rp + (int64_t)t * Mp * Np);
-                const uint32_t c = (uint32_t)tb.cinv[t];
 #pragma unroll
                 for (int e = 0; e < 4; ++e) {
-                    const uint32_t xr = mod_u16(__umul24((w >> (8 * e)) & 255u, c), crt_modulus(t), crt_magic(t));
-                    const double xd = (double)xr;
+                    const double xd = (double)((w >> (8 * e)) & 255u);
 #pragma unroll
                     for (int l = 0; l < 3; ++l) L[e][l] = fma(xd, tb.mt[t][l], L[e][l]);
                 }
@@ -1157,6 +1147,7 @@ int dg_ozaki_dgemm(char opA, char opB, int64_t m, int64_t n, int64_t k, double a
         cg.a = sa; cg.b = sb; cg.res = reinterpret_cast<uint8_t*>(ws + p.off_w);
         cg.Mp = p.Mp; cg.Np = p.Np; cg.Kp = p.Kp; cg.a_slice = p.a_slice; cg.b_slice = p.b_slice;
         cg.tiles_m = (int32_t)(p.Mp / BM); cg.tiles_n = (int32_t)(p.Np / BN); cg.nmod = slices;
+        for (int t = 0; t < kMaxModuli; ++t) cg.cinv[t] = tables.cinv[t];
         const int64_t grid = (int64_t)cg.tiles_m * cg.tiles_n * slices;
         if (grid > INT32_MAX) return fail(DG_ERR_INVALID, "dg_ozaki_dgemm: too many tiles");
         hipLaunchKernelGGL(crt_gemm_kernel, dim3((unsigned)grid), dim3(CNT), kCrtLds, st, cg);
