@@ -75,7 +75,10 @@ struct Prob {
     int32_t seg0, nseg;    // segment range in the segment table
     int32_t tile0[2];      // first global tile id of this problem, per tile shape
     int32_t tiles_m[2], tiles_n[2];
-    int32_t pad_;
+    // direct-to-LDS kernel: bit 0 (bit 1) set when the row-contiguous A (B)
+    // starts 8 bytes past a 16-byte boundary; its loads then start one row
+    // earlier and the tile grid runs over rows + 1
+    int32_t shift;
 };
 
 // Tile scheduling of the FP64 GEMM launch.
@@ -98,18 +101,39 @@ enum {
 int launch_gemm_f64(char opA, char opB, const std::vector<Prob>& probs,
                     const std::vector<Seg>& segs, hipStream_t stream, int mode);
 
-// Host half of a launch: drops empty problems, assigns tile ranges and tells
-// whether every segment allows 128-bit loads (`vec`) and whether every tile
-// is interior for the direct-to-LDS kernel (`dma`: m and n multiples of 128,
-// every segment's K a multiple of 8, and `vec`).
-int prepare_gemm(const std::vector<Prob>& probs_in, const std::vector<Seg>& segs, std::vector<Prob>& probs,
-                 bool* vec, int tiles[2], bool* dma = nullptr);
-// Device half: tables already resident.  The persistent mode needs a device
-// int `counter`, which it zeroes with a kernel before the launch (graph safe).
-// With `dma` (every tile interior, see prepare_gemm) the 128x128 non-persistent
-// launches use the direct-to-LDS kernel.
-int launch_gemm_f64_dev(char opA, char opB, bool vec, int mode, int cfg, const Prob* dp, int np, const Seg* ds,
-                        const int tiles[2], int* counter, hipStream_t stream, bool dma = false);
+// A batch split by kernel.  Each problem is routed on its own properties,
+// so it sums in the same order alone or inside any batch: the direct-to-LDS
+// kernel takes it when every segment has K > 0 and even leading dimensions,
+// its k-contiguous operand (A with opA = T, B with opB = N) has 16-byte
+// aligned bases and even K, and its row-contiguous operand (A with opA = N,
+// B with opB = T) has 8-byte aligned bases with the same offset within 16
+// bytes in every segment; the register-staged kernel takes the others.
+// DG_GEMM_NO_DMA routes everything to the register-staged kernel.
+struct GemmBatch {
+    std::vector<Prob> dma, gen;   // tile ranges assigned per list
+    int tiles_dma = 0;            // 128x128 tiles of `dma`
+    int tiles_gen[2] = {0, 0};    // tiles of `gen` per tile shape
+    bool vec = true;              // every `gen` segment allows 128-bit loads
+    bool dma_edge = false;        // some `dma` problem has edge tiles, a row shift or partial k-steps
+};
+int prepare_gemm(char opA, char opB, const std::vector<Prob>& probs_in, const std::vector<Seg>& segs,
+                 GemmBatch& out);
+
+// The same batch with its tables resident on the device.
+struct GemmBatchDev {
+    const Prob* dma = nullptr;
+    int ndma = 0, tiles_dma = 0;
+    const Prob* gen = nullptr;
+    int ngen = 0, tiles_gen[2] = {0, 0};
+    bool vec = true, dma_edge = false;
+    const Seg* segs = nullptr;
+};
+// Device half.  The persistent mode needs a device int `counter`, which it
+// zeroes with a kernel before each launch (graph safe).  The direct-to-LDS
+// list runs that kernel in every mode and tile setting, the other list the
+// register-staged kernel of tile shape cfg.
+int launch_gemm_f64_dev(char opA, char opB, int mode, int cfg, const GemmBatchDev& b, int* counter,
+                        hipStream_t stream);
 // Tile shape used by the one-shot entry points (DG_GEMM_TILE env override).
 int default_tile_cfg();
 

@@ -390,6 +390,10 @@ __device__ __forceinline__ void wait_vmcnt() {
 typedef __attribute__((address_space(3))) void lds_void_t;
 typedef __attribute__((address_space(1))) const void gvoid_t;
 
+// Zeros for the loads of k past a segment's end and of the padding step
+// that makes a tile's step count even (never written).
+__device__ __align__(256) double g_dma_zero[128];
+
 // Per-lane source of one operand's loads within a segment: `p` points at
 // this lane's first element of the current k-step, `second` is the offset of
 // the wave's second load and `step` the advance per k-step (all in doubles).
@@ -399,27 +403,63 @@ struct OperandCursor {
 };
 
 // Cursor at k = 0 of a segment: `base` is the segment's operand pointer,
-// `r0` the tile's first row, `ld` the leading dimension.
-template <bool KC>
-__device__ __forceinline__ OperandCursor operand_cursor(const double* base, int64_t ld, int64_t r0, int wave, int lane) {
+// `r0` the tile's first row, `ld` the leading dimension, `rows` the
+// operand's row count.
+//  * k-contiguous operand: rows past the end are read from the last row
+//    (their products land in rows or columns of C that are never stored).
+//  * row-contiguous operand: a lane loads the 16-byte aligned row pair
+//    (2i, 2i+1) of a grid that starts `shift` (0 or 1) rows before `base`;
+//    rows past the end read the last pair.  Every such pair holds at least
+//    one element of the operand, and a 16-byte aligned access never crosses
+//    a page, so the load stays inside mapped memory; the row before the
+//    first (shift 1) and the row after the last (odd row count) only feed
+//    rows or columns of C that are never stored.
+// EDGE = false: the tile is interior and shift is 0 (no clamping).
+template <bool KC, bool EDGE>
+__device__ __forceinline__ OperandCursor operand_cursor(const double* base, int64_t ld, int64_t r0, int64_t rows,
+                                                        int shift, int wave, int lane) {
     if constexpr (KC) {
         // rows 32w .. 32w+31 in two loads of 16 rows; lane -> row lane/4, position lane%4
         const int row = lane >> 2;
         const int chunk = (lane & 3) ^ swz(row);
-        return OperandCursor{base + (r0 + 32 * wave + row) * ld + 2 * chunk, 16 * ld, DBK};
+        const int64_t ra = EDGE ? min(r0 + 32 * wave + row, rows - 1) : r0 + 32 * wave + row;
+        const int64_t rb = EDGE ? min(r0 + 32 * wave + row + 16, rows - 1) : ra + 16;
+        return OperandCursor{base + ra * ld + 2 * chunk, (rb - ra) * ld, DBK};
     } else {
-        // k-rows 2w, 2w+1 of 128 rows; lane -> rows 2*lane, 2*lane+1
-        return OperandCursor{base + r0 + 2 * lane + (int64_t)(2 * wave) * ld, ld, DBK * ld};
+        // k-rows 2w, 2w+1 of 128 rows; lane -> rows 2*lane, 2*lane+1 of the pair grid
+        if constexpr (!EDGE) return OperandCursor{base + r0 + 2 * lane + (int64_t)(2 * wave) * ld, ld, DBK * ld};
+        const int64_t last = (rows + shift - 1) & ~(int64_t)1;  // first row of the last pair
+        const int64_t r = min(r0 + 2 * lane, last);
+        return OperandCursor{base - shift + r + (int64_t)(2 * wave) * ld, ld, DBK * ld};
     }
 }
 
+// Cursor past the last segment: every load reads zeros.
+__device__ __forceinline__ OperandCursor zero_cursor(int lane) { return OperandCursor{g_dma_zero + 2 * lane, 0, 0}; }
+
 // One k-step of one operand into `dst` (its part of a stage), then advance.
+// With tail != 0 the step is a segment's last and only its first `tail`
+// (even for k-contiguous operands) k are real: the lanes past them load zeros.
 template <bool KC>
-__device__ __forceinline__ void issue_operand(char* dst, OperandCursor& c, int wave) {
+__device__ __forceinline__ void issue_operand(char* dst, OperandCursor& c, int wave, int lane, int tail) {
     char* d0 = KC ? dst + (32 * wave) * 64 : dst + (2 * wave) * RS;
     char* d1 = KC ? d0 + 16 * 64 : d0 + RS;
-    __builtin_amdgcn_global_load_lds((gvoid_t*)c.p, (lds_void_t*)d0, 16, 0, 0);
-    __builtin_amdgcn_global_load_lds((gvoid_t*)(c.p + c.second), (lds_void_t*)d1, 16, 0, 0);
+    const double* p0 = c.p;
+    const double* p1 = c.p + c.second;
+    if (tail) {
+        const double* z = g_dma_zero + 2 * lane;
+        if constexpr (KC) {
+            // this lane's k pair is 2 * chunk, 2 * chunk + 1 of the step
+            const int k2 = 2 * ((lane & 3) ^ swz(lane >> 2));
+            if (k2 >= tail) { p0 = z; p1 = z; }
+        } else {
+            // load 0 carries k-row 2w, load 1 k-row 2w + 1
+            if (2 * wave >= tail) p0 = z;
+            if (2 * wave + 1 >= tail) p1 = z;
+        }
+    }
+    __builtin_amdgcn_global_load_lds((gvoid_t*)p0, (lds_void_t*)d0, 16, 0, 0);
+    __builtin_amdgcn_global_load_lds((gvoid_t*)p1, (lds_void_t*)d1, 16, 0, 0);
     c.p += c.step;
 }
 
@@ -441,7 +481,10 @@ __device__ __forceinline__ void read_operand(double (&f)[4][2], const char* part
 }
 }  // namespace dmak
 
-template <bool AK, bool BKC>
+// EDGE = false: every tile interior, no row shift, every segment's K a
+// multiple of 8 and an even step count (the same arithmetic without the
+// edge paths).
+template <bool AK, bool BKC, bool EDGE>
 __device__ __forceinline__ void dma_tile(const Prob* __restrict__ probs, int nprob, const Seg* __restrict__ segs,
                                          int tile, char* lds) {
     using dmak::DBM; using dmak::DBN; using dmak::DBK; using dmak::DSTG;
@@ -467,27 +510,45 @@ __device__ __forceinline__ void dma_tile(const Prob* __restrict__ probs, int npr
     const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
     const int lr = lane & 15, lg = lane >> 4;
 
-    // issue cursor over the problem's segments, one k-step (8) at a time
+    // issue cursor over the problem's segments, one k-step (8) at a time; a
+    // segment's last step may be partial, and past the last segment the
+    // cursors read zeros (the padding step of an odd step count)
+    const int sa = EDGE ? P.shift & 1 : 0, sb = EDGE ? (P.shift >> 1) & 1 : 0;  // row shifts of a row-contiguous A / B
     int iseg = P.seg0;
     Seg is = segs[iseg];
-    int64_t left = is.klen / DBK;  // k-steps left in the current segment
-    dmak::OperandCursor ca = dmak::operand_cursor<AK>(is.a, is.lda, m0, wave, lane);
-    dmak::OperandCursor cb = dmak::operand_cursor<BKC>(is.b, is.ldb, n0, wave, lane);
+    int64_t left = (is.klen + DBK - 1) / DBK;  // k-steps left in the current segment
+    int tail = (int)(is.klen % DBK);
+    dmak::OperandCursor ca = dmak::operand_cursor<AK, EDGE>(is.a, is.lda, m0, P.m, sa, wave, lane);
+    dmak::OperandCursor cb = dmak::operand_cursor<BKC, EDGE>(is.b, is.ldb, n0, P.n, sb, wave, lane);
     int ist = 0;
     auto issue_next = [&]() {
         char* st = lds + ist * SSZ;
-        dmak::issue_operand<AK>(st, ca, wave);
-        dmak::issue_operand<BKC>(st + APART, cb, wave);
+        if (EDGE && left == 1 && tail) {
+            dmak::issue_operand<AK>(st, ca, wave, lane, tail);
+            dmak::issue_operand<BKC>(st + APART, cb, wave, lane, tail);
+        } else {
+            dmak::issue_operand<AK>(st, ca, wave, lane, 0);
+            dmak::issue_operand<BKC>(st + APART, cb, wave, lane, 0);
+        }
         ist = (ist + 1) & (DSTG - 1);
-        if (--left == 0 && ++iseg < P.seg0 + P.nseg) {
-            is = segs[iseg];
-            left = is.klen / DBK;
-            ca = dmak::operand_cursor<AK>(is.a, is.lda, m0, wave, lane);
-            cb = dmak::operand_cursor<BKC>(is.b, is.ldb, n0, wave, lane);
+        if (--left == 0) {
+            if (++iseg < P.seg0 + P.nseg) {
+                is = segs[iseg];
+                left = (is.klen + DBK - 1) / DBK;
+                tail = (int)(is.klen % DBK);
+                ca = dmak::operand_cursor<AK, EDGE>(is.a, is.lda, m0, P.m, sa, wave, lane);
+                cb = dmak::operand_cursor<BKC, EDGE>(is.b, is.ldb, n0, P.n, sb, wave, lane);
+            } else {
+                left = INT64_MAX;
+                tail = 0;
+                ca = dmak::zero_cursor(lane);
+                cb = dmak::zero_cursor(lane);
+            }
         }
     };
-    int64_t nk = 0;  // k-steps over all segments: even (checked on the host), >= 2
-    for (int sg = P.seg0; sg < P.seg0 + P.nseg; ++sg) nk += segs[sg].klen / DBK;
+    int64_t nk = 0;  // k-steps over all segments, rounded up to even (>= 2)
+    for (int sg = P.seg0; sg < P.seg0 + P.nseg; ++sg) nk += (segs[sg].klen + DBK - 1) / DBK;
+    nk += nk & 1;
 
     dv4 acc[4][4];
 #pragma unroll
@@ -532,16 +593,18 @@ __device__ __forceinline__ void dma_tile(const Prob* __restrict__ probs, int npr
     }
 #undef DMA_STEP
 
-    // acc[i][j][v] = sum for C(m0 + wm + 16i + lr, n0 + wn + 16j + lg + 4v); the tile is interior
+    // acc[i][j][v] = sum for C(m0 + wm + 16i + lr - sa, n0 + wn + 16j + lg + 4v - sb)
     const double alpha = P.alpha, beta = P.beta;
+    const bool interior = !EDGE || (sa == 0 && sb == 0 && m0 + DBM <= P.m && n0 + DBN <= P.n);
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-        const int64_t m = m0 + wm + 16 * i + lr;
+        const int64_t m = m0 + wm + 16 * i + lr - sa;
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const int64_t n = n0 + wn + 16 * j + lg + 4 * v;
+                const int64_t n = n0 + wn + 16 * j + lg + 4 * v - sb;
+                if (!interior && (m < 0 || m >= P.m || n < 0 || n >= P.n)) continue;
                 __attribute__((address_space(1))) double* cp =
                     (__attribute__((address_space(1))) double*)(P.c + m + n * P.ldc);
                 double val = alpha * acc[i][j][v];
@@ -554,15 +617,15 @@ __device__ __forceinline__ void dma_tile(const Prob* __restrict__ probs, int npr
 
 // MODE as gemm_f64_kernel: 0 tile = blockIdx.x, 1 XCD-contiguous ranges,
 // 2 persistent (tiles taken from `counter`, zeroed before the launch).
-template <bool AK, bool BKC, int MODE>
+template <bool AK, bool BKC, bool EDGE, int MODE>
 __global__ __launch_bounds__(256, 2) void gemm_f64_dma_kernel(const Prob* __restrict__ probs, int nprob,
                                                                const Seg* __restrict__ segs, int total_tiles,
                                                                int* __restrict__ counter) {
     extern __shared__ __attribute__((aligned(1024))) char lds[];
     if (MODE == 0) {
-        dma_tile<AK, BKC>(probs, nprob, segs, blockIdx.x, lds);
+        dma_tile<AK, BKC, EDGE>(probs, nprob, segs, blockIdx.x, lds);
     } else if (MODE == 1) {
-        dma_tile<AK, BKC>(probs, nprob, segs, xcd_remap(blockIdx.x, total_tiles), lds);
+        dma_tile<AK, BKC, EDGE>(probs, nprob, segs, xcd_remap(blockIdx.x, total_tiles), lds);
     } else {
         __shared__ int next;
         while (true) {
@@ -571,20 +634,26 @@ __global__ __launch_bounds__(256, 2) void gemm_f64_dma_kernel(const Prob* __rest
             __syncthreads();
             const int tile = next;
             if (tile >= total_tiles) break;
-            dma_tile<AK, BKC>(probs, nprob, segs, tile, lds);
+            dma_tile<AK, BKC, EDGE>(probs, nprob, segs, tile, lds);
         }
     }
 }
 
-template <bool AK, bool BKC>
-void launch_dma(int mode, const Prob* dp, int np, const Seg* ds, int tiles, int grid, int* counter, hipStream_t st) {
+template <bool AK, bool BKC, bool EDGE>
+void launch_dma_e(int mode, const Prob* dp, int np, const Seg* ds, int tiles, int grid, int* counter, hipStream_t st) {
     const size_t lb = dmak::lds_bytes(AK, BKC);
     if (mode == DG_GEMM_MODE_ROUND_ROBIN)
-        hipLaunchKernelGGL((gemm_f64_dma_kernel<AK, BKC, 0>), dim3(tiles), dim3(256), lb, st, dp, np, ds, tiles, counter);
+        hipLaunchKernelGGL((gemm_f64_dma_kernel<AK, BKC, EDGE, 0>), dim3(tiles), dim3(256), lb, st, dp, np, ds, tiles, counter);
     else if (mode == DG_GEMM_MODE_XCD)
-        hipLaunchKernelGGL((gemm_f64_dma_kernel<AK, BKC, 1>), dim3(tiles), dim3(256), lb, st, dp, np, ds, tiles, counter);
+        hipLaunchKernelGGL((gemm_f64_dma_kernel<AK, BKC, EDGE, 1>), dim3(tiles), dim3(256), lb, st, dp, np, ds, tiles, counter);
     else
-        hipLaunchKernelGGL((gemm_f64_dma_kernel<AK, BKC, 2>), dim3(grid), dim3(256), lb, st, dp, np, ds, tiles, counter);
+        hipLaunchKernelGGL((gemm_f64_dma_kernel<AK, BKC, EDGE, 2>), dim3(grid), dim3(256), lb, st, dp, np, ds, tiles, counter);
+}
+template <bool AK, bool BKC>
+void launch_dma(bool edge, int mode, const Prob* dp, int np, const Seg* ds, int tiles, int grid, int* counter,
+                hipStream_t st) {
+    if (edge) launch_dma_e<AK, BKC, true>(mode, dp, np, ds, tiles, grid, counter, st);
+    else launch_dma_e<AK, BKC, false>(mode, dp, np, ds, tiles, grid, counter, st);
 }
 
 __global__ void zero_counter_kernel(int* counter) { *counter = 0; }
@@ -642,40 +711,59 @@ int launch_scale(int64_t m, int64_t n, double beta, double* C, int64_t ldc, hipS
     return DG_OK;
 }
 
-int prepare_gemm(const std::vector<Prob>& probs_in, const std::vector<Seg>& segs, std::vector<Prob>& probs,
-                 bool* vec, int tiles_out[2], bool* dma) {
-    probs.clear();
-    probs.reserve(probs_in.size());
-    int64_t tiles[2] = {0, 0};
+int prepare_gemm(char opA, char opB, const std::vector<Prob>& probs_in, const std::vector<Seg>& segs,
+                 GemmBatch& out) {
+    out = GemmBatch{};
+    const bool ak = opA == 'T', bkc = opB != 'T';  // k-contiguous operands
+    static const bool no_dma = getenv("DG_GEMM_NO_DMA") != nullptr;
+    int64_t tdma = 0, tgen[2] = {0, 0};
     const int bm[2] = {Cfg128::BM, Cfg256::BM}, bn[2] = {Cfg128::BN, Cfg256::BN};
-    bool v = true, d = true;
     for (const Prob& p0 : probs_in) {
         if (p0.m <= 0 || p0.n <= 0 || p0.nseg <= 0) continue;
-        Prob p = p0;
-        if (p.m % dmak::DBM || p.n % dmak::DBN) d = false;
-        for (int c = 0; c < 2; ++c) {
-            p.tiles_m[c] = (int32_t)((p.m + bm[c] - 1) / bm[c]);
-            p.tiles_n[c] = (int32_t)((p.n + bn[c] - 1) / bn[c]);
-            p.tile0[c] = (int32_t)tiles[c];
-            tiles[c] += (int64_t)p.tiles_m[c] * p.tiles_n[c];
-        }
-        if (tiles[0] > INT32_MAX) return fail(DG_ERR_INVALID, "gemm: %lld output tiles exceed one launch", (long long)tiles[0]);
-        probs.push_back(p);
-        int64_t steps = 0;
-        for (int s = p.seg0; s < p.seg0 + p.nseg; ++s) {
+        bool v = true, d = !no_dma;
+        int off_a = -1, off_b = -1;  // byte offset within 16 of a row-contiguous A / B
+        for (int s = p0.seg0; s < p0.seg0 + p0.nseg; ++s) {
             const Seg& g = segs[s];
-            steps += g.klen / dmak::DBK;
-            if ((reinterpret_cast<uintptr_t>(g.a) & 15) || (reinterpret_cast<uintptr_t>(g.b) & 15) || (g.lda & 1) ||
-                (g.ldb & 1))
-                v = false;
-            if (g.klen % dmak::DBK) d = false;
+            const int oa = (int)(reinterpret_cast<uintptr_t>(g.a) & 15), ob = (int)(reinterpret_cast<uintptr_t>(g.b) & 15);
+            if (oa || ob || (g.lda & 1) || (g.ldb & 1)) v = false;
+            if ((g.lda & 1) || (g.ldb & 1) || g.klen <= 0) d = false;
+            if (ak) { if (oa || (g.klen & 1)) d = false; }   // k pairs, 16-byte aligned
+            else { if ((oa & 7) || (off_a >= 0 && oa != off_a)) d = false; off_a = oa; }
+            if (bkc) { if (ob || (g.klen & 1)) d = false; }
+            else { if ((ob & 7) || (off_b >= 0 && ob != off_b)) d = false; off_b = ob; }
         }
-        if (steps % 2) d = false;  // the direct-to-LDS loop runs k-steps in pairs
+        Prob p = p0;
+        if (!d) {
+            out.vec = out.vec && v;
+            for (int c = 0; c < 2; ++c) {
+                p.tiles_m[c] = (int32_t)((p.m + bm[c] - 1) / bm[c]);
+                p.tiles_n[c] = (int32_t)((p.n + bn[c] - 1) / bn[c]);
+                p.tile0[c] = (int32_t)tgen[c];
+                tgen[c] += (int64_t)p.tiles_m[c] * p.tiles_n[c];
+            }
+            out.gen.push_back(p);
+            continue;
+        }
+        const int sa = off_a == 8, sb = off_b == 8;
+        p.shift = sa | (sb << 1);
+        int64_t steps = 0;
+        bool whole = true;  // every segment a whole number of k-steps
+        for (int s = p.seg0; s < p.seg0 + p.nseg; ++s) {
+            steps += (segs[s].klen + dmak::DBK - 1) / dmak::DBK;
+            if (segs[s].klen % dmak::DBK) whole = false;
+        }
+        if (p.shift || p.m % dmak::DBM || p.n % dmak::DBN || !whole || (steps & 1)) out.dma_edge = true;
+        p.tiles_m[0] = p.tiles_m[1] = (int32_t)((p.m + sa + dmak::DBM - 1) / dmak::DBM);
+        p.tiles_n[0] = p.tiles_n[1] = (int32_t)((p.n + sb + dmak::DBN - 1) / dmak::DBN);
+        p.tile0[0] = p.tile0[1] = (int32_t)tdma;
+        tdma += (int64_t)p.tiles_m[0] * p.tiles_n[0];
+        out.dma.push_back(p);
     }
-    *vec = v;
-    if (dma) *dma = v && d && !probs.empty();
-    tiles_out[0] = (int)tiles[0];
-    tiles_out[1] = (int)tiles[1];
+    if (tdma > INT32_MAX || tgen[0] > INT32_MAX || tgen[1] > INT32_MAX)
+        return fail(DG_ERR_INVALID, "gemm: output tiles exceed one launch");
+    out.tiles_dma = (int)tdma;
+    out.tiles_gen[0] = (int)tgen[0];
+    out.tiles_gen[1] = (int)tgen[1];
     return DG_OK;
 }
 
@@ -687,41 +775,36 @@ int persistent_grid(int cfg) {
     return cus * (cfg == DG_GEMM_TILE_256x128 ? Cfg256::OCC : cfg == DG_GEMM_TILE_128x128W8 ? Cfg128w8::OCC : Cfg128::OCC);
 }
 
-int launch_gemm_f64_dev(char opA, char opB, bool vec, int mode, int cfg, const Prob* dp, int np, const Seg* ds,
-                        const int tiles[2], int* counter, hipStream_t stream, bool dma) {
-    const int t = tiles[cfg == DG_GEMM_TILE_256x128 ? 1 : 0];
-    if (t <= 0) return DG_OK;
-    // Batches whose tiles are all interior run the direct-to-LDS kernel in every
-    // scheduling mode and tile setting, so every variant sums in the same order.
-    // DG_GEMM_NO_DMA=1 keeps them on the register-staged kernel (A/B comparisons).
-    static const bool no_dma = getenv("DG_GEMM_NO_DMA") != nullptr;
-    if (dma && !no_dma) {
-        const int t0 = tiles[0];
+int launch_gemm_f64_dev(char opA, char opB, int mode, int cfg, const GemmBatchDev& b, int* counter,
+                        hipStream_t stream) {
+    if (mode == DG_GEMM_MODE_PERSISTENT && !counter)
+        return fail(DG_ERR_INVALID, "gemm: persistent launch without a work counter");
+    if (b.tiles_dma > 0) {
+        const int t0 = b.tiles_dma;
         int grid = 0;
         if (mode == DG_GEMM_MODE_PERSISTENT) {
-            if (!counter) return fail(DG_ERR_INVALID, "gemm: persistent launch without a work counter");
             grid = persistent_grid(DG_GEMM_TILE_128x128);
             if (grid > t0) grid = t0;
             hipLaunchKernelGGL(zero_counter_kernel, dim3(1), dim3(1), 0, stream, counter);
         }
         const bool ak = opA == 'T', bkc = opB != 'T';
-        if (ak && bkc) launch_dma<true, true>(mode, dp, np, ds, t0, grid, counter, stream);
-        else if (ak) launch_dma<true, false>(mode, dp, np, ds, t0, grid, counter, stream);
-        else if (bkc) launch_dma<false, true>(mode, dp, np, ds, t0, grid, counter, stream);
-        else launch_dma<false, false>(mode, dp, np, ds, t0, grid, counter, stream);
+        if (ak && bkc) launch_dma<true, true>(b.dma_edge, mode, b.dma, b.ndma, b.segs, t0, grid, counter, stream);
+        else if (ak) launch_dma<true, false>(b.dma_edge, mode, b.dma, b.ndma, b.segs, t0, grid, counter, stream);
+        else if (bkc) launch_dma<false, true>(b.dma_edge, mode, b.dma, b.ndma, b.segs, t0, grid, counter, stream);
+        else launch_dma<false, false>(b.dma_edge, mode, b.dma, b.ndma, b.segs, t0, grid, counter, stream);
         DG_LAUNCH_CHECK();
-        return DG_OK;
     }
+    const int t = b.tiles_gen[cfg == DG_GEMM_TILE_256x128 ? 1 : 0];
+    if (t <= 0) return DG_OK;
     int grid = 0;
     if (mode == DG_GEMM_MODE_PERSISTENT) {
-        if (!counter) return fail(DG_ERR_INVALID, "gemm: persistent launch without a work counter");
         grid = persistent_grid(cfg);
         if (grid > t) grid = t;
         hipLaunchKernelGGL(zero_counter_kernel, dim3(1), dim3(1), 0, stream, counter);
     }
-    if (cfg == DG_GEMM_TILE_256x128) launch_cfg<Cfg256>(opA, opB, vec, mode, dp, np, ds, t, grid, counter, stream);
-    else if (cfg == DG_GEMM_TILE_128x128W8) launch_cfg<Cfg128w8>(opA, opB, vec, mode, dp, np, ds, t, grid, counter, stream);
-    else launch_cfg<Cfg128>(opA, opB, vec, mode, dp, np, ds, t, grid, counter, stream);
+    if (cfg == DG_GEMM_TILE_256x128) launch_cfg<Cfg256>(opA, opB, b.vec, mode, b.gen, b.ngen, b.segs, t, grid, counter, stream);
+    else if (cfg == DG_GEMM_TILE_128x128W8) launch_cfg<Cfg128w8>(opA, opB, b.vec, mode, b.gen, b.ngen, b.segs, t, grid, counter, stream);
+    else launch_cfg<Cfg128>(opA, opB, b.vec, mode, b.gen, b.ngen, b.segs, t, grid, counter, stream);
     DG_LAUNCH_CHECK();
     return DG_OK;
 }
@@ -739,19 +822,29 @@ int default_tile_cfg() {
 
 int launch_gemm_f64(char opA, char opB, const std::vector<Prob>& probs_in, const std::vector<Seg>& segs,
                     hipStream_t stream, int mode) {
-    std::vector<Prob> probs;
-    bool vec = true, dma = false;
-    int tiles[2] = {0, 0};
-    int rc = prepare_gemm(probs_in, segs, probs, &vec, tiles, &dma);
+    GemmBatch hb;
+    int rc = prepare_gemm(opA, opB, probs_in, segs, hb);
     if (rc) return rc;
-    if (probs.empty()) return DG_OK;
-    const Prob* dp = static_cast<const Prob*>(cached_table(probs.data(), probs.size() * sizeof(Prob), stream));
-    if (!dp) return DG_ERR_ALLOC;
-    const Seg* ds = static_cast<const Seg*>(cached_table(segs.data(), segs.size() * sizeof(Seg), stream));
-    if (!ds) return DG_ERR_ALLOC;
+    if (hb.dma.empty() && hb.gen.empty()) return DG_OK;
+    GemmBatchDev b;
+    auto table = [&](const auto& v, auto** dst) -> bool {
+        using T = typename std::decay_t<decltype(v)>::value_type;
+        *dst = nullptr;
+        if (v.empty()) return true;
+        *dst = static_cast<const T*>(cached_table(v.data(), v.size() * sizeof(T), stream));
+        return *dst != nullptr;
+    };
+    if (!table(hb.dma, &b.dma) || !table(hb.gen, &b.gen) || !table(segs, &b.segs))
+        return DG_ERR_ALLOC;
+    b.ndma = (int)hb.dma.size();
+    b.tiles_dma = hb.tiles_dma;
+    b.ngen = (int)hb.gen.size();
+    b.tiles_gen[0] = hb.tiles_gen[0];
+    b.tiles_gen[1] = hb.tiles_gen[1];
+    b.vec = hb.vec;
+    b.dma_edge = hb.dma_edge;
     if (mode == DG_GEMM_MODE_PERSISTENT) mode = DG_GEMM_MODE_XCD;  // needs a counter: plans only
-    return launch_gemm_f64_dev(opA, opB, vec, mode, default_tile_cfg(), dp, (int)probs.size(), ds, tiles, nullptr,
-                               stream, dma);
+    return launch_gemm_f64_dev(opA, opB, mode, default_tile_cfg(), b, nullptr, stream);
 }
 
 }  // namespace dg

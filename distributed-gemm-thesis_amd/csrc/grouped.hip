@@ -14,10 +14,7 @@
 struct dg_grouped_plan {
     struct Group {
         char opA, opB;
-        bool vec, dma = false;
-        int tiles[2], np;
-        dg::Prob* dprobs = nullptr;
-        dg::Seg* dsegs = nullptr;
+        dg::GemmBatchDev b;         // device tables owned by the plan
     };
     std::vector<Group> groups;
     int mode = DG_PLAN_AUTOTUNE;   // requested
@@ -154,8 +151,7 @@ int launch_ozaki_problem(const dg_grouped_plan* p, int i, hipStream_t st) {
 
 int plan_launch(dg_grouped_plan* p, int variant, int cfg, hipStream_t st, bool with_ozaki = true) {
     for (auto& g : p->groups) {
-        int rc = launch_gemm_f64_dev(g.opA, g.opB, g.vec, variant, cfg, g.dprobs, g.np, g.dsegs, g.tiles, p->counter, st,
-                                     g.dma);
+        int rc = launch_gemm_f64_dev(g.opA, g.opB, variant, cfg, g.b, p->counter, st);
         if (rc) return rc;
     }
     if (with_ozaki)
@@ -173,31 +169,36 @@ int build_groups(dg_grouped_plan* p, const dg_gemm_desc* descs, int count, const
     for (int i = 0; i < count; ++i)
         if (!use || (*use)[i]) sel.push_back(descs[i]);
     for (auto& kv : by_ops(sel.data(), (int)sel.size())) {
-        std::vector<Prob> probs0, probs;
+        std::vector<Prob> probs0;
         std::vector<Seg> segs;
         build_group(sel.data(), kv.second, probs0, segs);
-        bool vec = true, dma = false;
-        int tiles[2] = {0, 0};
-        int rc = prepare_gemm(probs0, segs, probs, &vec, tiles, &dma);
+        GemmBatch hb;
+        int rc = prepare_gemm(kv.first.first, kv.first.second, probs0, segs, hb);
         if (rc) return rc;
-        if (probs.empty()) continue;
+        if (hb.dma.empty() && hb.gen.empty()) continue;
         dg_grouped_plan::Group g;
         g.opA = kv.first.first;
         g.opB = kv.first.second;
-        g.vec = vec;
-        g.dma = dma;
-        g.tiles[0] = tiles[0];
-        g.tiles[1] = tiles[1];
-        g.np = (int)probs.size();
-        if (hipMalloc(&g.dprobs, probs.size() * sizeof(Prob)) != hipSuccess ||
-            hipMalloc(&g.dsegs, segs.size() * sizeof(Seg)) != hipSuccess) {
-            p->groups.push_back(g);
-            return fail(DG_ERR_ALLOC, "plan: table alloc");
-        }
         p->groups.push_back(g);
-        if (hipMemcpy(g.dprobs, probs.data(), probs.size() * sizeof(Prob), hipMemcpyHostToDevice) != hipSuccess ||
-            hipMemcpy(g.dsegs, segs.data(), segs.size() * sizeof(Seg), hipMemcpyHostToDevice) != hipSuccess)
-            return fail(DG_ERR_HIP, "plan: table upload");
+        GemmBatchDev& b = p->groups.back().b;  // owned by the plan from here on (freed on failure too)
+        auto upload = [&](const auto& v, auto** dst) -> bool {
+            using T = typename std::decay_t<decltype(v)>::value_type;
+            *dst = nullptr;
+            if (v.empty()) return true;
+            void* d = nullptr;
+            if (hipMalloc(&d, v.size() * sizeof(T)) != hipSuccess) return false;
+            *dst = static_cast<const T*>(d);
+            return hipMemcpy(d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice) == hipSuccess;
+        };
+        if (!upload(hb.dma, &b.dma) || !upload(hb.gen, &b.gen) || !upload(segs, &b.segs))
+            return fail(DG_ERR_ALLOC, "plan: table upload");
+        b.ndma = (int)hb.dma.size();
+        b.tiles_dma = hb.tiles_dma;
+        b.ngen = (int)hb.gen.size();
+        b.tiles_gen[0] = hb.tiles_gen[0];
+        b.tiles_gen[1] = hb.tiles_gen[1];
+        b.vec = hb.vec;
+        b.dma_edge = hb.dma_edge;
     }
     return DG_OK;
 }
@@ -511,8 +512,9 @@ int dg_grouped_plan_destroy(dg_grouped_plan* p) {
     if (p->graph) (void)hipGraphExecDestroy(p->graph);
     if (p->capture_stream) (void)hipStreamDestroy(p->capture_stream);
     for (auto& g : p->groups) {
-        if (g.dprobs) (void)hipFree(g.dprobs);
-        if (g.dsegs) (void)hipFree(g.dsegs);
+        if (g.b.dma) (void)hipFree(const_cast<dg::Prob*>(g.b.dma));
+        if (g.b.gen) (void)hipFree(const_cast<dg::Prob*>(g.b.gen));
+        if (g.b.segs) (void)hipFree(const_cast<dg::Seg*>(g.b.segs));
     }
     if (p->counter) (void)hipFree(p->counter);
     delete p;

@@ -17,26 +17,28 @@ pytestmark = pytest.mark.gpu
 TOL = 1e-13
 
 
-def colmajor(rows, cols, ld, rng, dev, integer=False, fill=None):
+def colmajor(rows, cols, ld, rng, dev, integer=False, fill=None, offset=0):
     """A (ld x cols) column-major device buffer whose top rows x cols block is
-    random; returns (device tensor, host 2-D view of the logical matrix)."""
+    random, starting `offset` doubles into its allocation (NaN before it);
+    returns (device tensor view, host 2-D view of the logical matrix)."""
     buf = np.full((cols, ld), np.nan if fill is None else fill, dtype=np.float64)
     if integer:
         vals = rng.integers(-8, 9, size=(cols, rows)).astype(np.float64)
     else:
         vals = rng.uniform(-1.0, 1.0, size=(cols, rows))
     buf[:, :rows] = vals
-    t = torch.from_numpy(buf.reshape(-1).copy()).to(dev)
+    flat = np.concatenate([np.full(offset, np.nan), buf.reshape(-1)])
+    t = torch.from_numpy(flat).to(dev)[offset:]
     return t, vals.T.copy()
 
 
-def run(dev, opA, opB, m, n, k, alpha, beta, pad=(0, 0, 0), integer=False, seed=0):
+def run(dev, opA, opB, m, n, k, alpha, beta, pad=(0, 0, 0), integer=False, seed=0, offset=(0, 0)):
     rng = np.random.default_rng(seed)
     ra, ca = (m, k) if opA == "N" else (k, m)
     rb, cb = (k, n) if opB == "N" else (n, k)
     lda, ldb, ldc = max(ra, 1) + pad[0], max(rb, 1) + pad[1], max(m, 1) + pad[2]
-    A, a = colmajor(ra, ca, lda, rng, dev, integer)
-    B, b = colmajor(rb, cb, ldb, rng, dev, integer)
+    A, a = colmajor(ra, ca, lda, rng, dev, integer, offset=offset[0])
+    B, b = colmajor(rb, cb, ldb, rng, dev, integer, offset=offset[1])
     C, c = colmajor(m, n, ldc, rng, dev, integer, fill=7.0)
     if beta == 0.0:
         # beta == 0 must not read C: poison the logical block with NaN
@@ -84,6 +86,48 @@ def test_all_interior_tiles_exact_integer(dev, opA, opB):
     got, ref = run(dev, opA, opB, 384, 256, 208, 1.0, 0.0, integer=True, seed=4)
     assert np.array_equal(got, ref)
     got, ref = run(dev, opA, opB, 128, 128, 16, -2.0, 0.5, integer=True, seed=5)
+    assert np.array_equal(got, ref)
+
+
+# Edge tiles of the direct-to-LDS kernel: even leading dimensions keep the
+# 128-bit path; the NaN padding of A and B (rows past the matrix) fails the
+# test if any load reads past a row or past a segment's K; odd row counts of
+# the row-contiguous operand (A for opA = N, B for opB = T) send the last row
+# or column of C to the remainder kernel.
+@pytest.mark.parametrize("opA,opB,m,n,k", [
+    ("N", "N", 301, 203, 86),    # odd m: last row split off; partial last k-step (6 of 8)
+    ("N", "T", 255, 129, 77),    # odd m and n: last row and column split off; odd K
+    ("T", "N", 129, 131, 94),    # k-contiguous A and B: clamped rows, partial k-step
+    ("T", "T", 130, 257, 62),    # odd n with opB = T: last column split off
+    ("N", "N", 1, 300, 18),      # a single row: all of it in the remainder kernel
+    ("N", "T", 300, 1, 40),      # a single column: likewise
+    ("T", "N", 5, 7, 2),         # one partial k-step, padded to two
+])
+def test_dma_edge_tiles_exact_integer(dev, opA, opB, m, n, k):
+    ra = m if opA == "N" else k
+    rb = k if opB == "N" else n
+    got, ref = run(dev, opA, opB, m, n, k, 1.0, 0.0, pad=(ra % 2 + 2, rb % 2 + 2, 3), integer=True, seed=8)
+    assert np.array_equal(got, ref)
+    got, ref = run(dev, opA, opB, m, n, k, -0.5, 2.0, pad=(ra % 2, rb % 2, 1), integer=True, seed=9)
+    assert np.array_equal(got, ref)
+
+
+# A row-contiguous operand (A for opA = N, B for opB = T) that starts 8 bytes
+# past a 16-byte boundary: the direct-to-LDS loads start one row earlier
+# (NaN there must not reach C).
+@pytest.mark.parametrize("opA,opB,m,n,k", [
+    ("N", "N", 301, 200, 96),
+    ("N", "T", 128, 256, 64),
+    ("T", "T", 200, 129, 40),
+    ("N", "T", 255, 257, 33),
+])
+def test_dma_shifted_row_operand_exact_integer(dev, opA, opB, m, n, k):
+    ra = m if opA == "N" else k
+    rb = k if opB == "N" else n
+    off = (1 if opA == "N" else 0, 1 if opB == "T" else 0)
+    got, ref = run(dev, opA, opB, m, n, k, 1.0, 0.0, pad=(ra % 2, rb % 2, 0), integer=True, seed=10, offset=off)
+    assert np.array_equal(got, ref)
+    got, ref = run(dev, opA, opB, m, n, k, 0.5, -1.0, pad=(ra % 2, rb % 2, 2), integer=True, seed=11, offset=off)
     assert np.array_equal(got, ref)
 
 

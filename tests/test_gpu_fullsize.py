@@ -112,9 +112,12 @@ def test_config5_grouped_full(dev):
     plan = dgamd.GroupedPlan(descs, dgamd.DG_PLAN_AUTOTUNE)
     plan.graph_launch()
     torch.cuda.synchronize()
-    # bitwise against per-problem DGEMM
+    # bitwise against per-problem DGEMM, tile by tile
     D = torch.full_like(C, float("nan"))
-    dgamd.dgemm_t("N", "N", n, n, n, 1.0, A, n, B, n, 0.0, D, n)
+    for i in range(drow):
+        for j in range(dcol):
+            m_, n_ = ro[i + 1] - ro[i], co[j + 1] - co[j]
+            dgamd.dgemm_t("N", "N", m_, n_, n, 1.0, A[ro[i]:], n, B[co[j] * n:], n, 0.0, D[ro[i] + co[j] * n:], n)
     assert torch.equal(C, D)
     del D
     for (m_, n_, k_, Ae, Be, Ce) in extras[::16]:
