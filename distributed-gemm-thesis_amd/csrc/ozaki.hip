@@ -376,7 +376,6 @@ struct CrtTables {
     int32_t cinv[kMaxModuli];           // M_t^-1 mod p_t
     double mt[kMaxModuli][3];           // M_t in 40-bit limbs (as doubles), least significant first
     uint32_t P[4];                      // P = prod p_t
-    uint32_t Ph[4];                     // floor(P / 2)
     double Pd;                          // P rounded to double
     double invPd;                       // 1 / Pd rounded to double
 };
@@ -459,8 +458,8 @@ __global__ __launch_bounds__(256) void crt_residue_kernel(const double* __restri
 struct CrtArgs {
     const int8_t* a;     // residues of A', [N][Mp][Kp]
     const int8_t* b;     // residues of B', [N][Np][Kp]
-    uint8_t* res;        // CRT digits x_t of V, [N][Np][Mp] (column-major per modulus)
-    int64_t Mp, Np, Kp;
+    uint8_t* res;        // CRT digits x_t of V, [N][Np][Mp] (column-major per modulus, planes res_plane apart)
+    int64_t Mp, Np, Kp, res_plane;
     int64_t a_slice, b_slice;
     int32_t tiles_m, tiles_n, nmod;
     int32_t cinv[kMaxModuli];  // (P / p_t)^-1 mod p_t
@@ -599,7 +598,7 @@ __global__ __launch_bounds__(CNT, OCC) void crt_gemm_kernel(CrtArgs g) {
     const int p = c_moduli[tmod];
     const int cinv = g.cinv[tmod];
     const double inv_p = 1.0 / p;
-    uint8_t* out = g.res + (int64_t)tmod * g.Mp * g.Np;
+    uint8_t* out = g.res + (int64_t)tmod * g.res_plane;
 #pragma unroll
     for (int j = 0; j < FN; ++j)
 #pragma unroll
@@ -616,89 +615,43 @@ __global__ __launch_bounds__(CNT, OCC) void crt_gemm_kernel(CrtArgs g) {
 
 // The balanced CRT value of one element, correctly rounded to double, from
 // Le[l] = sum over 32-bit limb positions l of S = sum_t x_t M_t (each Le[l]
-// below 2^40, so the carries fit 64 bits).
+// below 2^40, so the carries fit 64 bits).  S = V + jP with j in [0, 16] and
+// |V| <= P/4 (the scaling bound), so S/P lies within 1/4 of the integer j and
+// j = rint(S * fl(1/P)) exactly (the floating estimate is off by ~1e-14);
+// V = S - jP follows exactly in limbs.  Branch-free.
 __host__ __device__ inline double crt_value(const uint64_t (&Le)[4], const CrtTables& tb) {
     // carry-propagate into 32-bit limbs s0..s4
-    uint32_t s0, s1, s2, s3, s4;
-    {
-        uint64_t c0 = Le[0];
-        s0 = (uint32_t)c0;
-        uint64_t c1 = Le[1] + (c0 >> 32);
-        s1 = (uint32_t)c1;
-        uint64_t c2 = Le[2] + (c1 >> 32);
-        s2 = (uint32_t)c2;
-        uint64_t c3 = Le[3] + (c2 >> 32);
-        s3 = (uint32_t)c3;
-        s4 = (uint32_t)(c3 >> 32);
-    }
-    // quotient estimate (S < 16 P; with S and 1/P rounded the estimate is off by at most one)
+    const uint64_t c0 = Le[0];
+    const uint64_t c1 = Le[1] + (c0 >> 32);
+    const uint64_t c2 = Le[2] + (c1 >> 32);
+    const uint64_t c3 = Le[3] + (c2 >> 32);
+    const uint32_t s0 = (uint32_t)c0, s1 = (uint32_t)c1, s2 = (uint32_t)c2, s3 = (uint32_t)c3;
+    const uint32_t s4 = (uint32_t)(c3 >> 32);
     const double Sd = (((((double)s4 * 4294967296.0 + (double)s3) * 4294967296.0) + (double)s2) * 4294967296.0 +
                        (double)s1) * 4294967296.0 + (double)s0;
-    uint32_t qt = (uint32_t)floor(Sd * tb.invPd);
-    if (qt > 16) qt = 16;
-    // R = S - qt * P (five limbs, two's complement)
-    int64_t r0, r1, r2, r3, r4;
-    {
-        uint64_t b0 = (uint64_t)qt * tb.P[0];
-        uint64_t b1 = (uint64_t)qt * tb.P[1] + (b0 >> 32);
-        uint64_t b2 = (uint64_t)qt * tb.P[2] + (b1 >> 32);
-        uint64_t b3 = (uint64_t)qt * tb.P[3] + (b2 >> 32);
-        const uint32_t b4 = (uint32_t)(b3 >> 32);
-        int64_t d = (int64_t)s0 - (int64_t)(uint32_t)b0;
-        r0 = d & 0xffffffff;
-        d = (int64_t)s1 - (int64_t)(uint32_t)b1 + (d >> 32);
-        r1 = d & 0xffffffff;
-        d = (int64_t)s2 - (int64_t)(uint32_t)b2 + (d >> 32);
-        r2 = d & 0xffffffff;
-        d = (int64_t)s3 - (int64_t)(uint32_t)b3 + (d >> 32);
-        r3 = d & 0xffffffff;
-        r4 = (int64_t)s4 - (int64_t)b4 + (d >> 32);  // sign of R
-    }
-    auto add_limbs = [&](const uint32_t (&v)[4], int sign) {
-        int64_t d = r0 + sign * (int64_t)v[0];
-        r0 = d & 0xffffffff;
-        d = r1 + sign * (int64_t)v[1] + (d >> 32);
-        r1 = d & 0xffffffff;
-        d = r2 + sign * (int64_t)v[2] + (d >> 32);
-        r2 = d & 0xffffffff;
-        d = r3 + sign * (int64_t)v[3] + (d >> 32);
-        r3 = d & 0xffffffff;
-        r4 += d >> 32;
-    };
-    auto greater_eq = [&](const uint32_t (&v)[4]) {  // R >= v, for R >= 0
-        if (r4 != 0) return true;
-        if ((uint32_t)r3 != v[3]) return (uint32_t)r3 > v[3];
-        if ((uint32_t)r2 != v[2]) return (uint32_t)r2 > v[2];
-        if ((uint32_t)r1 != v[1]) return (uint32_t)r1 > v[1];
-        return (uint32_t)r0 >= v[0];
-    };
-    if (r4 < 0) add_limbs(tb.P, 1);
-    else if (greater_eq(tb.P)) add_limbs(tb.P, -1);
-    // balanced: R > floor(P/2) -> R - P
-    bool neg = false;
-    if (greater_eq(tb.Ph) && !((uint32_t)r3 == tb.Ph[3] && (uint32_t)r2 == tb.Ph[2] &&
-                               (uint32_t)r1 == tb.Ph[1] && (uint32_t)r0 == tb.Ph[0])) {
-        add_limbs(tb.P, -1);  // now negative: magnitude P - R
-        neg = true;
-    }
-    uint64_t lo = ((uint64_t)(uint32_t)r1 << 32) | (uint32_t)r0;
-    uint64_t hi = ((uint64_t)(uint32_t)r3 << 32) | (uint32_t)r2;
-    if (neg) {  // two's complement of the 128-bit value
-        lo = ~lo + 1;
-        hi = ~hi + (lo == 0 ? 1 : 0);
-    }
-    // correctly rounded double of the 128-bit magnitude (hi, lo)
-    double v;
-    if (hi == 0) {
-        v = (double)lo;
-    } else {
-        const int z = __builtin_clzll(hi);
-        const uint64_t top = z ? (hi << z) | (lo >> (64 - z)) : hi;
-        const uint64_t rest = z ? (lo << z) : lo;
-        v = ldexp((double)(top | (rest != 0 ? 1ull : 0ull)), 64 - z);
-    }
-    if (neg) v = -v;
-    return v;
+    const uint32_t j = (uint32_t)rint(Sd * tb.invPd);
+    // V = S - j P over 32-bit limbs; |V| < 2^127, so limbs 0..3 hold it in two's complement
+    const uint64_t b0 = (uint64_t)j * tb.P[0];
+    const uint64_t b1 = (uint64_t)j * tb.P[1] + (b0 >> 32);
+    const uint64_t b2 = (uint64_t)j * tb.P[2] + (b1 >> 32);
+    const uint64_t b3 = (uint64_t)j * tb.P[3] + (b2 >> 32);
+    const uint64_t lo_s = ((uint64_t)s1 << 32) | s0, hi_s = ((uint64_t)s3 << 32) | s2;
+    const uint64_t lo_b = (b1 << 32) | (uint32_t)b0, hi_b = (b3 << 32) | (uint32_t)b2;
+    uint64_t lo = lo_s - lo_b;
+    uint64_t hi = hi_s - hi_b - (lo_s < lo_b ? 1u : 0u);
+    const bool neg = (int64_t)hi < 0;
+    // magnitude
+    const uint64_t nlo = ~lo + 1;
+    const uint64_t nhi = ~hi + (nlo == 0 ? 1u : 0u);
+    lo = neg ? nlo : lo;
+    hi = neg ? nhi : hi;
+    // correctly rounded double of the 128-bit magnitude (hi, lo): the top 64
+    // bits with a sticky bit for everything below, converted once
+    const int z = hi ? __builtin_clzll(hi) : 64;
+    const uint64_t top = z == 64 ? lo : (z ? (hi << z) | (lo >> (64 - z)) : hi);
+    const uint64_t rest = z == 64 ? 0 : (z ? (lo << z) : lo);
+    const double v = ldexp((double)(top | (rest != 0 ? 1ull : 0ull)), z == 64 ? 0 : 64 - z);
+    return neg ? -v : v;
 }
 
 // Four consecutive rows per thread (one 32-bit load of CRT digits per modulus).
@@ -710,7 +663,7 @@ __host__ __device__ inline double crt_value(const uint64_t (&Le)[4], const CrtTa
 // C = alpha * ldexp(V, e_i + f_j - 2q) + beta*C, four rows per 32-byte access
 // when C allows it.
 __global__ void crt_reconstruct_kernel(const uint8_t* __restrict__ res, CrtTables tb, int64_t m, int64_t n,
-                                       int64_t Mp, int64_t Np, const int32_t* __restrict__ ea,
+                                       int64_t Mp, int64_t plane, const int32_t* __restrict__ ea,
                                        const int32_t* __restrict__ eb, int q, double alpha, double beta,
                                        double* __restrict__ C, int64_t ldc) {
     const int64_t qm = (m + 3) / 4;
@@ -728,7 +681,7 @@ __global__ void crt_reconstruct_kernel(const uint8_t* __restrict__ res, CrtTable
 #pragma unroll
         for (int t = 0; t < kMaxModuli; ++t) {
             if (t < tb.n) {
-                const uint32_t w = *reinterpret_cast<const uint32_t*>(rp + (int64_t)t * Mp * Np);
+                const uint32_t w = *reinterpret_cast<const uint32_t*>(rp + (int64_t)t * plane);
 #pragma unroll
                 for (int e = 0; e < 4; ++e) {
                     const double xd = (double)((w >> (8 * e)) & 255u);
@@ -882,7 +835,7 @@ constexpr int64_t kMaxChunk = 256;  // k-chunk of the max reduction
 int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
 
 struct Plan {
-    int64_t Mp, Np, Kp, esz, a_slice, b_slice, nchunk_a, nchunk_b, kchunk_max;
+    int64_t Mp, Np, Kp, esz, a_slice, b_slice, res_plane, nchunk_a, nchunk_b, kchunk_max;
     size_t off_a, off_b, off_ea, off_eb, off_part, off_w, total;
 };
 
@@ -918,7 +871,6 @@ int crt_setup(int64_t k, int nmod, CrtTables* tb, int* q_out) {
         tb->cinv[t] = (int32_t)(((x0 % p) + p) % p);
     }
     limbs(P, tb->P);
-    limbs(P / 2, tb->Ph);
     tb->Pd = (double)P;
     tb->invPd = 1.0 / tb->Pd;
     return DG_OK;
@@ -947,6 +899,7 @@ Plan make_crt_plan(int64_t m, int64_t n, int64_t k, int nmod) {
     p.Kp = round_up(k > 0 ? k : 1, KSB);
     p.a_slice = p.Mp * p.Kp;
     p.b_slice = p.Np * p.Kp;
+    p.res_plane = p.Mp * p.Np;
     p.kchunk_max = kMaxChunk;
     while ((k + p.kchunk_max - 1) / p.kchunk_max > 4096) p.kchunk_max *= 2;
     p.nchunk_a = (k + p.kchunk_max - 1) / p.kchunk_max;
@@ -959,7 +912,7 @@ Plan make_crt_plan(int64_t m, int64_t n, int64_t k, int nmod) {
     p.off_ea = take((size_t)p.Mp * 4);
     p.off_eb = take((size_t)p.Np * 4);
     p.off_part = take((size_t)p.nchunk_a * (size_t)(m > n ? m : n) * 8);
-    p.off_w = take((size_t)nmod * (size_t)p.Mp * (size_t)p.Np);  // residues of V
+    p.off_w = take((size_t)nmod * (size_t)p.res_plane);  // CRT digits of V
     p.total = off;
     return p;
 }
@@ -1146,6 +1099,7 @@ int dg_ozaki_dgemm(char opA, char opB, int64_t m, int64_t n, int64_t k, double a
         CrtArgs cg{};
         cg.a = sa; cg.b = sb; cg.res = reinterpret_cast<uint8_t*>(ws + p.off_w);
         cg.Mp = p.Mp; cg.Np = p.Np; cg.Kp = p.Kp; cg.a_slice = p.a_slice; cg.b_slice = p.b_slice;
+        cg.res_plane = p.res_plane;
         cg.tiles_m = (int32_t)(p.Mp / BM); cg.tiles_n = (int32_t)(p.Np / BN); cg.nmod = slices;
         for (int t = 0; t < kMaxModuli; ++t) cg.cinv[t] = tables.cinv[t];
         const int64_t grid = (int64_t)cg.tiles_m * cg.tiles_n * slices;
@@ -1155,7 +1109,7 @@ int dg_ozaki_dgemm(char opA, char opB, int64_t m, int64_t n, int64_t k, double a
         const int64_t blocks = (((m + 3) / 4) * n + 255) / 256;
         if (blocks > INT32_MAX) return fail(DG_ERR_INVALID, "dg_ozaki_dgemm: too many output elements");
         hipLaunchKernelGGL(crt_reconstruct_kernel, dim3((unsigned)blocks), dim3(256), 0, st, cg.res, tables, m, n, p.Mp,
-                           p.Np, ea, eb, q, alpha, beta, C, ldc);
+                           p.res_plane, ea, eb, q, alpha, beta, C, ldc);
         DG_LAUNCH_CHECK();
         return DG_OK;
     }
