@@ -354,14 +354,20 @@ __global__ __launch_bounds__(CNT, OCC) void oz_gemm_kernel(OzArgs g) {
 // ------------------------------------------------------------ Ozaki scheme II
 // Ozaki, Uchino, Imamura, "Ozaki Scheme II: A GEMM-oriented emulation of
 // floating-point matrix multiplication using an integer modular technique"
-// (2025).  A' = rint(op(A)_ik * 2^(q - e_i)) (rounded to nearest, which
-// halves the RMS scaling error of truncation) and B' likewise are integers
-// of magnitude <= 2^q; V = A'B' is computed modulo N pairwise coprime moduli
-// p_t <= 256 (one INT8 GEMM each, residues balanced into [-128, 127]) and
-// rebuilt exactly by the Chinese remainder theorem, V = sum_t x_t (P/p_t)
-// mod P, balanced; q is chosen so that k * 4^q <= P/4 (P = prod p_t), hence
-// |V| <= P/4 is recovered without ambiguity.
-// C = alpha * ldexp(double(V), e_i + f_j - 2q) + beta*C with double(V)
+// (2025).  A' = rint(op(A)_ik * 2^sigma_i) (rounded to nearest, which
+// halves the RMS scaling error of truncation) and B' = rint(op(B)_kj *
+// 2^tau_j) are integers; the shifts follow the paper's Cauchy-Schwarz bound
+// (its fast mode): every scaled line has 2-norm <= 2^Q before rounding, with
+// Q the largest such that 2^(2Q) + ceil(sqrt(k)) 2^Q + k <= P/4
+// (P = prod p_t), so |V| = |A'B'| <= P/4.  The norm bound is an exact integer
+// sum (crt_bound_term), so the shifts do not depend on summation order; a
+// line whose norm is far below sqrt(k) times its largest entry (wide-ranging
+// data) gets more bits than the per-entry bound would give it, up to 53.
+// V is computed modulo N pairwise coprime moduli p_t <= 256 (one INT8 GEMM
+// each, residues balanced into [-128, 127]) and rebuilt exactly by the
+// Chinese remainder theorem, V = sum_t x_t (P/p_t) mod P, balanced.
+// Shifts are stored as e' = Q - sigma (e'_i, f'_j), and
+// C = alpha * ldexp(double(V), e'_i + f'_j - 2Q) + beta*C with double(V)
 // correctly rounded.
 
 constexpr int kMaxModuli = 16;
@@ -775,6 +781,82 @@ __global__ void oz_exponent_kernel(const double* __restrict__ part, int64_t rows
     }
 }
 
+// Ozaki scheme II scaling (Cauchy-Schwarz bound).  Entry bound: |x| = m 2^ex
+// (m in [0.5, 1)) counts ceil(256 m)^2 2^(16 - 2d), d = e - ex >= 0 for the
+// line's max exponent e, rounded up to an integer (at least 1) when 2d > 16:
+// an exact integer upper bound of 2^32 (x 2^-e)^2 whose sums do not depend on
+// the summation order.
+__device__ __forceinline__ uint64_t crt_bound_term(double x, int e) {
+    if (x == 0.0) return 0;
+    int ex;
+    const double m = frexp(fabs(x), &ex);
+    const uint64_t M = (uint64_t)ceil(m * 256.0);
+    const uint64_t M2 = M * M;
+    const int sh = 2 * (e - ex) - 16;
+    if (sh <= 0) return M2 << (-sh);
+    if (sh >= 17) return 1;
+    return (M2 + ((uint64_t)1 << sh) - 1) >> sh;
+}
+
+// Per k-chunk sums of the entry bounds of each line (same access patterns
+// as oz_partial_max_kernel); e holds the lines' max exponents.
+__global__ void crt_partial_bound_kernel(const double* __restrict__ x, int64_t rows, int64_t K, int64_t rs, int64_t ks,
+                                         int64_t kchunk, const int32_t* __restrict__ e, uint64_t* __restrict__ part) {
+    const int64_t k0 = blockIdx.y * kchunk;
+    const int64_t k1 = min(K, k0 + kchunk);
+    if (ks != 1) {
+        for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < rows; r += (int64_t)gridDim.x * blockDim.x) {
+            const double* xr = x + r * rs;
+            const int er = e[r];
+            uint64_t s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+            int64_t k = k0;
+            for (; k + 4 <= k1; k += 4) {
+                s0 += crt_bound_term(xr[k * ks], er);
+                s1 += crt_bound_term(xr[(k + 1) * ks], er);
+                s2 += crt_bound_term(xr[(k + 2) * ks], er);
+                s3 += crt_bound_term(xr[(k + 3) * ks], er);
+            }
+            for (; k < k1; ++k) s0 += crt_bound_term(xr[k * ks], er);
+            part[blockIdx.y * rows + r] = (s0 + s1) + (s2 + s3);
+        }
+    } else {
+        const int lane = threadIdx.x & 63;
+        const int64_t waves = (int64_t)gridDim.x * (blockDim.x >> 6);
+        for (int64_t r = blockIdx.x * (int64_t)(blockDim.x >> 6) + (threadIdx.x >> 6); r < rows; r += waves) {
+            const double* xr = x + r * rs;
+            const int er = e[r];
+            uint64_t sum = 0;
+            for (int64_t k = k0 + lane; k < k1; k += 64) sum += crt_bound_term(xr[k], er);
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) sum += __shfl_xor(sum, off);
+            if (lane == 0) part[blockIdx.y * rows + r] = sum;
+        }
+    }
+}
+
+// Line shifts: S = sum of the chunk bounds, sigma the largest shift with
+// 2^(2(sigma + e)) S <= 2^(2Q + 32) (sigma = Q + 16 - e - ceil(log4 S)),
+// capped at 53 - e; an all-zero line keeps sigma = 0.  Stored in place of the
+// max exponent as e' = Q - sigma, the form the residue and reconstruction
+// kernels take.
+__global__ void crt_shift_kernel(const uint64_t* __restrict__ part, int64_t rows, int64_t rows_p, int nchunk, int Q,
+                                 int32_t* __restrict__ e) {
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < rows_p; r += (int64_t)gridDim.x * blockDim.x) {
+        int sig = 0;
+        if (r < rows) {
+            uint64_t S = 0;
+            for (int c = 0; c < nchunk; ++c) S += part[(int64_t)c * rows + r];
+            if (S != 0) {
+                const int bl = S > 1 ? 64 - __builtin_clzll(S - 1) : 0;  // bit length of S - 1
+                const int j = (bl + 1) >> 1;                              // ceil(log4 S)
+                const int er = e[r];
+                sig = min(53 - er, Q + 16 - er - j);
+            }
+        }
+        e[r] = Q - sig;
+    }
+}
+
 // Writes the s digits of op(X)(row, k) into slices[t][row][Kp] (zeros in the
 // padding).  A 64x64 tile goes through LDS so that both the read of op(X)
 // and the k-contiguous slice writes are coalesced whatever op(X)'s layout.
@@ -843,11 +925,17 @@ struct Plan {
 int crt_setup(int64_t k, int nmod, CrtTables* tb, int* q_out) {
     unsigned __int128 P = 1;
     for (int t = 0; t < nmod; ++t) P *= (unsigned)crt_modulus(t);
-    // largest q <= 53 with k * 4^q * 4 <= P
-    int q = 53;
-    while (q > 0) {
-        unsigned __int128 need = (unsigned __int128)(k > 0 ? k : 1) << (2 * q + 2);
-        if (2 * q + 2 < 127 && need <= P) break;
+    // largest Q with 2^(2Q) + ceil(sqrt(k)) 2^Q + k <= P/4: lines of 2-norm
+    // <= 2^Q before rounding keep |A'B'| <= P/4 (Cauchy-Schwarz)
+    const uint64_t kk = (uint64_t)(k > 0 ? k : 0);
+    uint64_t rk = (uint64_t)sqrt((double)kk);
+    while (rk * rk > kk) --rk;
+    while ((rk + 1) * (rk + 1) <= kk) ++rk;
+    if (rk * rk < kk) ++rk;  // ceil(sqrt(k))
+    int q = 62;
+    while (q >= 1) {
+        const unsigned __int128 need = ((unsigned __int128)1 << (2 * q)) + ((unsigned __int128)rk << q) + kk;
+        if (need <= (P >> 2)) break;
         --q;
     }
     if (q < 1) return fail(DG_ERR_UNSUPPORTED, "dg_ozaki_dgemm: %d moduli leave no bits for k=%lld", nmod, (long long)k);
@@ -1027,8 +1115,13 @@ int dg_ozaki_dgemm(char opA, char opB, int64_t m, int64_t n, int64_t k, double a
         if (k > 131071) return fail(DG_ERR_UNSUPPORTED, "dg_ozaki_dgemm: CRT form needs k < 131072 (exact int32 sums)");
         int rc0 = crt_setup(k, slices, &tables, &q);
         if (rc0) return rc0;
-        // |A - 2^(e-q) A'| < 2^(e-q) per entry: the product error is below k * 2^(1-q) * 2^(e_i+f_j)
-        if (err_bound) *err_bound = (double)k * ldexp(1.0, 1 - q);
+        // every line's shift reaches sigma + e >= qe = min(53, Q - ceil(log4 k)) (its
+        // bound sum is at most k 2^32), so |A - 2^-sigma A'| <= 2^(e - qe - 1) per entry
+        // and the product error is below k * 2^(1-qe) * 2^(e_i+f_j)
+        int l4 = 0;
+        while (((int64_t)1 << (2 * l4)) < k) ++l4;
+        const int qe = q - l4 < 53 ? q - l4 : 53;
+        if (err_bound) *err_bound = (double)k * ldexp(1.0, 1 - qe);
     } else if (err_bound) {
         *err_bound = (double)k * (slices + 3) * ldexp(1.0, -bits * slices);
     }
@@ -1087,6 +1180,21 @@ int dg_ozaki_dgemm(char opA, char opB, int64_t m, int64_t n, int64_t k, double a
         hipLaunchKernelGGL(oz_exponent_kernel, dim3((unsigned)((p.Np + NT - 1) / NT)), dim3(NT), 0, st, part, n, p.Np,
                            nchunk, eb);
         DG_LAUNCH_CHECK();
+        if (crt) {
+            // line shifts from the Cauchy-Schwarz bound (the chunk buffer now holds bound sums)
+            uint64_t* bp = reinterpret_cast<uint64_t*>(part);
+            gx = pm_grid(m, a_ks);
+            hipLaunchKernelGGL(crt_partial_bound_kernel, dim3(gx, gy), dim3(NT), 0, st, A, m, k, a_rs, a_ks, p.kchunk_max,
+                               ea, bp);
+            hipLaunchKernelGGL(crt_shift_kernel, dim3((unsigned)((p.Mp + NT - 1) / NT)), dim3(NT), 0, st, bp, m, p.Mp,
+                               nchunk, q, ea);
+            gx = pm_grid(n, b_ks);
+            hipLaunchKernelGGL(crt_partial_bound_kernel, dim3(gx, gy), dim3(NT), 0, st, B, n, k, b_rs, b_ks, p.kchunk_max,
+                               eb, bp);
+            hipLaunchKernelGGL(crt_shift_kernel, dim3((unsigned)((p.Np + NT - 1) / NT)), dim3(NT), 0, st, bp, n, p.Np,
+                               nchunk, q, eb);
+            DG_LAUNCH_CHECK();
+        }
     }
     if (crt) {
         dim3 ga((unsigned)(p.Kp / RT_K), (unsigned)(p.Mp / RT_R));

@@ -240,10 +240,15 @@ int64_t dg_ozaki_workspace(int64_t m, int64_t n, int64_t k, int slices, int kind
  * aligned.  If err_bound is not NULL it receives k*(s+3)*2^(-b*s) (b = 7 for
  * INT8, 5 for BF16): |C - op(A)op(B)|_ij <= err_bound * 2^(e_i + f_j) for the
  * product part, e_i / f_j being the row / column scales.  With
- * DG_OZAKI_INT8_CRT, `slices` is the number of moduli N; rows and columns are
- * scaled to q-bit integers (q <= 53, the largest with k*4^q*4 <= the moduli's
- * product), their integer product is formed exactly from N INT8 GEMMs and
- * rebuilt by the Chinese remainder theorem, and err_bound = k*2^(1-q).
+ * DG_OZAKI_INT8_CRT, `slices` is the number of moduli N; every row of op(A)
+ * and column of op(B) is scaled by a power of two to integers (at most 53
+ * bits) whose lines have 2-norm <= 2^Q, Q the largest with
+ * 2^(2Q) + ceil(sqrt(k))*2^Q + k <= P/4 (P the moduli's product; the
+ * Cauchy-Schwarz scaling of Ozaki scheme II), their integer product is formed
+ * exactly from N INT8 GEMMs and rebuilt by the Chinese remainder theorem, and
+ * err_bound = k*2^(1-q) with q = min(53, Q - ceil(log4 k)) (every line gets
+ * at least q bits; lines whose norm is far below sqrt(k) times their largest
+ * entry get more).
  * New relative to the reference (no split DGEMM exists there); it replaces
  * the same cublasDgemm call sites as dg_dgemm (src/25DCannon.cpp:324-325). */
 int dg_ozaki_dgemm(char opA, char opB, int64_t m, int64_t n, int64_t k, double alpha,

@@ -324,26 +324,67 @@ def ozaki_dgemm(a, b, s, alpha=1.0, beta=0.0, c=None, bits=OZ_INT8_BITS, kchunk=
 # ---------------------------------------------------------------- Ozaki scheme II (CRT)
 # Ozaki, Uchino, Imamura, "Ozaki Scheme II: A GEMM-oriented emulation of
 # floating-point matrix multiplication using an integer modular technique"
-# (2025).  Rows of A / columns of B are scaled to integers of q bits (rounded
-# to nearest); their exact integer product is computed modulo N pairwise
-# coprime moduli <= 256 (one INT8 GEMM each) and rebuilt by the Chinese
-# remainder theorem.
+# (2025).  Rows of A / columns of B are scaled by powers of two to integers
+# (rounded to nearest) whose exact integer product is computed modulo N
+# pairwise coprime moduli <= 256 (one INT8 GEMM each) and rebuilt by the
+# Chinese remainder theorem.  The scaling follows the paper's Cauchy-Schwarz
+# ("fast mode") bound: each scaled row of A and column of B has 2-norm at
+# most 2^Q, so |A'B'| <= 2^(2Q) <= P/4; the norm is bounded by an exact
+# integer sum (crt_bound_sums), so the GPU and this restatement pick the same
+# shifts whatever their summation order.
 
 CRT_MODULI = (256, 255, 253, 251, 247, 241, 239, 233, 229, 227, 223, 217, 211, 199, 197, 193, 191, 181, 179, 173)
 
 
-def crt_q(K, nmod):
-    """Bits q per operand so that k * 2^(2q) stays below P/4 (P = product of
-    the first nmod moduli), capped at 53 (integers exact in FP64)."""
+def crt_Q(K, nmod):
+    """Largest Q with 2^(2Q) + ceil(sqrt(K)) 2^Q + K <= P/4 (P = product of
+    the first nmod moduli): rows and columns of 2-norm <= 2^Q before the
+    rounding to integers (which adds at most 1/2 per entry) then keep
+    |A'B'| <= P/4.  None when no Q >= 1 fits."""
     import math
     P = 1
     for p in CRT_MODULI[:nmod]:
         P *= p
-    logp = math.log2(P)
-    q = int((logp - 2 - math.log2(max(K, 1))) // 2)
-    while q > 0 and K * 4 ** q * 4 > P:
-        q -= 1
-    return min(q, 53)
+    r = math.isqrt(max(K, 0))
+    rk = r if r * r == K else r + 1
+    Q = P.bit_length() // 2
+    while Q >= 1 and 4 * (2 ** (2 * Q) + rk * 2 ** Q + K) > P:
+        Q -= 1
+    return Q if Q >= 1 else None
+
+
+def crt_bound_sums(X, e, axis):
+    """Per row (axis=1) or column (axis=0) with max-exponent e (max|x| < 2^e):
+    an exact integer upper bound S of 2^32 * sum_k (x_k 2^-e)^2.  Entry
+    |x| = m 2^ex (m in [0.5, 1)) counts as ceil(256 m)^2 * 2^(16 - 2d),
+    d = e - ex >= 0, rounded up to an integer (so at least 1) when 2d > 16;
+    zeros count 0."""
+    shape = [1, 1]
+    shape[1 - axis] = -1
+    m, ex = np.frexp(np.abs(X))
+    M = np.ceil(m * 256.0).astype(np.int64)
+    M2 = M * M
+    d = e.reshape(shape).astype(np.int64) - ex.astype(np.int64)
+    sh = 2 * d - 16
+    up = np.left_shift(M2, np.clip(-sh, 0, 16))
+    mid = np.right_shift(M2 + np.left_shift(np.int64(1), np.clip(sh, 0, 16)) - 1, np.clip(sh, 0, 16))
+    t = np.where(sh <= 0, up, np.where(sh >= 17, 1, mid))
+    t = np.where(X == 0, 0, t)
+    return t.sum(axis=axis).astype(np.int64)
+
+
+def crt_shifts(e, S, Q):
+    """The scaling as an exponent e' = Q - sigma per line (the form crt_scale
+    and the reconstruction take): sigma is the largest shift with
+    2^(2(sigma + e)) S <= 2^(2Q + 32), i.e. sigma = Q + 16 - e - j with
+    j = ceil(log4 S) exactly, capped at 53 - e (|x 2^sigma| < 2^53, integers
+    exact in FP64); an all-zero line (S = 0) keeps sigma = 0."""
+    Sl = [int(v) for v in np.asarray(S).ravel()]
+    j = np.array([(max(v - 1, 0).bit_length() + 1) // 2 for v in Sl], dtype=np.int64)
+    el = np.asarray(e).astype(np.int64).ravel()
+    sig = np.minimum(53 - el, Q + 16 - el - j)
+    sig = np.where(np.array(Sl, dtype=object) == 0, 0, sig).astype(np.int64)
+    return (Q - sig).astype(np.int64).reshape(np.asarray(e).shape)
 
 
 def crt_bal(x, p):
@@ -354,8 +395,8 @@ def crt_bal(x, p):
 
 def crt_scale(X, e, axis, q):
     """Integers rint(x * 2^(q - e)), rounded to nearest (ties to even), as
-    int64: the scaling is exact in FP64 for q <= 53 and |x| < 2^e gives
-    |result| <= 2^q."""
+    int64 (one exponent e per row, axis=1, or column, axis=0): a power-of-two
+    scaling, exact in FP64 while |x 2^(q - e)| < 2^53."""
     shape = [1, 1]
     shape[1 - axis] = -1
     return np.rint(np.ldexp(X, (q - e).reshape(shape))).astype(np.int64)
@@ -394,12 +435,15 @@ def ozaki2_dgemm(a, b, nmod, alpha=1.0, beta=0.0, c=None):
     """C = alpha * A B + beta * C through nmod INT8 modular products (Ozaki
     scheme II) on 2-D float64 arrays a (M x K), b (K x N).  The rebuilt
     integer V = A'B' is exact; the result is alpha * ldexp(double(V),
-    e_i + f_j - 2q) (+ beta * C), double(V) correctly rounded."""
+    e'_i + f'_j - 2Q) (+ beta * C), double(V) correctly rounded, with
+    e' = Q - sigma the shifts of crt_shifts."""
     M, K = a.shape
     N = b.shape[1]
-    q = crt_q(K, nmod)
-    e = oz_exponents(a, 1)
-    f = oz_exponents(b, 0)
+    q = crt_Q(K, nmod)
+    if q is None:
+        raise ValueError("%d moduli leave no bits for K=%d" % (nmod, K))
+    e = crt_shifts(oz_exponents(a, 1), crt_bound_sums(a, oz_exponents(a, 1), 1), q)
+    f = crt_shifts(oz_exponents(b, 0), crt_bound_sums(b, oz_exponents(b, 0), 0), q)
     Ai = crt_scale(a, e, 1, q)
     Bi = crt_scale(b, f, 0, q)
     ps = CRT_MODULI[:nmod]

@@ -128,16 +128,69 @@ def test_crt_reconstruct_exact(nmod):
     assert list(got) == vals
 
 
-def test_crt_q_bound():
+def test_crt_Q_bound():
+    import math
     for K in (1, 100, 4096, 16384, 100000):
         for nm in range(2, 17):
-            q = oracle.crt_q(K, nm)
-            P = 1
-            for p in oracle.CRT_MODULI[:nm]:
-                P *= p
-            if q > 0:
-                assert K * 4 ** q * 4 <= P
-                assert q == 53 or K * 4 ** (q + 1) * 4 > P
+            P = math.prod(oracle.CRT_MODULI[:nm])
+            Q = oracle.crt_Q(K, nm)
+            rk = math.isqrt(K) + (0 if math.isqrt(K) ** 2 == K else 1)
+            fits = lambda q: 4 * (2 ** (2 * q) + rk * 2 ** q + K) <= P
+            if Q is None:
+                assert not fits(1)
+            else:
+                assert fits(Q) and not fits(Q + 1)
+
+
+def test_crt_bound_sums_known_answers():
+    # row max 0.75 = 0.75 * 2^0 (e = 0): 256*0.75 = 192 -> 192^2 * 2^16
+    X = np.array([[0.75, -0.375, 0.0, 2.0 ** -9 * 0.6]])
+    e = oracle.oz_exponents(X, 1)
+    assert e.tolist() == [0]
+    S = oracle.crt_bound_sums(X, e, 1)
+    # -0.375 = 0.75 * 2^-1: d = 1 -> 192^2 * 2^14; 0.6 * 2^-9: m = 0.6, ex = -9, d = 9 ->
+    # ceil(153.6) = 154, ceil(154^2 / 2^2); zero -> 0
+    assert S.tolist() == [192 ** 2 * 2 ** 16 + 192 ** 2 * 2 ** 14 + -(-154 ** 2 // 4)]
+    # d = 10: ceil(M^2 / 2^4); d = 30: 1
+    Y = np.array([[0.5, 0.5 * 2.0 ** -10], [0.5, 0.99 * 2.0 ** -30]]).T   # columns
+    f = oracle.oz_exponents(Y, 0)
+    S2 = oracle.crt_bound_sums(Y, f, 0)
+    assert S2.tolist() == [128 ** 2 * 2 ** 16 + -(-128 ** 2 // 2 ** 4), 128 ** 2 * 2 ** 16 + 1]
+
+
+def test_crt_bound_sums_is_a_tight_upper_bound():
+    rng = np.random.default_rng(3)
+    X = wide(rng, (20, 300))
+    e = oracle.oz_exponents(X, 1)
+    S = oracle.crt_bound_sums(X, e, 1)
+    for i in range(20):
+        exact = sum((Fraction(float(x)) * Fraction(2) ** (-int(e[i]))) ** 2 for x in X[i]) * 2 ** 32
+        assert Fraction(int(S[i])) >= exact
+        assert Fraction(int(S[i])) <= exact * Fraction(102, 100) + 300
+
+
+@pytest.mark.parametrize("nmod", [6, 10, 14, 16])
+def test_crt_shifts_bound_the_product(nmod):
+    # |A'B'| <= P/4 with exact integers, and each shift is the largest the norm bound allows
+    import math
+    rng = np.random.default_rng(nmod)
+    K = 64
+    a = wide(rng, (6, K))
+    b = rng.uniform(-1, 1, (K, 5))
+    Q = oracle.crt_Q(K, nmod)
+    ea = oracle.crt_shifts(oracle.oz_exponents(a, 1), oracle.crt_bound_sums(a, oracle.oz_exponents(a, 1), 1), Q)
+    fb = oracle.crt_shifts(oracle.oz_exponents(b, 0), oracle.crt_bound_sums(b, oracle.oz_exponents(b, 0), 0), Q)
+    A = oracle.crt_scale(a, ea, 1, Q)
+    B = oracle.crt_scale(b, fb, 0, Q)
+    P = math.prod(oracle.CRT_MODULI[:nmod])
+    V = [[sum(int(A[i, k]) * int(B[k, j]) for k in range(K)) for j in range(5)] for i in range(6)]
+    assert max(abs(v) for r in V for v in r) <= P // 4
+    e = oracle.oz_exponents(a, 1)
+    S = oracle.crt_bound_sums(a, e, 1)
+    for i in range(6):
+        sig = Q - int(ea[i])
+        assert 2 ** (2 * (sig + int(e[i]))) * int(S[i]) <= 2 ** (2 * Q + 32) or sig == 53 - int(e[i])
+        assert sig == 53 - int(e[i]) or 2 ** (2 * (sig + 1 + int(e[i]))) * int(S[i]) > 2 ** (2 * Q + 32)
 
 
 def test_crt_scale_rounds_to_nearest_even():
@@ -160,16 +213,22 @@ def test_crt_exact_for_short_integers():
 
 @pytest.mark.parametrize("nmod", [8, 12, 16])
 def test_crt_error_within_bound(nmod):
+    # |a - a' 2^-s| <= 2^(-s-1) per entry with s = Q - e', so per product
+    # term the error is at most 2^(e+f'-Q-1) + 2^(f+e'-Q-1) + 2^(e'+f'-2Q-2)
     rng = np.random.default_rng(nmod)
     a = wide(rng, (5, 40))
     b = wide(rng, (40, 6))
     got = oracle.ozaki2_dgemm(a, b, nmod)
-    q = oracle.crt_q(40, nmod)
+    Q = oracle.crt_Q(40, nmod)
     e = oracle.oz_exponents(a, 1)
     f = oracle.oz_exponents(b, 0)
+    ea = oracle.crt_shifts(e, oracle.crt_bound_sums(a, e, 1), Q)
+    fb = oracle.crt_shifts(f, oracle.crt_bound_sums(b, f, 0), Q)
     for i in range(5):
         for j in range(6):
             exact = sum(Fraction(a[i, k]) * Fraction(b[k, j]) for k in range(40))
-            bound = Fraction(40 * 2) * Fraction(2) ** (int(e[i] + f[j]) - q)
-            slack = Fraction(abs(float(exact))) * Fraction(2) ** -52
-            assert abs(Fraction(got[i, j]) - exact) <= bound + slack
+            two = Fraction(2)
+            term = (two ** (int(e[i]) + int(fb[j]) - Q - 1) + two ** (int(f[j]) + int(ea[i]) - Q - 1)
+                    + two ** (int(ea[i]) + int(fb[j]) - 2 * Q - 2))
+            slack = Fraction(abs(float(exact))) * two ** -52
+            assert abs(Fraction(got[i, j]) - exact) <= 40 * term + slack
