@@ -6,10 +6,10 @@ alpha=1, beta=0.5.  One step = one dg_summa call over the packed
 block-cyclic operands already resident in HBM (every virtual rank's
 local GEMM, all 32 panels, one launch).
 
-    python bench.py [--steps K] [--warmup W] [--gpus 1] [--workload summa|cannon|dgemm]
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--workload summa|cannon|dgemm|grouped|cannon2x2|sgemm]
 
 Prints one JSON line.  `roofline` is the dominant kernel against the FP64
-matrix peak (78.6 TFLOP/s), timed with HIP events on the launch stream;
+matrix peak (78.6 TFLOP/s; the FP32 matrix peak, 157.3, for sgemm), timed with HIP events on the launch stream;
 `cpu_baseline` is the host BLAS dgemm of the oracle on a bounded sample of
 the same work, on this host's cores.
 """
@@ -23,6 +23,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "distributed-gemm-thesis_amd"))
 
 FP64_PEAK = 78.6  # TFLOP/s, MI355X dense FP64 matrix
+FP32_PEAK = 157.3  # TFLOP/s, MI355X dense FP32 matrix
 HBM_PEAK = 8000.0  # GB/s
 
 WORKLOADS = {
@@ -32,6 +33,7 @@ WORKLOADS = {
     "grouped": "FP64 grouped batch: 7x5 uneven tiles of M=N=K=20000 + 256 GEMMs with log-uniform M,N,K in "
                "[64,4096], one autotuned plan replayed as a HIP graph",
     "cannon2x2": "FP64 2D Cannon 2x2x1 virtual grid, M=N=K=2048, 1024^2 tiles, alpha=1, beta=0",
+    "sgemm": "FP32 SGEMM NN M=N=K=16384, alpha=1, beta=0 (FP32 matrix cores)",
 }
 
 
@@ -160,6 +162,19 @@ def setup(workload, dev):
             dgamd.cannon25d_t("N", "N", M, N, K, 1.0, bufs[0], bufs[1], 0.0, Cp, p, 1, dgamd.DG_SCHED_REFERENCE)
 
         return step, (M, N, K), {"workload": WORKLOADS[workload], "M": M, "N": N, "K": K, "grid": "2x2x1"}, "cannon2048"
+    if workload == "sgemm":
+        n = 16384
+        f32 = dict(dtype=torch.float32, device=dev)
+        A = torch.empty(n * n, **f32)
+        B = torch.empty(n * n, **f32)
+        C = torch.empty(n * n, **f32)
+        dgamd.generate_t(1, 0, A)
+        dgamd.generate_t(2, 0, B)
+
+        def step():
+            dgamd.sgemm_t("N", "N", n, n, n, 1.0, A, n, B, n, 0.0, C, n)
+
+        return step, (n, n, n), {"workload": WORKLOADS[workload], "M": n, "N": n, "K": n}, "sgemm16384"
     n = 16384
     A = torch.empty(n * n, **f64)
     B = torch.empty(n * n, **f64)
@@ -272,9 +287,10 @@ def grouped_ozaki_leg(step, reps=3):
             "max_rel_err_vs_fp64": float("%.3e" % err)}
 
 
-def cpu_baseline(budget_s):
-    """Host BLAS dgemm (the oracle's numpy/OpenBLAS) on 4096^3 products,
-    repeated for about `budget_s` seconds; TFLOP/s = 2*4096^3*reps / time."""
+def cpu_baseline(budget_s, single=False):
+    """Host BLAS dgemm (the oracle's numpy/OpenBLAS; sgemm with `single`) on
+    4096^3 products, repeated for about `budget_s` seconds;
+    TFLOP/s = 2*4096^3*reps / time."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import numpy as np
     import oracle  # noqa: F401  (the checker's host dgemm is numpy's BLAS)
@@ -290,9 +306,10 @@ def cpu_baseline(budget_s):
         pass
     n = 4096
     rng = np.random.default_rng(1)
-    a = rng.uniform(-1, 1, (n, n))
-    b = rng.uniform(-1, 1, (n, n))
-    c = np.empty((n, n))
+    dt = np.float32 if single else np.float64
+    a = rng.uniform(-1, 1, (n, n)).astype(dt)
+    b = rng.uniform(-1, 1, (n, n)).astype(dt)
+    c = np.empty((n, n), dtype=dt)
     np.matmul(a, b, out=c)  # warm-up
     reps, t0 = 0, time.perf_counter()
     while True:
@@ -303,8 +320,8 @@ def cpu_baseline(budget_s):
             break
     tf = 2.0 * n ** 3 * reps / el / 1e12
     return {"value": round(tf, 4), "unit": "TFLOP/s", "cores": threads or os.cpu_count(), "kind": "port",
-            "sample": "%d host-BLAS (numpy OpenBLAS) dgemm calls of 4096x4096x4096 in %.1f s; "
-                      "the 16384^3 step is 64 such products" % (reps, el)}
+            "sample": "%d host-BLAS (numpy OpenBLAS) %s calls of 4096x4096x4096 in %.1f s; "
+                      "the 16384^3 step is 64 such products" % (reps, "sgemm" if single else "dgemm", el)}
 
 
 def main():
@@ -322,6 +339,8 @@ def main():
         raise SystemExit("no gfx950 device: " + dgamd.last_error())
     step, (M, N, K), cfg, tag = setup(args.workload, dev)
     flops = 2.0 * M * N * K
+    fp32 = args.workload == "sgemm"
+    peak = FP32_PEAK if fp32 else FP64_PEAK
 
     for _ in range(args.warmup):
         step()
@@ -353,11 +372,11 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32" if fp32 else "f64",
         "data": "synthetic: SplitMix64-seeded uniform[-1,1) (seeds 1,2,3), generated and packed on the GPU",
         "config": cfg,
-        "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP64_PEAK, "unit": "TFLOP/s",
-                     "frac": round(achieved / FP64_PEAK, 4), "traffic": load_traffic(tag)},
+        "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
+                     "frac": round(achieved / peak, 4), "traffic": load_traffic(tag)},
     }
     if args.workload == "grouped":
         out["ozaki_form"] = grouped_ozaki_leg(step)
@@ -369,7 +388,7 @@ def main():
         torch.cuda.empty_cache()
         out["ozaki"] = ozaki_leg(dev, int8_s, bf16_s, crt_m)
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(args.cpu_seconds, single=fp32)
     print(json.dumps(out), flush=True)
 
 

@@ -9,7 +9,10 @@
 #include <cstring>
 #include <vector>
 
+#include <hip/hip_runtime.h>
+
 #include "25DCannon.hpp"
+#include "cmatrix.h"
 #include "2DBlockSequentialDecomposition.hpp"
 #include "gridLayout.hpp"
 #include "pblasDecomposition.hpp"
@@ -154,6 +157,33 @@ void test_validation_and_errors() {
     report("Dtest_equality, error codes, helpers", ok, 0.0);
 }
 
+// MatrixInit on the GPU (FP64 and FP32) equals the host generation bit for bit.
+void test_generation() {
+    const long long n = 10007;
+    std::vector<double> hd(n), gd(n);
+    std::vector<float> hf(n), gf(n);
+    setMatrixSeed(99);
+    MatrixInit(hd.data(), n, -1);
+    setMatrixSeed(99);
+    MatrixInit(hf.data(), n, -1);
+    double* dd = nullptr;
+    float* df = nullptr;
+    bool ok = hipMalloc(&dd, n * sizeof(double)) == hipSuccess && hipMalloc(&df, n * sizeof(float)) == hipSuccess;
+    if (ok) {
+        setMatrixSeed(99);
+        MatrixInit(dd, n, 0);
+        setMatrixSeed(99);
+        MatrixInit(df, n, 0);
+        ok = hipMemcpy(gd.data(), dd, n * sizeof(double), hipMemcpyDeviceToHost) == hipSuccess &&
+             hipMemcpy(gf.data(), df, n * sizeof(float), hipMemcpyDeviceToHost) == hipSuccess;
+    }
+    ok = ok && memcmp(hd.data(), gd.data(), n * sizeof(double)) == 0 && memcmp(hf.data(), gf.data(), n * sizeof(float)) == 0;
+    for (long long i = 0; ok && i < n; ++i) ok = hf[i] == (float)hd[i];
+    if (dd) (void)hipFree(dd);
+    if (df) (void)hipFree(df);
+    report("MatrixInit<double/float> GPU == host", ok, 0.0);
+}
+
 }  // namespace
 
 int main() {
@@ -166,6 +196,7 @@ int main() {
     test_block_sequential();
     test_grid_layout();
     test_validation_and_errors();
+    test_generation();
     printf("%s: %d failure(s)\n", failures ? "FAILED" : "PASSED", failures);
     return failures ? 1 : 0;
 }

@@ -1,0 +1,407 @@
+// FP32 GEMM on the CDNA4 matrix cores (v_mfma_f32_16x16x4_f32).
+//
+// The FP64 engine's direct-to-LDS pipeline at the same byte rate: a k-step
+// is 16 floats (64 bytes per row), so a 128x128 tile moves exactly the bytes
+// of an FP64 k-step of 8 and does twice the flops, which the matrix cores
+// deliver at twice the FP64 rate.
+//   * 128x128 tiles, 4 waves of 64x64 (4x4 MFMA tiles of 16x16), two
+//     workgroups per CU, a 4-stage LDS ring of 16 KB stages filled by
+//     global_load_lds_dwordx4 (no register staging).
+//   * k-contiguous operand (A for opA = T, B for opB = N): LDS [row][k],
+//     64-byte rows, 16-byte chunk c of row r at position c ^ ((r >> 1) & 3);
+//     a lane reads its four k with one 128-bit read.
+//   * row-contiguous operand (A for opA = N, B for opB = T): LDS [k][row]
+//     without padding, two k-rows per 1 KB load; row chunk c (4 rows) of
+//     k-row kk sits at position c ^ (4 * ((kk >> 2) & 3)) of its k-row, so
+//     the 32-bit reads of the four k-rows one MFMA sub-step needs fall in 64
+//     distinct banks.  The permutation is applied on the global side (each
+//     lane fetches the chunk that belongs at its LDS slot).
+// Lane l of an MFMA supplies k = 4*(l/16) + s at sub-step s (s = 0..3) for
+// both operands, so each product term pairs the right A and B elements
+// whatever order the matrix core sums in.  Result layout of
+// v_mfma_f32_16x16x4_f32: lane l, register v holds D(4*(l/16) + v, l%16),
+// so each lane stores four consecutive rows of one C column (16 bytes).
+// The barrier runs one step ahead, as in the FP64 kernel: at the end of step
+// k each wave waits for its own loads of step k+2, one barrier publishes
+// them and frees the stage of step k, which then receives step k+4.
+//
+// Edges: rows past the end of a k-contiguous operand read its last row;
+// row chunks past the end of a row-contiguous operand read the last 16-byte
+// aligned chunk, which holds a valid element and so cannot cross a page;
+// the products of those rows land only in unstored rows / columns of C.
+// k past K reads a zero page (K must be a multiple of 4 when an operand is
+// k-contiguous, so a 16-byte chunk never straddles K), and an odd step count
+// gets one zero step.  Operands that are not 16-byte aligned, leading
+// dimensions that are not multiples of 4, or K % 4 != 0 with a k-contiguous
+// operand run sgemm_generic_kernel (LDS-tiled FMA) instead.
+#include "dg_internal.hpp"
+
+namespace dg {
+
+namespace {
+
+typedef float fv4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(3))) void lds_void_t;
+typedef __attribute__((address_space(1))) const void gvoid_t;
+
+namespace sk {
+constexpr int BM = 128, BN = 128, BK = 16, STG = 4;
+constexpr int PART = 128 * BK * 4;  // 8 KB per operand per stage
+constexpr int SSZ = 2 * PART;
+constexpr size_t LDS_BYTES = (size_t)STG * SSZ;  // 64 KB
+constexpr int GROUP_M = 8;
+
+__host__ __device__ constexpr int swz(int r) { return (r >> 1) & 3; }
+
+template <int N>
+__device__ __forceinline__ void wait_vmcnt() {
+    static_assert(N >= 0 && N < 64, "vmcnt field");
+    // gfx9 s_waitcnt: vmcnt in bits 3:0 and 15:14; expcnt and lgkmcnt left at their maxima
+    __builtin_amdgcn_s_waitcnt((N & 0xF) | ((N >> 4) << 14) | (0x7 << 4) | (0xF << 8));
+}
+
+// Zeros for k past K and for the padding step (never written): 16 bytes per lane.
+__device__ __align__(256) float g_sgemm_zero[256];
+
+struct Cursor {
+    const float* p;
+    int64_t second, step;  // offset of the wave's second load, advance per k-step (floats)
+};
+
+// Cursor of this lane at k = 0.  `r0` is the tile's first row of the
+// operand, `rows` its row count.
+template <bool KC, bool EDGE>
+__device__ __forceinline__ Cursor make_cursor(const float* base, int64_t ld, int64_t r0, int64_t rows, int wave,
+                                              int lane) {
+    if constexpr (KC) {
+        // rows 32w .. 32w+31 in two loads of 16 rows; lane -> row lane/4, slot lane%4
+        const int row = lane >> 2;
+        const int chunk = (lane & 3) ^ swz(row);
+        int64_t ra = r0 + 32 * wave + row, rb = ra + 16;
+        if (EDGE) {
+            ra = min(ra, rows - 1);
+            rb = min(rb, rows - 1);
+        }
+        return Cursor{base + ra * ld + 4 * chunk, (rb - ra) * ld, BK};
+    } else {
+        // k-rows 4w + h (first load) and 4w + 2 + h (second), h = lane / 32;
+        // slot lane % 32 of the k-row holds row chunk (lane % 32) ^ 4w
+        const int h = lane >> 5;
+        const int c = (lane & 31) ^ (4 * wave);
+        int64_t r = r0 + 4 * c;
+        if (EDGE) r = min(r, (rows - 1) & ~(int64_t)3);
+        return Cursor{base + r + (int64_t)(4 * wave + h) * ld, 2 * ld, BK * ld};
+    }
+}
+
+__device__ __forceinline__ Cursor zero_cursor(int lane) { return Cursor{g_sgemm_zero + 4 * lane, 0, 0}; }
+
+// One k-step of one operand into `dst`; with tail != 0 only the first `tail`
+// k of the step are real and the lanes past them load zeros.
+template <bool KC>
+__device__ __forceinline__ void issue_operand(char* dst, Cursor& c, int wave, int lane, int tail) {
+    char* d0 = KC ? dst + (32 * wave) * 64 : dst + (2 * wave) * 1024;
+    char* d1 = KC ? d0 + 16 * 64 : d0 + 1024;
+    const float* p0 = c.p;
+    const float* p1 = c.p + c.second;
+    if (tail) {
+        const float* z = g_sgemm_zero + 4 * lane;
+        if constexpr (KC) {
+            const int k4 = 4 * ((lane & 3) ^ swz(lane >> 2));
+            if (k4 >= tail) { p0 = z; p1 = z; }
+        } else {
+            const int h = lane >> 5;
+            if (4 * wave + h >= tail) p0 = z;
+            if (4 * wave + 2 + h >= tail) p1 = z;
+        }
+    }
+    __builtin_amdgcn_global_load_lds((gvoid_t*)p0, (lds_void_t*)d0, 16, 0, 0);
+    __builtin_amdgcn_global_load_lds((gvoid_t*)p1, (lds_void_t*)d1, 16, 0, 0);
+    c.p += c.step;
+}
+
+// f[i][s] = X(row w0 + 16i + lr, k = 4 lg + s) of one stage.
+template <bool KC>
+__device__ __forceinline__ void read_operand(float (&f)[4][4], const char* part, int w0, int lr, int lg) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        if constexpr (KC) {
+            const fv4 v = *reinterpret_cast<const fv4*>(part + (w0 + 16 * i + lr) * 64 + 16 * (lg ^ swz(lr)));
+#pragma unroll
+            for (int s = 0; s < 4; ++s) f[i][s] = v[s];
+        } else {
+            const int row = w0 + 16 * i + lr;
+            const int slot = (row >> 2) ^ (4 * lg);
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                const int kk = 4 * lg + s;
+                f[i][s] = *reinterpret_cast<const float*>(part + ((kk >> 1) * 64 + (kk & 1) * 32 + slot) * 16 +
+                                                          (row & 3) * 4);
+            }
+        }
+    }
+}
+}  // namespace sk
+
+struct SgemmArgs {
+    const float* a;
+    const float* b;
+    float* c;
+    int64_t lda, ldb, ldc;
+    int64_t m, n, k;
+    float alpha, beta;
+    int tiles_m, tiles_n;
+    int cvec;  // C 16-byte aligned with ldc % 4 == 0: vector stores
+};
+
+__device__ __forceinline__ int xcd_remap(int bid, int total) {
+    constexpr int X = 8;
+    const int q = total / X, r = total % X;
+    const int x = bid % X, slot = bid / X;
+    return x * q + min(x, r) + slot;
+}
+
+// EDGE = false: m, n multiples of 128, K a multiple of 16 with an even step count.
+template <bool AK, bool BKC, bool EDGE>
+__global__ __launch_bounds__(256, 2) void sgemm_dma_kernel(SgemmArgs P) {
+    using namespace sk;
+    extern __shared__ __attribute__((aligned(1024))) char lds[];
+    const int total = P.tiles_m * P.tiles_n;
+    const int tile = xcd_remap(blockIdx.x, total);
+    const int per_group = GROUP_M * P.tiles_n;
+    const int group = tile / per_group;
+    const int first_m = group * GROUP_M;
+    const int gsize = min(P.tiles_m - first_m, GROUP_M);
+    const int in_group = tile - group * per_group;
+    const int64_t m0 = (int64_t)(first_m + in_group % gsize) * BM;
+    const int64_t n0 = (int64_t)(in_group / gsize) * BN;
+
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
+    const int lr = lane & 15, lg = lane >> 4;
+
+    int64_t left = (P.k + BK - 1) / BK;
+    const int tail = (int)(P.k % BK);
+    Cursor ca = make_cursor<AK, EDGE>(P.a, P.lda, m0, P.m, wave, lane);
+    Cursor cb = make_cursor<BKC, EDGE>(P.b, P.ldb, n0, P.n, wave, lane);
+    int ist = 0;
+    auto issue_next = [&]() {
+        char* st = lds + ist * SSZ;
+        const int t = (EDGE && left == 1) ? tail : 0;
+        issue_operand<AK>(st, ca, wave, lane, t);
+        issue_operand<BKC>(st + PART, cb, wave, lane, t);
+        ist = (ist + 1) & (STG - 1);
+        if (--left == 0) {
+            left = INT64_MAX;
+            ca = zero_cursor(lane);
+            cb = zero_cursor(lane);
+        }
+    };
+    int64_t nk = (P.k + BK - 1) / BK;
+    nk += nk & 1;
+
+    fv4 acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = fv4{0.f, 0.f, 0.f, 0.f};
+
+    for (int64_t s = 0; s < STG && s < nk; ++s) issue_next();
+    // steps 0 and 1 published (4 loads per wave and step)
+    if (nk >= 4) wait_vmcnt<8>(); else if (nk == 3) wait_vmcnt<4>(); else wait_vmcnt<0>();
+    __builtin_amdgcn_s_barrier();
+    float fa0[4][4], fb0[4][4], fa1[4][4], fb1[4][4];
+    read_operand<AK>(fa0, lds, wm, lr, lg);
+    read_operand<BKC>(fb0, lds + PART, wn, lr, lg);
+    // LDS read instructions per step and the MFMAs interleaved after each
+    constexpr int NR = (AK ? 4 : 16) + (BKC ? 4 : 16);
+    constexpr int M1 = NR <= 8 ? 4 : (NR <= 20 ? 2 : 1);
+    constexpr int MREST = 64 - 2 - NR * M1;
+    int64_t kt = 0;
+#define SGEMM_STEP(CA, CB, NA, NB)                                                                  \
+    {                                                                                               \
+        const char* nst = lds + (int)((kt + 1) & (STG - 1)) * SSZ; /* harmless past the last step */ \
+        read_operand<AK>(NA, nst, wm, lr, lg);                                                      \
+        read_operand<BKC>(NB, nst + PART, wn, lr, lg);                                              \
+        _Pragma("unroll") for (int s_ = 0; s_ < 4; ++s_)                                            \
+        _Pragma("unroll") for (int i = 0; i < 4; ++i)                                               \
+        _Pragma("unroll") for (int j = 0; j < 4; ++j)                                               \
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(CA[i][s_], CB[j][s_], acc[i][j], 0, 0, 0); \
+        __builtin_amdgcn_sched_group_barrier(0x008, 2, 0);                                          \
+        _Pragma("unroll") for (int g_ = 0; g_ < NR; ++g_) {                                         \
+            __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);                                      \
+            __builtin_amdgcn_sched_group_barrier(0x008, M1, 0);                                     \
+        }                                                                                           \
+        __builtin_amdgcn_sched_group_barrier(0x008, MREST, 0);                                      \
+        if (kt + 2 < nk) {                                                                          \
+            if (kt + 3 < nk) wait_vmcnt<4>(); else wait_vmcnt<0>();                                 \
+            __builtin_amdgcn_s_barrier();                                                           \
+            if (kt + STG < nk) issue_next();                                                        \
+        }                                                                                           \
+        ++kt;                                                                                       \
+    }
+    while (kt < nk) {
+        SGEMM_STEP(fa0, fb0, fa1, fb1);
+        SGEMM_STEP(fa1, fb1, fa0, fb0);
+    }
+#undef SGEMM_STEP
+
+    // acc[i][j][v] = sum for C(m0 + wm + 16i + 4lg + v, n0 + wn + 16j + lr)
+    const float alpha = P.alpha, beta = P.beta;
+    const bool interior = !EDGE || (m0 + BM <= P.m && n0 + BN <= P.n);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int64_t m = m0 + wm + 16 * i + 4 * lg;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int64_t n = n0 + wn + 16 * j + lr;
+            if (interior && P.cvec) {
+                __attribute__((address_space(1))) fv4* cp =
+                    (__attribute__((address_space(1))) fv4*)(P.c + m + n * P.ldc);
+                fv4 val = alpha * acc[i][j];
+                if (beta != 0.f) {
+                    const fv4 old = *cp;
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) val[v] = fmaf(beta, old[v], val[v]);
+                }
+                *cp = val;
+            } else {
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    if (m + v >= P.m || n >= P.n) continue;
+                    __attribute__((address_space(1))) float* cp =
+                        (__attribute__((address_space(1))) float*)(P.c + (m + v) + n * P.ldc);
+                    float val = alpha * acc[i][j][v];
+                    if (beta != 0.f) val = fmaf(beta, *cp, val);
+                    *cp = val;
+                }
+            }
+        }
+    }
+}
+
+// Any layout: 64x64 tiles, k-steps of 16 staged through LDS with guarded
+// loads (zeros past the edges), each thread a 4x4 block of C in FP32 FMAs.
+__global__ __launch_bounds__(256) void sgemm_generic_kernel(int ta, int tb, SgemmArgs P) {
+    __shared__ float As[16][64 + 1];
+    __shared__ float Bs[16][64 + 1];
+    const int t = threadIdx.x;
+    const int tx = t & 15, ty = t >> 4;
+    const int64_t m0 = (int64_t)blockIdx.x * 64, n0 = (int64_t)blockIdx.y * 64;
+    float acc[4][4] = {};
+    for (int64_t k0 = 0; k0 < P.k; k0 += 16) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int e = t + 256 * q;
+            // A: element (row, kk) of op(A); stored (row, kk) at row + kk*lda (N) or kk + row*lda (T)
+            int r, kk;
+            if (!ta) { r = e & 63; kk = e >> 6; } else { kk = e & 15; r = e >> 4; }
+            int64_t gr = m0 + r, gk = k0 + kk;
+            As[kk][r] = (gr < P.m && gk < P.k) ? (ta ? P.a[gk + gr * P.lda] : P.a[gr + gk * P.lda]) : 0.f;
+            // B: element (kk, col) of op(B); stored at kk + col*ldb (N) or col + kk*ldb (T)
+            if (!tb) { kk = e & 15; r = e >> 4; } else { r = e & 63; kk = e >> 6; }
+            gr = n0 + r; gk = k0 + kk;
+            Bs[kk][r] = (gr < P.n && gk < P.k) ? (tb ? P.b[gr + gk * P.ldb] : P.b[gk + gr * P.ldb]) : 0.f;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < 16; ++kk) {
+            float a[4], b[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) a[i] = As[kk][tx + 16 * i];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) b[j] = Bs[kk][ty + 16 * j];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) acc[i][j] = fmaf(a[i], b[j], acc[i][j]);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int64_t m = m0 + tx + 16 * i;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int64_t n = n0 + ty + 16 * j;
+            if (m >= P.m || n >= P.n) continue;
+            float* cp = P.c + m + n * P.ldc;
+            float val = P.alpha * acc[i][j];
+            if (P.beta != 0.f) val = fmaf(P.beta, *cp, val);
+            *cp = val;
+        }
+    }
+}
+
+__global__ void sgemm_scale_kernel(int64_t m, int64_t n, float beta, float* __restrict__ C, int64_t ldc) {
+    const int64_t total = m * n;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = e % m, j = e / m;
+        float* p = C + i + j * ldc;
+        *p = beta == 0.f ? 0.f : beta * *p;
+    }
+}
+
+template <bool AK, bool BKC>
+void launch_sgemm_dma(bool edge, const SgemmArgs& a, int tiles, hipStream_t st) {
+    if (edge)
+        hipLaunchKernelGGL((sgemm_dma_kernel<AK, BKC, true>), dim3(tiles), dim3(256), sk::LDS_BYTES, st, a);
+    else
+        hipLaunchKernelGGL((sgemm_dma_kernel<AK, BKC, false>), dim3(tiles), dim3(256), sk::LDS_BYTES, st, a);
+}
+
+bool aligned16(const void* p) { return ((uintptr_t)p & 15) == 0; }
+
+}  // namespace
+
+}  // namespace dg
+
+extern "C" int dg_sgemm(char opA, char opB, int64_t m, int64_t n, int64_t k, float alpha, const float* A,
+                        int64_t lda, const float* B, int64_t ldb, float beta, float* C, int64_t ldc,
+                        void* stream) {
+    using namespace dg;
+    const char oa = norm_op(opA), ob = norm_op(opB);
+    if (!oa) return fail(DG_ERR_INVALID, "dg_sgemm: invalid opA '%c'", opA);
+    if (!ob) return fail(DG_ERR_INVALID, "dg_sgemm: invalid opB '%c'", opB);
+    if (m < 0 || n < 0 || k < 0) return fail(DG_ERR_INVALID, "dg_sgemm: negative dimension");
+    const int64_t rowsA = oa == 'N' ? m : k;
+    const int64_t rowsB = ob == 'N' ? k : n;
+    if (lda < (rowsA > 1 ? rowsA : 1)) return fail(DG_ERR_INVALID, "dg_sgemm: lda %lld < %lld", (long long)lda, (long long)rowsA);
+    if (ldb < (rowsB > 1 ? rowsB : 1)) return fail(DG_ERR_INVALID, "dg_sgemm: ldb %lld < %lld", (long long)ldb, (long long)rowsB);
+    if (ldc < (m > 1 ? m : 1)) return fail(DG_ERR_INVALID, "dg_sgemm: ldc %lld < %lld", (long long)ldc, (long long)m);
+    if (m == 0 || n == 0) return DG_OK;
+    int rc = require_device();
+    if (rc != DG_OK) return rc;
+    hipStream_t st = as_stream(stream);
+    if (!C) return fail(DG_ERR_INVALID, "dg_sgemm: null C");
+    if (k == 0 || alpha == 0.f) {
+        int64_t blocks = (m * n + 255) / 256;
+        if (blocks > 8192) blocks = 8192;
+        hipLaunchKernelGGL(sgemm_scale_kernel, dim3((unsigned)blocks), dim3(256), 0, st, m, n, beta, C, ldc);
+        DG_LAUNCH_CHECK();
+        return DG_OK;
+    }
+    if (!A || !B) return fail(DG_ERR_INVALID, "dg_sgemm: null matrix pointer");
+    const bool ak = oa == 'T', bkc = ob == 'N';  // operands stored contiguous along k
+    const int64_t tm = (m + sk::BM - 1) / sk::BM, tn = (n + sk::BN - 1) / sk::BN;
+    if (tm * tn > INT32_MAX || (m + 63) / 64 > INT32_MAX || (n + 63) / 64 > 65535)
+        return fail(DG_ERR_INVALID, "dg_sgemm: %lld x %lld output is too large", (long long)m, (long long)n);
+    SgemmArgs a{A, B, C, lda, ldb, ldc, m, n, k, alpha, beta, (int)tm, (int)tn, 0};
+    a.cvec = aligned16(C) && ldc % 4 == 0;
+    const bool dma = aligned16(A) && aligned16(B) && lda % 4 == 0 && ldb % 4 == 0 && (!(ak || bkc) || k % 4 == 0);
+    if (dma) {
+        const int64_t steps = (k + sk::BK - 1) / sk::BK;
+        const bool edge = m % sk::BM || n % sk::BN || k % sk::BK || (steps & 1);
+        const int tiles = (int)(tm * tn);
+        if (ak && bkc) launch_sgemm_dma<true, true>(edge, a, tiles, st);
+        else if (ak) launch_sgemm_dma<true, false>(edge, a, tiles, st);
+        else if (bkc) launch_sgemm_dma<false, true>(edge, a, tiles, st);
+        else launch_sgemm_dma<false, false>(edge, a, tiles, st);
+    } else {
+        hipLaunchKernelGGL(sgemm_generic_kernel, dim3((unsigned)((m + 63) / 64), (unsigned)((n + 63) / 64)), dim3(256),
+                           0, st, (int)ak, (int)(ob == 'T'), a);
+    }
+    DG_LAUNCH_CHECK();
+    return DG_OK;
+}

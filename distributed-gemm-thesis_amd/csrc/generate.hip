@@ -18,41 +18,58 @@ __device__ __forceinline__ uint64_t splitmix64_at(uint64_t seed, uint64_t i) {
 // Second, independent stream for the exponent of the wide distribution.
 constexpr uint64_t kExpSeedXor = 0xD1B54A32D192ED03ull;
 
-__global__ void generate_kernel(uint64_t seed, int dist, double* __restrict__ out, int64_t n) {
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        const uint64_t z = splitmix64_at(seed, (uint64_t)i);
-        const double u01 = (double)(z >> 11) * 0x1.0p-53;  // exact: 53-bit integer times 2^-53
-        double v;
-        if (dist == DG_DIST_UNIFORM_01) {
-            v = u01;
-        } else {
-            v = 2.0 * u01 - 1.0;  // exact in [-1, 1)
-            if (dist == DG_DIST_WIDE) {
-                const uint64_t ze = splitmix64_at(seed ^ kExpSeedXor, (uint64_t)i);
-                const int e = (int)((ze >> 32) % 41ull) - 20;
-                v = ldexp(v, e);  // exact power-of-two scaling
-            }
-        }
-        out[i] = v;
+__device__ __forceinline__ double generated_value(uint64_t seed, int dist, int64_t i) {
+    const uint64_t z = splitmix64_at(seed, (uint64_t)i);
+    const double u01 = (double)(z >> 11) * 0x1.0p-53;  // exact: 53-bit integer times 2^-53
+    if (dist == DG_DIST_UNIFORM_01) return u01;
+    double v = 2.0 * u01 - 1.0;  // exact in [-1, 1)
+    if (dist == DG_DIST_WIDE) {
+        const uint64_t ze = splitmix64_at(seed ^ kExpSeedXor, (uint64_t)i);
+        const int e = (int)((ze >> 32) % 41ull) - 20;
+        v = ldexp(v, e);  // exact power-of-two scaling
     }
+    return v;
+}
+
+__global__ void generate_kernel(uint64_t seed, int dist, double* __restrict__ out, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = generated_value(seed, dist, i);
+}
+
+// FP32 form: the same FP64 values rounded to nearest (ties to even).
+__global__ void generate_f32_kernel(uint64_t seed, int dist, float* __restrict__ out, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = (float)generated_value(seed, dist, i);
 }
 
 }  // namespace
 
 }  // namespace dg
 
-extern "C" int dg_generate(uint64_t seed, int dist, double* ptr, int64_t n, void* stream) {
+template <typename T>
+static int dg_generate_common(uint64_t seed, int dist, T* ptr, int64_t n, void* stream, const char* name) {
     using namespace dg;
-    if (n < 0) return fail(DG_ERR_INVALID, "dg_generate: negative size");
+    if (n < 0) return fail(DG_ERR_INVALID, "%s: negative size", name);
     if (dist != DG_DIST_UNIFORM_PM1 && dist != DG_DIST_WIDE && dist != DG_DIST_UNIFORM_01)
-        return fail(DG_ERR_INVALID, "dg_generate: unknown distribution %d", dist);
+        return fail(DG_ERR_INVALID, "%s: unknown distribution %d", name, dist);
     if (n == 0) return DG_OK;
-    if (!ptr) return fail(DG_ERR_INVALID, "dg_generate: null pointer");
+    if (!ptr) return fail(DG_ERR_INVALID, "%s: null pointer", name);
     int rc = require_device();
     if (rc != DG_OK) return rc;
     int64_t blocks = (n + 255) / 256;
     if (blocks > 8192) blocks = 8192;
-    hipLaunchKernelGGL(generate_kernel, dim3((unsigned)blocks), dim3(256), 0, as_stream(stream), seed, dist, ptr, n);
+    if constexpr (sizeof(T) == 8)
+        hipLaunchKernelGGL(generate_kernel, dim3((unsigned)blocks), dim3(256), 0, as_stream(stream), seed, dist, ptr, n);
+    else
+        hipLaunchKernelGGL(generate_f32_kernel, dim3((unsigned)blocks), dim3(256), 0, as_stream(stream), seed, dist, ptr, n);
     DG_LAUNCH_CHECK();
     return DG_OK;
+}
+
+extern "C" int dg_generate(uint64_t seed, int dist, double* ptr, int64_t n, void* stream) {
+    return dg_generate_common(seed, dist, ptr, n, stream, "dg_generate");
+}
+
+extern "C" int dg_generate_f32(uint64_t seed, int dist, float* ptr, int64_t n, void* stream) {
+    return dg_generate_common(seed, dist, ptr, n, stream, "dg_generate_f32");
 }

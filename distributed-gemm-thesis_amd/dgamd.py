@@ -65,6 +65,8 @@ SIGNATURES = {
     "dg_version": (_int, []),
     "dg_device_ok": (_int, []),
     "dg_dgemm": (_int, [_chr, _chr, _i64, _i64, _i64, _dbl, _ptr, _i64, _ptr, _i64, _dbl, _ptr, _i64, _ptr]),
+    "dg_sgemm": (_int, [_chr, _chr, _i64, _i64, _i64, ctypes.c_float, _ptr, _i64, _ptr, _i64, ctypes.c_float, _ptr,
+                        _i64, _ptr]),
     "dg_block_splits": (_int, [_i64, _int, _i64p]),
     "dg_virtual_grid": (_int, [_int, ctypes.POINTER(_int), ctypes.POINTER(_int)]),
     "dg_pack_block2d": (_int, [_ptr, _i64, _i64, _i64, _i64p, _int, _i64p, _int, _ptr, _ptr]),
@@ -82,6 +84,7 @@ SIGNATURES = {
                         _int, _ptr]),
     "dg_layer_reduce": (_int, [_ptr, _ptr, _i64, _int, _ptr]),
     "dg_generate": (_int, [_u64, _int, _ptr, _i64, _ptr]),
+    "dg_generate_f32": (_int, [_u64, _int, _ptr, _i64, _ptr]),
     "dg_grouped_dgemm": (_int, [_descp, _int, _ptr]),
     "dg_grouped_plan_create": (_int, [_descp, _int, _int, ctypes.POINTER(ctypes.c_void_p)]),
     "dg_grouped_plan_create_form": (_int, [_descp, _int, _int, _int, _int, ctypes.POINTER(ctypes.c_void_p)]),
@@ -191,8 +194,17 @@ def dgemm_t(opA, opB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc):
                                       _dp(C), ldc, _stream()))
 
 
+def sgemm_t(opA, opB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc):
+    check("dg_sgemm", lib().dg_sgemm(_op(opA), _op(opB), m, n, k, alpha, _dp(A), lda, _dp(B), ldb, beta,
+                                      _dp(C), ldc, _stream()))
+
+
 def generate_t(seed, dist, t):
-    check("dg_generate", lib().dg_generate(seed, dist, _dp(t), t.numel(), _stream()))
+    import torch
+    if t.dtype == torch.float32:
+        check("dg_generate_f32", lib().dg_generate_f32(seed, dist, _dp(t), t.numel(), _stream()))
+    else:
+        check("dg_generate", lib().dg_generate(seed, dist, _dp(t), t.numel(), _stream()))
 
 
 def pack_block2d_t(G, ldg, rows, cols, row_off, col_off, packed):

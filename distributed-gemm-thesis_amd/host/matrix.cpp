@@ -219,23 +219,15 @@ void generateMatrixGPU(T* array, const long long size, const int deviceID) {
     int dist;
     take_seed(&seed, &dist);
     const bool dev = isDevicePointer(array);
-    double* target = nullptr;
-    if (dev && std::is_same<T, double>::value) {
-        target = reinterpret_cast<double*>(array);
+    T* target = array;
+    if (!dev) HIP_CHECK(hipMalloc(&target, sizeof(T) * (size_t)(size > 0 ? size : 1)));
+    if constexpr (std::is_same<T, double>::value) {
+        DG_CHECK(dg_generate(seed, dist, target, size, nullptr));
     } else {
-        HIP_CHECK(hipMalloc(&target, sizeof(double) * (size_t)(size > 0 ? size : 1)));
+        DG_CHECK(dg_generate_f32(seed, dist, target, size, nullptr));
     }
-    DG_CHECK(dg_generate(seed, dist, target, size, nullptr));
-    if ((void*)target != (void*)array) {
-        if (std::is_same<T, double>::value) {
-            HIP_CHECK(hipMemcpy(array, target, sizeof(double) * size, dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost));
-        } else {
-            std::vector<double> tmp((size_t)size);
-            HIP_CHECK(hipMemcpy(tmp.data(), target, sizeof(double) * size, hipMemcpyDeviceToHost));
-            std::vector<T> conv((size_t)size);
-            for (long long i = 0; i < size; ++i) conv[(size_t)i] = (T)tmp[(size_t)i];
-            HIP_CHECK(hipMemcpy(array, conv.data(), sizeof(T) * size, dev ? hipMemcpyHostToDevice : hipMemcpyHostToHost));
-        }
+    if (target != array) {
+        HIP_CHECK(hipMemcpy(array, target, sizeof(T) * size, hipMemcpyDeviceToHost));
         HIP_CHECK(hipFree(target));
     }
     HIP_CHECK(hipDeviceSynchronize());

@@ -52,6 +52,22 @@ int dg_dgemm(char opA, char opB, int64_t m, int64_t n, int64_t k,
              const double* B, int64_t ldb,
              double beta, double* C, int64_t ldc, void* stream);
 
+/* C = alpha*op(A)*op(B) + beta*C in FP32 on the FP32 matrix cores
+ * (v_mfma_f32_16x16x4_f32; FP32 products and sums), same conventions as
+ * dg_dgemm (column-major, 'N'/'T'/'C', beta == 0 does not read C).
+ * 16-byte aligned A and B with lda, ldb multiples of 4 (and K a multiple of
+ * 4 when an operand is stored contiguous along k) take the direct-to-LDS
+ * kernel; other layouts a slower LDS-tiled FMA kernel.
+ * Replaces: the FP32 instantiations of the reference's GEMM entry points,
+ *           slateGEMM<float> (src/Wrappers/Slate/slateWrapped.cpp:334-335)
+ *           and cosmaPreDistributedOptimalGemm<float>
+ *           (src/Wrappers/Cosma/cosmaWrapped.cpp:430-431), one rank's
+ *           local product. */
+int dg_sgemm(char opA, char opB, int64_t m, int64_t n, int64_t k,
+             float alpha, const float* A, int64_t lda,
+             const float* B, int64_t ldb,
+             float beta, float* C, int64_t ldc, void* stream);
+
 /* ------------------------------------------------------ 2D block layout -- */
 
 /* Split `n` into `parts` contiguous ranges, the last range absorbing the
@@ -291,6 +307,11 @@ int dg_grouped_plan_problem_form(const dg_grouped_plan* plan, int index, int* mo
  * Replaces: MatrixInit / generateMatrixGPU (cuRAND, clock seeded),
  *           src/Utilities/cmatrix.cpp:104-169, 239-297. */
 int dg_generate(uint64_t seed, int dist, double* ptr, int64_t n, void* stream);
+
+/* FP32 form of dg_generate: the same FP64 values rounded to nearest, ties to
+ * even.  Replaces: generateMatrixGPU<float> / MatrixInit<float>,
+ * src/Utilities/cmatrix.cpp:301-309. */
+int dg_generate_f32(uint64_t seed, int dist, float* ptr, int64_t n, void* stream);
 
 #ifdef __cplusplus
 }

@@ -221,6 +221,12 @@ def generate(seed, dist, n):
     return out
 
 
+def generate_f32(seed, dist, n):
+    """FP32 form of `generate` (generateMatrixGPU<float>, cmatrix.cpp:301-309):
+    the FP64 values rounded to nearest, ties to even (IEEE conversion)."""
+    return generate(seed, dist, n).astype(np.float32)
+
+
 # ---------------------------------------------------------------- Ozaki split DGEMM
 # Ozaki, Ogita, Oishi, Rump, "Error-free transformations of matrix
 # multiplication by using fast routines of matrix multiplication and its

@@ -1,0 +1,154 @@
+"""FP32 MFMA GEMM (dg_sgemm) and FP32 generation (dg_generate_f32).
+
+Oracle: the host BLAS dgemm in FP64 on the FP32 inputs (exact products,
+near-exact sums).  Inputs whose products and partial sums are small integers
+are exact in FP32 whatever the summation order, so those results must match
+bit for bit.  For random data the tolerance is the standard a-priori bound
+for any summation order in FP32 arithmetic (Higham, Accuracy and Stability
+of Numerical Algorithms, 2nd ed., eq. 3.13):
+    |fl(alpha*A*B + beta*C) - (alpha*A*B + beta*C)|
+        <= gamma_{k+2} (|alpha| |A| |B| + |beta| |C|),  gamma_n = n u / (1 - n u),
+u = 2^-24, taken in the Frobenius norm relative to the exact result.
+"""
+import numpy as np
+import pytest
+import torch
+
+import dgamd
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+U32 = 2.0 ** -24
+
+
+def colmajor(rows, cols, ld, rng, dev, integer=False, fill=None, offset=0):
+    """(ld x cols) column-major FP32 device buffer, `offset` floats into its
+    allocation (NaN before it), whose top rows x cols block is random;
+    returns (device view, host FP64 copy of the logical matrix)."""
+    buf = np.full((cols, ld), np.nan if fill is None else fill, dtype=np.float32)
+    if integer:
+        vals = rng.integers(-8, 9, size=(cols, rows)).astype(np.float32)
+    else:
+        vals = rng.uniform(-1.0, 1.0, size=(cols, rows)).astype(np.float32)
+    buf[:, :rows] = vals
+    flat = np.concatenate([np.full(offset, np.nan, dtype=np.float32), buf.reshape(-1)])
+    t = torch.from_numpy(flat).to(dev)[offset:]
+    return t, vals.T.astype(np.float64)
+
+
+def run(dev, opA, opB, m, n, k, alpha, beta, pad=(0, 0, 0), integer=False, seed=0, offset=(0, 0, 0)):
+    """Returns (GPU result as FP64, FP64 oracle result, error bound)."""
+    rng = np.random.default_rng(seed)
+    ra, ca = (m, k) if opA == "N" else (k, m)
+    rb, cb = (k, n) if opB == "N" else (n, k)
+    lda, ldb, ldc = max(ra, 1) + pad[0], max(rb, 1) + pad[1], max(m, 1) + pad[2]
+    A, a = colmajor(ra, ca, lda, rng, dev, integer, offset=offset[0])
+    B, b = colmajor(rb, cb, ldb, rng, dev, integer, offset=offset[1])
+    C, c = colmajor(m, n, ldc, rng, dev, integer, fill=7.0, offset=offset[2])
+    if beta == 0.0:
+        # beta == 0 must not read C: poison the logical block with NaN
+        C.view(n, ldc)[:, :m] = float("nan")
+        c = np.zeros_like(c)
+    dgamd.sgemm_t(opA, opB, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc)
+    torch.cuda.synchronize()
+    out = C.cpu().numpy().reshape(n, ldc)
+    got = out[:, :m].T.astype(np.float64)
+    ref = oracle.host_dgemm(opA, opB, alpha, a, b, beta, c)
+    absprod = oracle.host_dgemm(opA, opB, abs(alpha), np.abs(a), np.abs(b), abs(beta), np.abs(c))
+    g = (k + 2) * U32 / (1 - (k + 2) * U32)
+    bound = g * np.linalg.norm(absprod) / max(np.linalg.norm(ref), 1e-300)
+    if pad[2]:
+        assert np.all(out[:, m:] == 7.0), "rows of the C padding were written"
+    return got, ref, bound
+
+
+def test_sgemm_mfma_layout_exact(dev):
+    # one 16x16 output block, k = 4: checks the operand / result lane layout
+    got, ref, _ = run(dev, "N", "N", 16, 16, 4, 1.0, 0.0, integer=True)
+    assert np.array_equal(got, ref)
+
+
+@pytest.mark.parametrize("opA,opB", [("N", "N"), ("N", "T"), ("T", "N"), ("T", "T")])
+def test_sgemm_interior_exact_integer(dev, opA, opB):
+    # whole 128x128 tiles, an even number of whole k-steps: the lean kernel
+    got, ref, _ = run(dev, opA, opB, 256, 384, 128, 1.0, 2.0, integer=True, seed=1)
+    assert np.array_equal(got, ref)
+
+
+@pytest.mark.parametrize("opA,opB", [("N", "N"), ("N", "T"), ("T", "N"), ("T", "T")])
+@pytest.mark.parametrize("m,n,k", [(300, 200, 76), (129, 1, 4), (1, 131, 20), (130, 254, 16), (5, 3, 48)])
+def test_sgemm_edges_exact_integer(dev, opA, opB, m, n, k):
+    # partial tiles, a partial last k-step, odd step counts (zero step)
+    got, ref, _ = run(dev, opA, opB, m, n, k, 1.0, 0.0, integer=True, seed=m + n + k)
+    assert np.array_equal(got, ref)
+
+
+@pytest.mark.parametrize("opA,opB,k,pad,offset", [
+    ("N", "N", 77, (0, 0, 0), (0, 0, 0)),   # K % 4 != 0 with a k-contiguous B: generic kernel
+    ("N", "T", 77, (0, 0, 0), (0, 0, 0)),   # both row-contiguous: direct-to-LDS kernel, any K
+    ("T", "T", 45, (0, 0, 0), (0, 0, 0)),   # k-contiguous A, K odd: generic kernel
+    ("N", "N", 64, (1, 0, 0), (0, 0, 0)),   # lda not a multiple of 4: generic kernel
+    ("N", "N", 64, (0, 0, 0), (1, 0, 0)),   # A 4 bytes off a 16-byte boundary: generic kernel
+    ("T", "N", 64, (0, 0, 0), (0, 4, 0)),   # B 16 bytes in: direct-to-LDS kernel
+    ("N", "N", 64, (0, 0, 1), (0, 0, 1)),   # C unaligned with odd ldc: scalar stores
+])
+def test_sgemm_layouts_exact_integer(dev, opA, opB, k, pad, offset):
+    got, ref, _ = run(dev, opA, opB, 200, 136, k, 1.0, 2.0, pad=pad, integer=True, seed=k, offset=offset)
+    assert np.array_equal(got, ref)
+
+
+@pytest.mark.parametrize("opA,opB", [("N", "N"), ("N", "T"), ("T", "N"), ("T", "T")])
+def test_sgemm_random_within_bound(dev, opA, opB):
+    got, ref, bound = run(dev, opA, opB, 512, 384, 640, 1.5, 0.5, seed=2)
+    err = oracle.rel_frobenius(got, ref)
+    assert err <= bound, (err, bound)
+
+
+def test_sgemm_random_large_within_bound(dev):
+    got, ref, bound = run(dev, "N", "N", 2048, 1536, 4096, 1.0, 0.0, seed=4)
+    err = oracle.rel_frobenius(got, ref)
+    # the a-priori bound is loose; a reduced-precision internal format (TF32,
+    # BF16 splitting: errors 100x and more above FP32's) would still pass it,
+    # so the error is also held to the host BLAS sgemm's own on the same FP32
+    # inputs (one long accumulation chain per element runs ~3x the host's
+    # blocked sums here)
+    rng = np.random.default_rng(4)
+    a = rng.uniform(-1.0, 1.0, size=(4096, 2048)).astype(np.float32).T
+    b = rng.uniform(-1.0, 1.0, size=(1536, 4096)).astype(np.float32).T
+    host = (a @ b).astype(np.float64)
+    err_host = oracle.rel_frobenius(host, ref)
+    print("sgemm 2048x1536x4096 rel err %.3e, host sgemm %.3e, a-priori bound %.3e" % (err, err_host, bound))
+    assert err <= bound, (err, bound)
+    assert err <= 8 * err_host, (err, err_host)
+
+
+def test_sgemm_beta_zero_ignores_nan(dev):
+    got, ref, bound = run(dev, "N", "N", 256, 256, 256, 1.0, 0.0, seed=3)
+    assert np.all(np.isfinite(got))
+    assert oracle.rel_frobenius(got, ref) <= bound
+
+
+def test_sgemm_k_zero_and_alpha_zero_scale_c(dev):
+    got, ref, _ = run(dev, "N", "N", 70, 50, 0, 1.0, 2.0, integer=True, seed=5)
+    assert np.array_equal(got, ref)
+    got, ref, _ = run(dev, "T", "N", 70, 50, 33, 0.0, 0.0, integer=True, seed=6)
+    assert np.array_equal(got, ref)
+
+
+def test_sgemm_rejects_bad_arguments(dev):
+    A = torch.zeros(64, dtype=torch.float32, device=dev)
+    with pytest.raises(dgamd.DGError):
+        dgamd.sgemm_t("X", "N", 8, 8, 8, 1.0, A, 8, A, 8, 0.0, A, 8)
+    with pytest.raises(dgamd.DGError):
+        dgamd.sgemm_t("N", "N", 8, 8, 8, 1.0, A, 7, A, 8, 0.0, A, 8)
+
+
+@pytest.mark.parametrize("dist", [dgamd.DG_DIST_UNIFORM_PM1, dgamd.DG_DIST_WIDE, dgamd.DG_DIST_UNIFORM_01])
+def test_generate_f32_bit_exact(dev, dist):
+    n = 100003
+    t = torch.empty(n, dtype=torch.float32, device=dev)
+    dgamd.generate_t(777, dist, t)
+    got = t.cpu().numpy()
+    ref = oracle.generate_f32(777, dist, n)
+    assert np.array_equal(got.view(np.uint32), ref.view(np.uint32))

@@ -51,6 +51,37 @@ def test_generate_matches_exact_rationals(seed):
     assert np.all(u >= -1.0) and np.all(u < 1.0)
 
 
+def nearest_f32_brute(x):
+    """The FP32 value nearest to the Fraction x, ties to the even significand,
+    by search over the two FP32 neighbours of a truncated candidate."""
+    lo = np.float32(float(x))  # any FP32 value within a few ulps
+    cands = {np.float32(lo)}
+    for _ in range(3):
+        cands |= {np.nextafter(c, np.float32(np.inf)) for c in list(cands)}
+        cands |= {np.nextafter(c, np.float32(-np.inf)) for c in list(cands)}
+    best = min(abs(Fraction(float(c)) - x) for c in cands)
+    ties = [c for c in cands if abs(Fraction(float(c)) - x) == best]
+    if len(ties) == 1:
+        return ties[0]
+    return [c for c in ties if (int(np.array(c).view(np.uint32)) & 1) == 0][0]
+
+
+@pytest.mark.parametrize("dist", [0, 1, 2])
+def test_generate_f32_is_nearest_even(dist):
+    got = oracle.generate_f32(4242, dist, 300)
+    exact = oracle.generate(4242, dist, 300)
+    for g, x in zip(got, exact):
+        assert g.view(np.uint32) == nearest_f32_brute(Fraction(float(x))).view(np.uint32)
+
+
+def test_generate_f32_ties_known_answers():
+    # a halfway case rounds to the even neighbour, a value past it away
+    assert nearest_f32_brute(Fraction(1) + Fraction(1, 2 ** 24)) == np.float32(1.0)
+    assert nearest_f32_brute(Fraction(1) + Fraction(3, 2 ** 24)) == np.float32(1.0 + 2.0 ** -22)
+    assert np.float32(np.float64(1.0 + 2.0 ** -24)) == np.float32(1.0)
+    assert np.float32(np.float64(1.0 + 3 * 2.0 ** -24)) == np.float32(1.0 + 2.0 ** -22)
+
+
 def test_generate_golden_fixture():
     import json, os
     fx = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "generate.json")))
