@@ -5,7 +5,7 @@
 // of an FP64 k-step of 8 and does twice the flops, which the matrix cores
 // deliver at twice the FP64 rate.
 //   * 128x128 tiles, 4 waves of 64x64 (4x4 MFMA tiles of 16x16), two
-//     workgroups per CU, a 4-stage LDS ring of 16 KB stages filled by
+//     workgroups per CU, a 3-stage LDS ring of 16 KB stages filled by
 //     global_load_lds_dwordx4 (no register staging).
 //   * k-contiguous operand (A for opA = T, B for opB = N): LDS [row][k],
 //     64-byte rows, 16-byte chunk c of row r at position c ^ ((r >> 1) & 3);
@@ -21,9 +21,11 @@
 // whatever order the matrix core sums in.  Result layout of
 // v_mfma_f32_16x16x4_f32: lane l, register v holds D(4*(l/16) + v, l%16),
 // so each lane stores four consecutive rows of one C column (16 bytes).
-// The barrier runs one step ahead, as in the FP64 kernel: at the end of step
-// k each wave waits for its own loads of step k+2, one barrier publishes
-// them and frees the stage of step k, which then receives step k+4.
+// The barrier runs one step ahead, as in the FP64 kernel: after 48 of the 64
+// MFMAs of step k each wave waits for its own loads of step k+2, one barrier
+// publishes them and frees the stage of step k (read during step k-1), which
+// then receives step k+3 (+1.4 % over a 4-stage ring with the barrier after
+// the MFMAs, at 16384^3).
 //
 // Edges: rows past the end of a k-contiguous operand read its last row;
 // row chunks past the end of a row-contiguous operand read the last 16-byte
@@ -45,10 +47,10 @@ typedef __attribute__((address_space(3))) void lds_void_t;
 typedef __attribute__((address_space(1))) const void gvoid_t;
 
 namespace sk {
-constexpr int BM = 128, BN = 128, BK = 16, STG = 4;
+constexpr int BM = 128, BN = 128, BK = 16, STG = 3;
 constexpr int PART = 128 * BK * 4;  // 8 KB per operand per stage
 constexpr int SSZ = 2 * PART;
-constexpr size_t LDS_BYTES = (size_t)STG * SSZ;  // 64 KB
+constexpr size_t LDS_BYTES = (size_t)STG * SSZ;  // 48 KB
 constexpr int GROUP_M = 8;
 
 __host__ __device__ constexpr int swz(int r) { return (r >> 1) & 3; }
@@ -191,7 +193,7 @@ __global__ __launch_bounds__(256, 2) void sgemm_dma_kernel(SgemmArgs P) {
         const int t = (EDGE && left == 1) ? tail : 0;
         issue_operand<AK>(st, ca, wave, lane, t);
         issue_operand<BKC>(st + PART, cb, wave, lane, t);
-        ist = (ist + 1) & (STG - 1);
+        ist = ist + 1 == STG ? 0 : ist + 1;
         if (--left == 0) {
             left = INT64_MAX;
             ca = zero_cursor(lane);
@@ -207,37 +209,50 @@ __global__ __launch_bounds__(256, 2) void sgemm_dma_kernel(SgemmArgs P) {
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[i][j] = fv4{0.f, 0.f, 0.f, 0.f};
 
+    static_assert(STG == 3, "the waits below assume a 3-stage ring");
     for (int64_t s = 0; s < STG && s < nk; ++s) issue_next();
-    // steps 0 and 1 published (4 loads per wave and step)
-    if (nk >= 4) wait_vmcnt<8>(); else if (nk == 3) wait_vmcnt<4>(); else wait_vmcnt<0>();
+    // steps 0 and 1 published (4 loads per wave and step; step 2 may be in flight)
+    if (nk >= 3) wait_vmcnt<4>(); else wait_vmcnt<0>();
     __builtin_amdgcn_s_barrier();
+    int rst = 1;  // stage of step kt + 1
     float fa0[4][4], fb0[4][4], fa1[4][4], fb1[4][4];
     read_operand<AK>(fa0, lds, wm, lr, lg);
     read_operand<BKC>(fb0, lds + PART, wn, lr, lg);
-    // LDS read instructions per step and the MFMAs interleaved after each
-    constexpr int NR = (AK ? 4 : 16) + (BKC ? 4 : 16);
-    constexpr int M1 = NR <= 8 ? 4 : (NR <= 20 ? 2 : 1);
-    constexpr int MREST = 64 - 2 - NR * M1;
+    // LDS read instructions of each operand per step and the MFMAs interleaved
+    // after each; MFMAs f = 16s + 4i + j in order, the barrier after f = 47
+    constexpr int NRA = AK ? 4 : 16, NRB = BKC ? 4 : 16;
+    constexpr int MA = NRA == 4 ? 8 : 2, MB = NRB == 4 ? 3 : 0;
     int64_t kt = 0;
+#define SGEMM_MFMAS(CA, CB, F0, F1)                                                                 \
+    _Pragma("unroll") for (int f_ = F0; f_ < F1; ++f_) {                                            \
+        const int s_ = f_ >> 4, i_ = (f_ >> 2) & 3, j_ = f_ & 3;                                    \
+        acc[i_][j_] = __builtin_amdgcn_mfma_f32_16x16x4f32(CA[i_][s_], CB[j_][s_], acc[i_][j_], 0, 0, 0); \
+    }
 #define SGEMM_STEP(CA, CB, NA, NB)                                                                  \
     {                                                                                               \
-        const char* nst = lds + (int)((kt + 1) & (STG - 1)) * SSZ; /* harmless past the last step */ \
+        const char* nst = lds + rst * SSZ; /* harmless past the last step */                        \
         read_operand<AK>(NA, nst, wm, lr, lg);                                                      \
-        read_operand<BKC>(NB, nst + PART, wn, lr, lg);                                              \
-        _Pragma("unroll") for (int s_ = 0; s_ < 4; ++s_)                                            \
-        _Pragma("unroll") for (int i = 0; i < 4; ++i)                                               \
-        _Pragma("unroll") for (int j = 0; j < 4; ++j)                                               \
-            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(CA[i][s_], CB[j][s_], acc[i][j], 0, 0, 0); \
+        SGEMM_MFMAS(CA, CB, 0, 48)                                                                  \
         __builtin_amdgcn_sched_group_barrier(0x008, 2, 0);                                          \
-        _Pragma("unroll") for (int g_ = 0; g_ < NR; ++g_) {                                         \
+        _Pragma("unroll") for (int g_ = 0; g_ < NRA; ++g_) {                                        \
             __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);                                      \
-            __builtin_amdgcn_sched_group_barrier(0x008, M1, 0);                                     \
+            __builtin_amdgcn_sched_group_barrier(0x008, MA, 0);                                     \
         }                                                                                           \
-        __builtin_amdgcn_sched_group_barrier(0x008, MREST, 0);                                      \
+        rst = rst + 1 == STG ? 0 : rst + 1;                                                         \
         if (kt + 2 < nk) {                                                                          \
-            if (kt + 3 < nk) wait_vmcnt<4>(); else wait_vmcnt<0>();                                 \
+            /* LDS reads retire in order: once at most 4 of the >= 4 reads just issued are pending, \
+               the stage of step kt (read during step kt-1) is no longer being read */             \
+            __builtin_amdgcn_s_waitcnt(0xF | (0x3 << 14) | (0x7 << 4) | (4 << 8));                  \
+            wait_vmcnt<0>(); /* step kt+2, the only loads in flight */                              \
             __builtin_amdgcn_s_barrier();                                                           \
             if (kt + STG < nk) issue_next();                                                        \
+        }                                                                                           \
+        read_operand<BKC>(NB, nst + PART, wn, lr, lg);                                              \
+        SGEMM_MFMAS(CA, CB, 48, 64)                                                                 \
+        __builtin_amdgcn_sched_group_barrier(0x008, 2, 0);                                          \
+        _Pragma("unroll") for (int g_ = 0; g_ < NRB; ++g_) {                                        \
+            __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);                                      \
+            __builtin_amdgcn_sched_group_barrier(0x008, MB, 0);                                     \
         }                                                                                           \
         ++kt;                                                                                       \
     }
@@ -246,6 +261,7 @@ __global__ __launch_bounds__(256, 2) void sgemm_dma_kernel(SgemmArgs P) {
         SGEMM_STEP(fa1, fb1, fa0, fb0);
     }
 #undef SGEMM_STEP
+#undef SGEMM_MFMAS
 
     // acc[i][j][v] = sum for C(m0 + wm + 16i + 4lg + v, n0 + wn + 16j + lr)
     const float alpha = P.alpha, beta = P.beta;
