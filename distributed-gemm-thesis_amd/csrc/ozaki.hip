@@ -21,12 +21,13 @@
 //
 // Slices are stored k-contiguous, [t][row][Kp] with rows padded to 256 and K
 // padded to 128 bytes with zeros, so the product kernel has no edge cases
-// until the final store.  The product kernel uses 256x256 output tiles, 4
-// waves of 128x128 (256 accumulator registers per lane) and a 4-stage LDS
-// ring of 64-byte k-steps filled by direct global-to-LDS loads.  The FP64
-// partial sums W live in HBM (workspace): a flush is one coalesced
-// read-modify-write of the wave's 128x128 block, a handful per tile, so the
-// registers hold only the 32-bit accumulators.
+// until the final store.  The scheme I product kernel uses 256x256 output
+// tiles, 8 waves of 128x64 (128 accumulator registers per lane) and a 5-stage
+// LDS ring of 64-byte k-steps filled by direct global-to-LDS loads (scheme II
+// moves 128-byte k-steps, described with its kernel).  The FP64 partial sums
+// W live in HBM (workspace): a flush is one coalesced read-modify-write of
+// the wave's 128x64 block, a handful per tile, so the registers hold only the
+// 32-bit accumulators.
 #include "dg_internal.hpp"
 
 // Every FP64 operation of the recombination is rounded on its own (no fused
