@@ -41,6 +41,13 @@ def stored_shapes(opA, opB, M, N, K):
     ("T", "N", 96, 96, 192, 4, 1, dgamd.DG_SCHED_REFERENCE),
     ("T", "T", 60, 90, 30, 3, 1, dgamd.DG_SCHED_STANDARD),
     ("N", "N", 64, 64, 64, 8, 4, dgamd.DG_SCHED_AUTO),
+    # grids where the reference's rotation offsets miss products: standard Cannon
+    ("N", "N", 72, 48, 60, 6, 2, dgamd.DG_SCHED_STANDARD),
+    ("T", "T", 72, 48, 60, 6, 2, dgamd.DG_SCHED_AUTO),
+    ("T", "N", 50, 40, 35, 5, 1, dgamd.DG_SCHED_STANDARD),
+    ("N", "T", 42, 84, 126, 7, 1, dgamd.DG_SCHED_STANDARD),
+    # p = c: one step per layer, every product on its own layer
+    ("N", "T", 64, 64, 64, 4, 4, dgamd.DG_SCHED_REFERENCE),
 ])
 def test_cannon_exact_vs_oracle(dev, opA, opB, M, N, K, p, c, sched):
     rng = np.random.default_rng(M + N + K + p + c)
@@ -49,8 +56,10 @@ def test_cannon_exact_vs_oracle(dev, opA, opB, M, N, K, p, c, sched):
     B = rng.integers(-4, 5, rb * cb).astype(np.float64)
     C = rng.integers(-4, 5, M * N).astype(np.float64)
     Ap, Bp, Cp = packed2d(A, ra, ca, p), packed2d(B, rb, cb, p), packed2d(C, M, N, p)
-    ref = oracle.cannon25d(opA, opB, M, N, K, 2.0, Ap, Bp, 0.5, Cp.copy(), p, c,
-                           sched_std=(sched == dgamd.DG_SCHED_STANDARD))
+    # DG_SCHED_AUTO takes the reference rotations where they cover every k tile
+    ref_covers = dgamd.lib().dg_cannon_m_index(p, c, dgamd.DG_SCHED_REFERENCE, 0, 0, 0, 0) >= 0
+    std = sched == dgamd.DG_SCHED_STANDARD or (sched == dgamd.DG_SCHED_AUTO and not ref_covers)
+    ref = oracle.cannon25d(opA, opB, M, N, K, 2.0, Ap, Bp, 0.5, Cp.copy(), p, c, sched_std=std)
     Ad, Bd, Cd = to_dev(Ap, dev), to_dev(Bp, dev), to_dev(Cp, dev)
     ws = torch.full((max(1, dgamd.cannon25d_workspace(M, N, p, c)),), float("nan"), dtype=torch.float64, device=dev)
     dgamd.cannon25d_t(opA, opB, M, N, K, 2.0, Ad, Bd, 0.5, Cd, p, c, sched, ws)
