@@ -362,7 +362,7 @@ def main():
     achieved = flops / avg / 1e12
 
     out = {
-        "metric": "FP64-equiv TFLOP/s @16384^3 (native vs Ozaki-INT8), % MFMA peak, rel err vs FP64",
+        "metric": "FP64-equiv TFLOP/s @16384\u00b3 (native vs Ozaki-INT8), % MFMA peak, rel err vs FP64",
         "value": round(value, 3),
         "unit": "TFLOP/s",
         "n_gpus": 1,
