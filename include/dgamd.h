@@ -8,6 +8,13 @@
  * message of the last error on the calling thread is available from
  * dg_last_error().
  *
+ * HIP graph capture: the GEMM, SUMMA, Cannon, pack and Ozaki entry points
+ * enqueue only kernel launches.  Problem and segment descriptors live in a
+ * device-resident cache keyed by their contents, so one call made before
+ * capture with the same shapes and pointers makes them resident; a call that
+ * would have to upload a new table during capture fails with DG_ERR_INVALID
+ * instead.
+ *
  * Reference interfaces that each entry point replaces are cited as
  * path:line inside bvrettos/Distributed-GEMM-Thesis.
  */
