@@ -11,16 +11,16 @@
 //     64-byte rows, 16-byte chunk c of row r at position c ^ ((r >> 1) & 3);
 //     a lane reads its four k with one 128-bit read.
 //   * row-contiguous operand (A for opA = N, B for opB = T): LDS [k][row]
-//     without padding, two k-rows per 1 KB load; row chunk c (4 rows) of
-//     k-row kk sits at position c ^ (4 * ((kk >> 2) & 3)) of its k-row, so
-//     the 32-bit reads of the four k-rows one MFMA sub-step needs fall in 64
-//     distinct banks.  The permutation is applied on the global side (each
-//     lane fetches the chunk that belongs at its LDS slot).
+//     without padding, two k-rows per 1 KB load.  Lane lr of an MFMA block i
+//     stands for row 4 lr + i, so a lane's rows of all four blocks are one
+//     16-byte chunk of a k-row: one 128-bit read per k sub-step (16 lanes read
+//     256 consecutive bytes) instead of four 32-bit ones.
 // Lane l of an MFMA supplies k = 4*(l/16) + s at sub-step s (s = 0..3) for
 // both operands, so each product term pairs the right A and B elements
 // whatever order the matrix core sums in.  Result layout of
-// v_mfma_f32_16x16x4_f32: lane l, register v holds D(4*(l/16) + v, l%16),
-// so each lane stores four consecutive rows of one C column (16 bytes).
+// v_mfma_f32_16x16x4_f32: lane l, register v holds D(4*(l/16) + v, l%16);
+// with either row mapping each lane stores runs of four consecutive rows of
+// one C column (16 bytes).
 // The barrier runs one step ahead, as in the FP64 kernel: after 48 of the 64
 // MFMAs of step k each wave waits for its own loads of step k+2, one barrier
 // publishes them and frees the stage of step k (read during step k-1), which
@@ -87,9 +87,9 @@ __device__ __forceinline__ Cursor make_cursor(const float* base, int64_t ld, int
         return Cursor{base + ra * ld + 4 * chunk, (rb - ra) * ld, BK};
     } else {
         // k-rows 4w + h (first load) and 4w + 2 + h (second), h = lane / 32;
-        // slot lane % 32 of the k-row holds row chunk (lane % 32) ^ 4w
+        // the lane moves rows 4 (lane % 32) .. + 3 of its k-row
         const int h = lane >> 5;
-        const int c = (lane & 31) ^ (4 * wave);
+        const int c = lane & 31;
         int64_t r = r0 + 4 * c;
         if (EDGE) r = min(r, (rows - 1) & ~(int64_t)3);
         return Cursor{base + r + (int64_t)(4 * wave + h) * ld, 2 * ld, BK * ld};
@@ -122,24 +122,25 @@ __device__ __forceinline__ void issue_operand(char* dst, Cursor& c, int wave, in
     c.p += c.step;
 }
 
-// f[i][s] = X(row w0 + 16i + lr, k = 4 lg + s) of one stage.
+// f[i][s] = X(row, k = 4 lg + s) of one stage, row = w0 + 16i + lr for a
+// k-contiguous operand and w0 + 4lr + i for a row-contiguous one (whose four
+// rows of a lane are then one 16-byte chunk of each k-row: 16 lanes read 256
+// consecutive bytes).
 template <bool KC>
 __device__ __forceinline__ void read_operand(float (&f)[4][4], const char* part, int w0, int lr, int lg) {
+    if constexpr (KC) {
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        if constexpr (KC) {
+        for (int i = 0; i < 4; ++i) {
             const fv4 v = *reinterpret_cast<const fv4*>(part + (w0 + 16 * i + lr) * 64 + 16 * (lg ^ swz(lr)));
 #pragma unroll
             for (int s = 0; s < 4; ++s) f[i][s] = v[s];
-        } else {
-            const int row = w0 + 16 * i + lr;
-            const int slot = (row >> 2) ^ (4 * lg);
+        }
+    } else {
 #pragma unroll
-            for (int s = 0; s < 4; ++s) {
-                const int kk = 4 * lg + s;
-                f[i][s] = *reinterpret_cast<const float*>(part + ((kk >> 1) * 64 + (kk & 1) * 32 + slot) * 16 +
-                                                          (row & 3) * 4);
-            }
+        for (int s = 0; s < 4; ++s) {
+            const fv4 v = *reinterpret_cast<const fv4*>(part + (4 * lg + s) * 512 + (w0 + 4 * lr) * 4);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) f[i][s] = v[i];
         }
     }
 }
@@ -220,8 +221,8 @@ __global__ __launch_bounds__(256, 2) void sgemm_dma_kernel(SgemmArgs P) {
     read_operand<BKC>(fb0, lds + PART, wn, lr, lg);
     // LDS read instructions of each operand per step and the MFMAs interleaved
     // after each; MFMAs f = 16s + 4i + j in order, the barrier after f = 47
-    constexpr int NRA = AK ? 4 : 16, NRB = BKC ? 4 : 16;
-    constexpr int MA = NRA == 4 ? 8 : 2, MB = NRB == 4 ? 3 : 0;
+    constexpr int NRA = 4, NRB = 4;  // one 128-bit read per block (k-contiguous) or k sub-step (row-contiguous)
+    constexpr int MA = 8, MB = 3;
     int64_t kt = 0;
 #define SGEMM_MFMAS(CA, CB, F0, F1)                                                                 \
     _Pragma("unroll") for (int f_ = F0; f_ < F1; ++f_) {                                            \
@@ -263,36 +264,43 @@ __global__ __launch_bounds__(256, 2) void sgemm_dma_kernel(SgemmArgs P) {
 #undef SGEMM_STEP
 #undef SGEMM_MFMAS
 
-    // acc[i][j][v] = sum for C(m0 + wm + 16i + 4lg + v, n0 + wn + 16j + lr)
+    // acc[i][j][v] = sum for C(row(i, 4lg + v), col(j, lr)): row(i, r) = wm + 16i + r
+    // for a k-contiguous A and wm + 4r + i for a row-contiguous one, col(j, c)
+    // likewise for B; each lane stores runs of four consecutive rows
     const float alpha = P.alpha, beta = P.beta;
     const bool interior = !EDGE || (m0 + BM <= P.m && n0 + BN <= P.n);
+    auto store4 = [&](int64_t m, int64_t n, fv4 acc4) {
+        if (interior && P.cvec) {
+            __attribute__((address_space(1))) fv4* cp = (__attribute__((address_space(1))) fv4*)(P.c + m + n * P.ldc);
+            fv4 val = alpha * acc4;
+            if (beta != 0.f) {
+                const fv4 old = *cp;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        const int64_t m = m0 + wm + 16 * i + 4 * lg;
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int64_t n = n0 + wn + 16 * j + lr;
-            if (interior && P.cvec) {
-                __attribute__((address_space(1))) fv4* cp =
-                    (__attribute__((address_space(1))) fv4*)(P.c + m + n * P.ldc);
-                fv4 val = alpha * acc[i][j];
-                if (beta != 0.f) {
-                    const fv4 old = *cp;
-#pragma unroll
-                    for (int v = 0; v < 4; ++v) val[v] = fmaf(beta, old[v], val[v]);
-                }
-                *cp = val;
-            } else {
-#pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    if (m + v >= P.m || n >= P.n) continue;
-                    __attribute__((address_space(1))) float* cp =
-                        (__attribute__((address_space(1))) float*)(P.c + (m + v) + n * P.ldc);
-                    float val = alpha * acc[i][j][v];
-                    if (beta != 0.f) val = fmaf(beta, *cp, val);
-                    *cp = val;
-                }
+                for (int v = 0; v < 4; ++v) val[v] = fmaf(beta, old[v], val[v]);
             }
+            *cp = val;
+        } else {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                if (m + v >= P.m || n >= P.n) continue;
+                __attribute__((address_space(1))) float* cp =
+                    (__attribute__((address_space(1))) float*)(P.c + (m + v) + n * P.ldc);
+                float val = alpha * acc4[v];
+                if (beta != 0.f) val = fmaf(beta, *cp, val);
+                *cp = val;
+            }
+        }
+    };
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int64_t n = n0 + (BKC ? wn + 16 * j + lr : wn + 4 * lr + j);
+        if constexpr (AK) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) store4(m0 + wm + 16 * i + 4 * lg, n, acc[i][j]);
+        } else {
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+                store4(m0 + wm + 16 * lg + 4 * v, n, fv4{acc[0][j][v], acc[1][j][v], acc[2][j][v], acc[3][j][v]});
         }
     }
 }
