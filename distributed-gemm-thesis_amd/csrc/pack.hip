@@ -16,14 +16,17 @@ typedef __attribute__((address_space(1))) const double gcdouble;
 typedef __attribute__((address_space(1))) double gdouble;
 typedef double dv2 __attribute__((ext_vector_type(2)));
 
-// Copies one run of `len` doubles; both sides contiguous.
-__device__ __forceinline__ void copy_run(const double* __restrict__ src, double* __restrict__ dst, int64_t len) {
-    gcdouble* s = (gcdouble*)src;
-    gdouble* d = (gdouble*)dst;
+// Copies one run of `len` elements; both sides contiguous.
+template <typename T>
+__device__ __forceinline__ void copy_run(const T* __restrict__ src, T* __restrict__ dst, int64_t len) {
+    typedef __attribute__((address_space(1))) const T gcT;
+    typedef __attribute__((address_space(1))) T gT;
+    gcT* s = (gcT*)src;
+    gT* d = (gT*)dst;
     int64_t x = threadIdx.x;
     // Four independent loads in flight per thread.
     for (; x + 3 * kThreads < len; x += 4 * kThreads) {
-        double a = s[x], b = s[x + kThreads], c = s[x + 2 * kThreads], e = s[x + 3 * kThreads];
+        T a = s[x], b = s[x + kThreads], c = s[x + 2 * kThreads], e = s[x + 3 * kThreads];
         d[x] = a; d[x + kThreads] = b; d[x + 2 * kThreads] = c; d[x + 3 * kThreads] = e;
     }
     for (; x < len; x += kThreads) d[x] = s[x];
@@ -92,8 +95,9 @@ __global__ void cyclic_kernel(const double* __restrict__ src, double* __restrict
     }
 }
 
-__global__ void copy_block_kernel(int64_t rows, int64_t cols, const double* __restrict__ src, int64_t lds,
-                                  double* __restrict__ dst, int64_t ldd) {
+template <typename T>
+__global__ void copy_block_kernel(int64_t rows, int64_t cols, const T* __restrict__ src, int64_t lds,
+                                  T* __restrict__ dst, int64_t ldd) {
     for (int64_t c = (int64_t)blockIdx.x * kColsPerBlock; c < cols; c += (int64_t)gridDim.x * kColsPerBlock) {
         const int64_t cend = c + kColsPerBlock < cols ? c + kColsPerBlock : cols;
         for (int64_t cc = c; cc < cend; ++cc) copy_run(src + cc * lds, dst + cc * ldd, rows);
@@ -272,19 +276,37 @@ int dg_unpack_block_cyclic(const double* packed, int64_t mb, int64_t nb, int P, 
     return dg::cyclic<false>("dg_unpack_block_cyclic", packed, G, ldg, rows, cols, mb, nb, P, Q, stream);
 }
 
-int dg_copy_block(int64_t rows, int64_t cols, const double* src, int64_t lds, double* dst, int64_t ldd,
-                  void* stream) {
-    using namespace dg;
-    if (rows < 0 || cols < 0) return fail(DG_ERR_INVALID, "dg_copy_block: negative size");
-    if (lds < rows || ldd < rows) return fail(DG_ERR_INVALID, "dg_copy_block: leading dimension < rows");
+}  // extern "C"
+
+namespace dg {
+namespace {
+template <typename T>
+int copy_block(const char* name, int64_t rows, int64_t cols, const T* src, int64_t lds, T* dst, int64_t ldd,
+               void* stream) {
+    if (rows < 0 || cols < 0) return fail(DG_ERR_INVALID, "%s: negative size", name);
+    if (lds < rows || ldd < rows) return fail(DG_ERR_INVALID, "%s: leading dimension < rows", name);
     if (rows == 0 || cols == 0) return DG_OK;
-    if (!src || !dst) return fail(DG_ERR_INVALID, "dg_copy_block: null pointer");
+    if (!src || !dst) return fail(DG_ERR_INVALID, "%s: null pointer", name);
     int rc = require_device();
     if (rc) return rc;
-    hipLaunchKernelGGL(copy_block_kernel, dim3(col_blocks(cols, 1)), dim3(kThreads), 0, as_stream(stream), rows, cols,
-                       src, lds, dst, ldd);
+    hipLaunchKernelGGL(copy_block_kernel<T>, dim3(col_blocks(cols, 1)), dim3(kThreads), 0, as_stream(stream), rows,
+                       cols, src, lds, dst, ldd);
     DG_LAUNCH_CHECK();
     return DG_OK;
+}
+}  // namespace
+}  // namespace dg
+
+extern "C" {
+
+int dg_copy_block(int64_t rows, int64_t cols, const double* src, int64_t lds, double* dst, int64_t ldd,
+                  void* stream) {
+    return dg::copy_block("dg_copy_block", rows, cols, src, lds, dst, ldd, stream);
+}
+
+int dg_copy_block_f32(int64_t rows, int64_t cols, const float* src, int64_t lds, float* dst, int64_t ldd,
+                      void* stream) {
+    return dg::copy_block("dg_copy_block_f32", rows, cols, src, lds, dst, ldd, stream);
 }
 
 int dg_layer_reduce(double* C, const double* partial, int64_t n, int nlayers, void* stream) {

@@ -72,6 +72,7 @@ SIGNATURES = {
     "dg_pack_block2d": (_int, [_ptr, _i64, _i64, _i64, _i64p, _int, _i64p, _int, _ptr, _ptr]),
     "dg_unpack_block2d": (_int, [_ptr, _i64p, _int, _i64p, _int, _ptr, _i64, _i64, _i64, _ptr]),
     "dg_copy_block": (_int, [_i64, _i64, _ptr, _i64, _ptr, _i64, _ptr]),
+    "dg_copy_block_f32": (_int, [_i64, _i64, _ptr, _i64, _ptr, _i64, _ptr]),
     "dg_numroc": (_i64, [_i64, _i64, _int, _int, _int]),
     "dg_block_cyclic_size": (_i64, [_i64, _i64, _i64, _i64, _int, _int]),
     "dg_pack_block_cyclic": (_int, [_ptr, _i64, _i64, _i64, _i64, _i64, _int, _int, _ptr, _ptr]),
@@ -220,7 +221,11 @@ def unpack_block2d_t(packed, row_off, col_off, G, ldg, rows, cols):
 
 
 def copy_block_t(rows, cols, src, lds, dst, ldd):
-    check("dg_copy_block", lib().dg_copy_block(rows, cols, _dp(src), lds, _dp(dst), ldd, _stream()))
+    import torch
+    if src.dtype == torch.float32:
+        check("dg_copy_block_f32", lib().dg_copy_block_f32(rows, cols, _dp(src), lds, _dp(dst), ldd, _stream()))
+    else:
+        check("dg_copy_block", lib().dg_copy_block(rows, cols, _dp(src), lds, _dp(dst), ldd, _stream()))
 
 
 def pack_block_cyclic_t(G, ldg, rows, cols, mb, nb, P, Q, packed):

@@ -164,9 +164,12 @@ void copyBlock(long long rows, long long columns, T* src, T* dst, const long lon
     if (rows * columns <= 0) return;
     if (!columnMajor) std::swap(rows, columns);
     const hipMemcpyKind dir = findMemcpyKind(src, dst);
-    if (dir == hipMemcpyDeviceToDevice && std::is_same<T, double>::value) {
-        DG_CHECK(dg_copy_block(rows, columns, reinterpret_cast<const double*>(src), lds,
-                               reinterpret_cast<double*>(dst), ldd, nullptr));
+    if (dir == hipMemcpyDeviceToDevice) {
+        if constexpr (std::is_same<T, double>::value) {
+            DG_CHECK(dg_copy_block(rows, columns, src, lds, dst, ldd, nullptr));
+        } else {
+            DG_CHECK(dg_copy_block_f32(rows, columns, src, lds, dst, ldd, nullptr));
+        }
         HIP_CHECK(hipDeviceSynchronize());
         return;
     }

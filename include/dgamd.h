@@ -122,6 +122,11 @@ int dg_unpack_block2d(const double* packed,
 int dg_copy_block(int64_t rows, int64_t cols, const double* src, int64_t lds,
                   double* dst, int64_t ldd, void* stream);
 
+/* FP32 form of dg_copy_block.
+ * Replaces: copyBlock<float>, src/Transfers/sharedTransfers.cpp:88. */
+int dg_copy_block_f32(int64_t rows, int64_t cols, const float* src, int64_t lds,
+                      float* dst, int64_t ldd, void* stream);
+
 /* ------------------------------------------------- block-cyclic layout -- */
 
 /* ScaLAPACK NUMROC: rows (or columns) of an n-long dimension split in nb

@@ -92,6 +92,20 @@ def test_copy_block(dev):
     assert np.array_equal(D.cpu().numpy(), ref)
 
 
+@pytest.mark.parametrize("rows,cols,lds,ldd", [(60, 50, 101, 77), (1, 1, 1, 1), (4099, 3, 4099, 4100), (257, 129, 300, 257)])
+def test_copy_block_f32(dev, rows, cols, lds, ldd):
+    """FP32 strided copy: every element of the block moved bit for bit (the
+    oracle's FP64 copy restated in float32 by index), nothing else written."""
+    rng = np.random.default_rng(rows + cols)
+    src = rng.standard_normal(lds * cols).astype(np.float32)
+    dst0 = np.full(ldd * cols, 9.0, dtype=np.float32)
+    ref = dst0.reshape(cols, ldd).copy()
+    ref[:, :rows] = src.reshape(cols, lds)[:, :rows]
+    S, D = dev_copy(src, dev), dev_copy(dst0, dev)
+    dgamd.copy_block_t(rows, cols, S, lds, D, ldd)
+    assert np.array_equal(D.cpu().numpy().view(np.uint32), ref.reshape(-1).view(np.uint32))
+
+
 @pytest.mark.parametrize("n,layers", [(1 << 20, 1), (1000001, 3), (7, 2)])
 def test_layer_reduce_bit_exact(dev, n, layers):
     rng = np.random.default_rng(n)
