@@ -8,10 +8,15 @@
 #include <cstdio>
 #include <cstring>
 #include <vector>
+#include <cstdlib>
+#include <unistd.h>
+#include <sstream>
+#include <fstream>
 
 #include <hip/hip_runtime.h>
 
 #include "25DCannon.hpp"
+#include "MatrixUtilities.hpp"
 #include "cmatrix.h"
 #include "2DBlockSequentialDecomposition.hpp"
 #include "gridLayout.hpp"
@@ -184,6 +189,28 @@ void test_generation() {
     report("MatrixInit<double/float> GPU == host", ok, 0.0);
 }
 
+// writeMatrix: one line per column of a column-major array (per row of a
+// row-major one), values space-separated, then an empty line; appends.
+void test_write_matrix() {
+    double m[6] = {1, 2, 3, 4, 5, 6};  // 2 x 3, column-major: columns (1 2) (3 4) (5 6)
+    float f[4] = {0.5f, 1.5f, 2.5f, 3.5f};
+    char path[] = "/tmp/dg_writematrix_XXXXXX";
+    const int fd = mkstemp(path);
+    bool ok = fd >= 0;
+    if (ok) {
+        close(fd);
+        std::string name(path);
+        writeMatrix(m, 2, 3, name);
+        writeMatrix(f, 2, 2, name, MatrixLayout::RowMajor);
+        std::ifstream in(name);
+        std::stringstream ss;
+        ss << in.rdbuf();
+        ok = ss.str() == "1 2 \n3 4 \n5 6 \n\n0.5 1.5 \n2.5 3.5 \n\n";
+        unlink(path);
+    }
+    report("writeMatrix column- and row-major text", ok, 0.0);
+}
+
 }  // namespace
 
 int main() {
@@ -197,6 +224,7 @@ int main() {
     test_grid_layout();
     test_validation_and_errors();
     test_generation();
+    test_write_matrix();
     printf("%s: %d failure(s)\n", failures ? "FAILED" : "PASSED", failures);
     return failures ? 1 : 0;
 }

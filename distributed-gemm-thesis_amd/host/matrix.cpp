@@ -1,5 +1,6 @@
 // Matrix generation, printing and block copies of the C++ host API (the
-// reference's src/Utilities/cmatrix.cpp and src/Transfers/sharedTransfers.cpp).
+// reference's src/Utilities/cmatrix.cpp, src/Utilities/MatrixUtilities.cpp and
+// src/Transfers/sharedTransfers.cpp).
 #include <cmath>
 #include <cstring>
 #include <fstream>
@@ -8,6 +9,7 @@
 #include <type_traits>
 #include <vector>
 
+#include "MatrixUtilities.hpp"
 #include "cmatrix.h"
 #include "transfers.hpp"
 
@@ -279,3 +281,40 @@ template void MatrixInit<double>(double*, const long long, const long long, int)
 template void MatrixInit<float>(float*, const long long, const long long, int);
 template void MatrixInit<double>(double*, const long long, int);
 template void MatrixInit<float>(float*, const long long, int);
+
+// ------------------------------------------------------------------ templated I/O (MatrixUtilities.hpp)
+
+template <typename scalar_t>
+void printMatrix(scalar_t* array, long long rows, long long columns, int rank, MatrixLayout layout) {
+    if (!array) {
+        dgHostError(-1, "printMatrix: null array");
+        return;
+    }
+    printf("Rank: %d\n", rank);
+    if (layout == MatrixLayout::ColumnMajor) std::swap(rows, columns);
+    for (long long i = 0; i < rows; ++i) {
+        for (long long j = 0; j < columns; ++j) printf("%lf ", (double)array[i * columns + j]);
+        printf("\n");
+    }
+    printf("\n");
+}
+
+template <typename scalar_t>
+void writeMatrix(scalar_t* array, long long rows, long long columns, std::string& filename, MatrixLayout layout) {
+    std::ofstream out(filename, std::ofstream::out | std::ofstream::app);
+    if (!out.is_open()) {
+        std::cerr << "Error opening file: " << filename << std::endl;
+        return;
+    }
+    if (layout == MatrixLayout::ColumnMajor) std::swap(rows, columns);
+    for (long long i = 0; i < rows; ++i) {
+        for (long long j = 0; j < columns; ++j) out << array[i * columns + j] << " ";
+        out << std::endl;
+    }
+    out << std::endl;
+}
+
+template void printMatrix<float>(float*, long long, long long, int, MatrixLayout);
+template void printMatrix<double>(double*, long long, long long, int, MatrixLayout);
+template void writeMatrix<float>(float*, long long, long long, std::string&, MatrixLayout);
+template void writeMatrix<double>(double*, long long, long long, std::string&, MatrixLayout);
