@@ -9,9 +9,11 @@
 // would have brought in; a SUMMA rank is one problem whose segments are the
 // owners' panels.
 //
-// Two kernels share the problem/segment tables and the tile order.  Batches
-// whose tiles are all interior run gemm_f64_dma_kernel (direct-to-LDS loads,
-// described with it below); the rest run gemm_f64_kernel:
+// Three kernels share the problem/segment tables and the tile order.  Batches
+// whose tiles are all interior run gemm_f64_dma_ar_kernel when A is
+// row-contiguous (opA = N: A straight into registers, B direct to LDS) and
+// gemm_f64_dma_kernel otherwise (both operands direct to LDS); both are
+// described below.  The rest run gemm_f64_kernel:
 // Tiling (256 threads = 4 waves of 64 lanes):
 //   block tile 128 x 128, k-step 16, LDS double buffered (2 blocks per CU);
 //   each wave owns a 64 x 64 sub-tile = 4 x 4 MFMA tiles of 16 x 16.
@@ -658,6 +660,221 @@ __global__ __launch_bounds__(256, 2) void gemm_f64_dma_kernel(const Prob* __rest
     }
 }
 
+// A in registers (opA = N, interior batches): each wave owns 32 rows of the
+// 128x128 tile and all 128 columns (2 x 8 MFMA tiles of 16x16), so no two
+// waves use the same element of A, and A needs neither LDS nor a barrier.
+// Lane (lr, lg) reads rows 2lr, 2lr+1 of its wave's 32 at k = 2lg + q with one
+// 16-byte global load per sub-step q: MFMA tile i = 0 takes the even rows,
+// i = 1 the odd ones.  A runs three k-steps ahead in a ring of registers
+// (loaded after the last MFMA that reads the slot); B goes through the
+// 3-stage direct-to-LDS ring of dma_tile, whose fragments of step k+1 are read
+// in two halves around the barrier.  Loads in issue order: B(k+3) at the
+// barrier of step k, A(k+3) after its MFMAs, so the barrier's wait for B(k+2)
+// leaves the two loads of A(k+2) in flight.
+template <bool BKC>
+__device__ __forceinline__ void dma_tile_ar(const Prob* __restrict__ probs, int nprob, const Seg* __restrict__ segs,
+                                            int tile, char* lds) {
+    using dmak::DBM; using dmak::DBN; using dmak::DBK; using dmak::DSTG;
+    using dmak::read_operand; using dmak::wait_vmcnt;
+    constexpr int SSZ = dmak::part<BKC>();
+    int lo = 0, hi = nprob - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (probs[mid].tile0[0] <= tile) lo = mid; else hi = mid - 1;
+    }
+    const Prob P = probs[lo];
+    const int local = tile - P.tile0[0];
+    const int per_group = GROUP_M * P.tiles_n[0];
+    const int group = local / per_group;
+    const int first_m = group * GROUP_M;
+    const int gsize = min(P.tiles_m[0] - first_m, GROUP_M);
+    const int in_group = local - group * per_group;
+    const int64_t m0 = (int64_t)(first_m + in_group % gsize) * DBM;
+    const int64_t n0 = (int64_t)(in_group / gsize) * DBN;
+
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lr = lane & 15, lg = lane >> 4;
+    const int64_t arow = m0 + 32 * wave + 2 * lr;
+
+    // issue cursors: every segment a whole number of k-steps, none past the end
+    int iseg = P.seg0;
+    Seg is = segs[iseg];
+    int64_t left = is.klen / DBK;
+    const double* ap = is.a + arow + (int64_t)(2 * lg) * is.lda;  // A at k = 2lg of the next step to issue
+    int64_t alda = is.lda;
+    dmak::OperandCursor cb = dmak::operand_cursor<BKC, false>(is.b, is.ldb, n0, P.n, 0, wave, lane);
+    const double* apend = ap;  // A of the step issued last
+    int64_t apld = alda;
+    int ist = 0;
+    auto issue_next = [&]() {
+        dmak::issue_operand<BKC>(lds + ist * SSZ, cb, wave, lane, 0);
+        ist = ist + 1 == DSTG ? 0 : ist + 1;
+        apend = ap;
+        apld = alda;
+        ap += DBK * alda;
+        if (--left == 0 && ++iseg < P.seg0 + P.nseg) {
+            is = segs[iseg];
+            left = is.klen / DBK;
+            ap = is.a + arow + (int64_t)(2 * lg) * is.lda;
+            alda = is.lda;
+            cb = dmak::operand_cursor<BKC, false>(is.b, is.ldb, n0, P.n, 0, wave, lane);
+        }
+    };
+    dv2 ra[3][2];  // ra[slot][q] = A(rows 2lr, 2lr + 1, k = 2lg + q) of the slot's step
+    auto load_a = [&](dv2 (&r)[2]) {
+        r[0] = gld2(apend);
+        r[1] = gld2(apend + apld);
+    };
+    int64_t nk = 0;  // even (an interior batch)
+    for (int sg = P.seg0; sg < P.seg0 + P.nseg; ++sg) nk += segs[sg].klen / DBK;
+
+    dv4 acc[2][8];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 8; ++j) acc[i][j] = dv4{0.0, 0.0, 0.0, 0.0};
+
+    static_assert(DSTG == 3, "the waits below assume a 3-stage ring");
+    issue_next();
+    load_a(ra[0]);
+    issue_next();
+    load_a(ra[1]);
+    if (nk > 2) {
+        issue_next();
+        load_a(ra[2]);
+        wait_vmcnt<6>();  // B of steps 0 and 1 landed; A(1), B(2), A(2) may be in flight
+    } else {
+        wait_vmcnt<0>();
+    }
+    __builtin_amdgcn_s_barrier();
+    int rst = 1;  // stage of step kt + 1
+    double fb0[2][4][2], fb1[2][4][2];  // f[h][jj][q] = B(row 64h + 16jj + lr, k = 2lg + q)
+    read_operand<BKC>(fb0[0], lds, 0, lr, lg);
+    read_operand<BKC>(fb0[1], lds, 64, lr, lg);
+    int64_t kt = 0;
+    // MFMAs f = 16q + 8i + j in order, the barrier after f = 23
+#define AR_MFMAS(SLOT, CB, F0, F1)                                                                  \
+    _Pragma("unroll") for (int f_ = F0; f_ < F1; ++f_) {                                            \
+        const int q_ = f_ >> 4, i_ = (f_ >> 3) & 1, j_ = f_ & 7;                                    \
+        acc[i_][j_] = __builtin_amdgcn_mfma_f64_16x16x4f64(CB[j_ >> 2][j_ & 3][q_], ra[SLOT][q_][i_], acc[i_][j_], 0, 0, 0); \
+    }
+// FULL: kt + 3 < nk is known (the barrier, the issue of B(kt+3) and the load of A(kt+3) all run)
+#define AR_STEP(FULL, SLOT, CB, NB)                                                                 \
+    {                                                                                               \
+        const char* nst = lds + rst * SSZ; /* harmless past the last step */                        \
+        read_operand<BKC>(NB[0], nst, 0, lr, lg);                                                   \
+        AR_MFMAS(SLOT, CB, 0, 24)                                                                   \
+        __builtin_amdgcn_sched_group_barrier(0x008, 2, 0);                                          \
+        _Pragma("unroll") for (int g_ = 0; g_ < 4; ++g_) {                                          \
+            __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);                                      \
+            __builtin_amdgcn_sched_group_barrier(0x008, 2, 0);                                      \
+        }                                                                                           \
+        __builtin_amdgcn_sched_group_barrier(0x008, 14, 0);                                         \
+        rst = rst + 1 == DSTG ? 0 : rst + 1;                                                        \
+        if (FULL || kt + 2 < nk) {                                                                  \
+            /* the stage of step kt was read during step kt-1: once at most the 4 reads just        \
+               issued are pending, none of it is still being read */                                \
+            __builtin_amdgcn_s_waitcnt(0xF | (0x3 << 14) | (0x7 << 4) | (4 << 8));                  \
+            wait_vmcnt<2>(); /* B(kt+2); A(kt+2), issued after it, may be in flight */              \
+            __builtin_amdgcn_s_barrier();                                                           \
+            if (FULL || kt + DSTG < nk) issue_next();                                                \
+        }                                                                                           \
+        read_operand<BKC>(NB[1], nst, 64, lr, lg);                                                  \
+        AR_MFMAS(SLOT, CB, 24, 32)                                                                  \
+        __builtin_amdgcn_sched_group_barrier(0x008, 2, 0);                                          \
+        _Pragma("unroll") for (int g_ = 0; g_ < 3; ++g_) {                                          \
+            __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);                                      \
+            __builtin_amdgcn_sched_group_barrier(0x008, 2, 0);                                      \
+        }                                                                                           \
+        if (FULL || kt + DSTG < nk) load_a(ra[SLOT]);                                               \
+        ++kt;                                                                                       \
+    }
+    // A slot = step % 3, B fragments alternate: six steps per turn.  The
+    // steady turns have no conditions (so the waits the compiler derives
+    // across the loop stay partial); the last 3 to 8 steps, nk even, follow.
+    while (kt + 9 <= nk) {
+        AR_STEP(true, 0, fb0, fb1);
+        AR_STEP(true, 1, fb1, fb0);
+        AR_STEP(true, 2, fb0, fb1);
+        AR_STEP(true, 0, fb1, fb0);
+        AR_STEP(true, 1, fb0, fb1);
+        AR_STEP(true, 2, fb1, fb0);
+    }
+    while (true) {
+        AR_STEP(false, 0, fb0, fb1);
+        AR_STEP(false, 1, fb1, fb0);
+        if (kt >= nk) break;
+        AR_STEP(false, 2, fb0, fb1);
+        AR_STEP(false, 0, fb1, fb0);
+        if (kt >= nk) break;
+        AR_STEP(false, 1, fb0, fb1);
+        AR_STEP(false, 2, fb1, fb0);
+        if (kt >= nk) break;
+    }
+#undef AR_STEP
+#undef AR_MFMAS
+
+    // acc[i][j][v] = sum for C(arow + i, n0 + 16j + lg + 4v)
+    const double alpha = P.alpha, beta = P.beta;
+    typedef __attribute__((address_space(1))) double gd;
+    typedef __attribute__((address_space(1))) dv2 gd2;
+    const bool vec = ((reinterpret_cast<uintptr_t>(P.c) & 15) == 0) && !(P.ldc & 1);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int64_t n = n0 + 16 * j + lg + 4 * v;
+            double* cp = P.c + arow + n * P.ldc;
+            dv2 val = dv2{alpha * acc[0][j][v], alpha * acc[1][j][v]};
+            if (vec) {
+                if (beta != 0.0) {
+                    const dv2 c = *(gd2*)cp;
+                    val = dv2{fma(beta, c.x, val.x), fma(beta, c.y, val.y)};
+                }
+                *(gd2*)cp = val;
+            } else {
+                if (beta != 0.0) val = dv2{fma(beta, ((gd*)cp)[0], val.x), fma(beta, ((gd*)cp)[1], val.y)};
+                ((gd*)cp)[0] = val.x;
+                ((gd*)cp)[1] = val.y;
+            }
+        }
+    }
+}
+
+template <bool BKC, int MODE>
+__global__ __launch_bounds__(256, 2) void gemm_f64_dma_ar_kernel(const Prob* __restrict__ probs, int nprob,
+                                                                  const Seg* __restrict__ segs, int total_tiles,
+                                                                  int* __restrict__ counter) {
+    extern __shared__ __attribute__((aligned(1024))) char lds[];
+    if (MODE == 0) {
+        dma_tile_ar<BKC>(probs, nprob, segs, blockIdx.x, lds);
+    } else if (MODE == 1) {
+        dma_tile_ar<BKC>(probs, nprob, segs, xcd_remap(blockIdx.x, total_tiles), lds);
+    } else {
+        __shared__ int next;
+        while (true) {
+            __syncthreads();  // the previous tile's LDS reads and `next` reads are complete
+            if (threadIdx.x == 0) next = atomicAdd(counter, 1);
+            __syncthreads();
+            const int tile = next;
+            if (tile >= total_tiles) break;
+            dma_tile_ar<BKC>(probs, nprob, segs, tile, lds);
+        }
+    }
+}
+
+template <bool BKC>
+void launch_dma_ar(int mode, const Prob* dp, int np, const Seg* ds, int tiles, int grid, int* counter, hipStream_t st) {
+    const size_t lb = (size_t)dmak::DSTG * dmak::part<BKC>();
+    if (mode == DG_GEMM_MODE_ROUND_ROBIN)
+        hipLaunchKernelGGL((gemm_f64_dma_ar_kernel<BKC, 0>), dim3(tiles), dim3(256), lb, st, dp, np, ds, tiles, counter);
+    else if (mode == DG_GEMM_MODE_XCD)
+        hipLaunchKernelGGL((gemm_f64_dma_ar_kernel<BKC, 1>), dim3(tiles), dim3(256), lb, st, dp, np, ds, tiles, counter);
+    else
+        hipLaunchKernelGGL((gemm_f64_dma_ar_kernel<BKC, 2>), dim3(grid), dim3(256), lb, st, dp, np, ds, tiles, counter);
+}
+
 template <bool AK, bool BKC, bool EDGE>
 void launch_dma_e(int mode, const Prob* dp, int np, const Seg* ds, int tiles, int grid, int* counter, hipStream_t st) {
     const size_t lb = dmak::lds_bytes(AK, BKC);
@@ -671,6 +888,10 @@ void launch_dma_e(int mode, const Prob* dp, int np, const Seg* ds, int tiles, in
 template <bool AK, bool BKC>
 void launch_dma(bool edge, int mode, const Prob* dp, int np, const Seg* ds, int tiles, int grid, int* counter,
                 hipStream_t st) {
+    static const bool no_areg = getenv("DG_GEMM_NO_AREG") != nullptr;
+    if constexpr (!AK) {
+        if (!edge && !no_areg) return launch_dma_ar<BKC>(mode, dp, np, ds, tiles, grid, counter, st);
+    }
     if (edge) launch_dma_e<AK, BKC, true>(mode, dp, np, ds, tiles, grid, counter, st);
     else launch_dma_e<AK, BKC, false>(mode, dp, np, ds, tiles, grid, counter, st);
 }

@@ -406,12 +406,13 @@ __device__ __forceinline__ int mod_exact(double x, int p, double inv_p) {
 
 // Balanced residues of A' = rint(op(X)(row, k) * 2^(q - e_row)) for every
 // modulus, [t][row][Kp] int8 (zeros in the padding).  A workgroup stages a
-// 32-row x 128-k tile of op(X) through LDS (loaded along whichever direction
-// is contiguous in memory); each thread then turns 16 consecutive k of one
-// row into one 16-byte store per modulus, so a wave writes whole 128-byte
-// lines.  Residue: q = rint(a/p) from a*fl(1/p), r = a - q*p exact by FMA
+// 16-row x 128-k tile of op(X) through LDS (loaded along whichever direction
+// is contiguous in memory; 16.5 KB, so eight workgroups share a CU and the
+// load and arithmetic phases of different workgroups overlap); each thread
+// then turns 8 consecutive k of one row into one 8-byte store per modulus,
+// so a wave writes whole 128-byte lines.  Residue: q = rint(a/p) from a*fl(1/p), r = a - q*p exact by FMA
 // (|r| <= p/2 + p), then one step into [-floor(p/2), ceil(p/2) - 1].
-constexpr int RT_R = 32, RT_K = 128;
+constexpr int RT_R = 16, RT_K = 128, RT_T = 256 / RT_R, RT_KPT = RT_K / RT_T;  // threads per row, k per thread
 __global__ __launch_bounds__(256) void crt_residue_kernel(const double* __restrict__ x, int64_t rows, int64_t K,
                                                           int64_t rs, int64_t ks, const int32_t* __restrict__ e, int q,
                                                           int nmod, int8_t* __restrict__ out, int64_t Kp,
@@ -428,19 +429,19 @@ __global__ __launch_bounds__(256) void crt_residue_kernel(const double* __restri
         }
     } else {
 #pragma unroll 4
-        for (int kk = t >> 5; kk < RT_K; kk += 8) {
-            const int64_t r = r0 + (t & 31), k = k0 + kk;
-            tile[t & 31][kk] = (r < rows && k < K) ? x[r * rs + k * ks] : 0.0;
+        for (int kk = t / RT_R; kk < RT_K; kk += 256 / RT_R) {
+            const int64_t r = r0 + (t % RT_R), k = k0 + kk;
+            tile[t % RT_R][kk] = (r < rows && k < K) ? x[r * rs + k * ks] : 0.0;
         }
     }
     __syncthreads();
-    const int rr = t >> 3;
-    const int kq = (t & 7) * 16;
+    const int rr = t / RT_T;
+    const int kq = (t % RT_T) * RT_KPT;
     const int64_t r = r0 + rr;
-    double a[16];
+    double a[RT_KPT];
     const int sh = r < rows ? q - e[r] : 0;
 #pragma unroll
-    for (int c = 0; c < 16; ++c) a[c] = r < rows ? rint(ldexp(tile[rr][kq + c], sh)) : 0.0;
+    for (int c = 0; c < RT_KPT; ++c) a[c] = r < rows ? rint(ldexp(tile[rr][kq + c], sh)) : 0.0;
     int8_t* dst = out + r * Kp + k0 + kq;
 #pragma unroll
     for (int m = 0; m < kMaxModuli; ++m) {
@@ -448,16 +449,17 @@ __global__ __launch_bounds__(256) void crt_residue_kernel(const double* __restri
             constexpr double one = 1.0;
             const int p = crt_modulus(m);
             const double pd = p, inv_p = one / crt_modulus(m);
-            uint32_t w[4] = {0, 0, 0, 0};
+            static_assert(RT_KPT == 8, "one 8-byte store per modulus");
+            uint32_t w[2] = {0, 0};
 #pragma unroll
-            for (int c = 0; c < 16; ++c) {
+            for (int c = 0; c < RT_KPT; ++c) {
                 const double qd = rint(a[c] * inv_p);
                 int v = (int)fma(-qd, pd, a[c]);
                 if (2 * v >= p) v -= p;
                 if (2 * v < -p) v += p;
                 w[c >> 2] |= (uint32_t)(v & 255) << (8 * (c & 3));
             }
-            *reinterpret_cast<iv4*>(dst + m * slice_bytes) = iv4{(int)w[0], (int)w[1], (int)w[2], (int)w[3]};
+            *reinterpret_cast<uint2*>(dst + m * slice_bytes) = uint2{w[0], w[1]};
         }
     }
 }

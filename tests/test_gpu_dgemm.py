@@ -164,3 +164,45 @@ def test_invalid_arguments(dev):
     L = dgamd.lib()
     assert L.dg_dgemm(b"X", b"N", 4, 4, 4, 1.0, None, 4, None, 4, 0.0, None, 4, None) == dgamd.DG_ERR_INVALID
     assert L.dg_dgemm(b"N", b"N", 4, 4, 4, 1.0, None, 3, None, 4, 0.0, None, 4, None) == dgamd.DG_ERR_INVALID
+
+
+@pytest.mark.parametrize("opB", ["N", "T"])
+@pytest.mark.parametrize("steps", [2, 4, 6, 8, 10, 12, 14, 16, 18, 26, 52])
+def test_interior_step_counts_exact(dev, opB, steps):
+    """Whole 128x128 tiles with opA = N (A held in registers, three k-steps
+    ahead, in turns of six steps): k-step counts from the shortest prologue
+    through whole turns and tails of 4, 6 and 8 steps, bit-exact on integer
+    data.  The second shape has beta != 0 and an odd ldc (element stores);
+    the padding rows of C stay untouched."""
+    k = 8 * steps
+    for m, n, beta, padc in [(256, 384, 0.0, 0), (384, 128, 0.5, 1)]:
+        got, ref = run(dev, "N", opB, m, n, k, 1.0, beta, pad=(0, 0, padc), integer=True, seed=steps)
+        assert np.array_equal(got, ref), (m, n, k, beta, padc)
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])  # autotune, one launch per tile, persistent
+def test_interior_grouped_modes_exact(dev, mode):
+    """A grouped batch of interior opA = N problems with different k-step
+    counts (and so different turn and tail splits), in every launch mode."""
+    rng = np.random.default_rng(40 + mode)
+    shapes = [(128, 256, 16), (256, 128, 96), (384, 256, 208), (128, 128, 144)]
+    descs, outs, keep = [], [], []  # descriptors hold raw addresses: keep A and B alive
+    for i, (m, n, k) in enumerate(shapes):
+        opB = "N" if i % 2 == 0 else "T"
+        rb, cb = (k, n) if opB == "N" else (n, k)
+        a = rng.integers(-8, 9, (m, k)).astype(np.float64)
+        b = rng.integers(-8, 9, (rb, cb)).astype(np.float64)
+        c = rng.integers(-8, 9, (m, n)).astype(np.float64)
+        A = torch.from_numpy(np.asfortranarray(a).ravel(order="F")).to(dev)
+        B = torch.from_numpy(np.asfortranarray(b).ravel(order="F")).to(dev)
+        C = torch.from_numpy(np.asfortranarray(c).ravel(order="F")).to(dev)
+        keep.append((A, B))
+        descs.append(dgamd.gemm_desc("N", opB, m, n, k, 2.0, A, m, B, rb, -1.0, C, m))
+        ref = 2.0 * (a @ (b if opB == "N" else b.T)) - c
+        outs.append((C, m, n, ref))
+    plan = dgamd.GroupedPlan(descs, mode)
+    plan.execute()
+    torch.cuda.synchronize()
+    bad = [(m, n, int((C.cpu().numpy().reshape(n, m).T != ref).sum())) for C, m, n, ref in outs
+           if not np.array_equal(C.cpu().numpy().reshape(n, m).T, ref)]
+    assert not bad, bad

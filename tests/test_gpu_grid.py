@@ -128,6 +128,12 @@ def test_config3_shape_scaled(dev):
     (100, 70, 50, 16, 8, 7, 3, 2),
     (33, 65, 129, 4, 16, 32, 2, 4),
     (40, 40, 40, 64, 64, 64, 2, 2),   # blocks larger than the matrix: ranks with no data
+    # whole 128x128 rank tiles: one k-step per panel (a segment change at every
+    # step, also inside the prologue), 2 to 26 steps, and 3-step panels
+    (512, 512, 16, 64, 64, 8, 2, 2),
+    (512, 512, 80, 64, 64, 8, 2, 2),
+    (512, 256, 208, 64, 64, 8, 2, 2),
+    (512, 256, 144, 128, 64, 24, 2, 2),
 ])
 def test_summa_exact_vs_oracle(dev, M, N, K, mb, nb, kb, P, Q):
     rng = np.random.default_rng(M * N + K)
