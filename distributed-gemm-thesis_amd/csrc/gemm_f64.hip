@@ -10,10 +10,9 @@
 // owners' panels.
 //
 // Three kernels share the problem/segment tables and the tile order.  Batches
-// whose tiles are all interior run gemm_f64_dma_ar_kernel when A is
-// row-contiguous (opA = N: A straight into registers, B direct to LDS) and
-// gemm_f64_dma_kernel otherwise (both operands direct to LDS); both are
-// described below.  The rest run gemm_f64_kernel:
+// whose tiles are all interior run gemm_f64_dma_ar_kernel (A straight into
+// registers, B direct to LDS); the others that take direct loads run
+// gemm_f64_dma_kernel (both operands direct to LDS); both are described below.  The rest run gemm_f64_kernel:
 // Tiling (256 threads = 4 waves of 64 lanes):
 //   block tile 128 x 128, k-step 16, LDS double buffered (2 blocks per CU);
 //   each wave owns a 64 x 64 sub-tile = 4 x 4 MFMA tiles of 16 x 16.
@@ -660,18 +659,19 @@ __global__ __launch_bounds__(256, 2) void gemm_f64_dma_kernel(const Prob* __rest
     }
 }
 
-// A in registers (opA = N, interior batches): each wave owns 32 rows of the
-// 128x128 tile and all 128 columns (2 x 8 MFMA tiles of 16x16), so no two
-// waves use the same element of A, and A needs neither LDS nor a barrier.
-// Lane (lr, lg) reads rows 2lr, 2lr+1 of its wave's 32 at k = 2lg + q with one
-// 16-byte global load per sub-step q: MFMA tile i = 0 takes the even rows,
-// i = 1 the odd ones.  A runs three k-steps ahead in a ring of registers
+// A in registers (interior batches): each wave owns 32 rows of the 128x128
+// tile and all 128 columns (2 x 8 MFMA tiles of 16x16), so no two waves use
+// the same element of A, and A needs neither LDS nor a barrier.  Lane
+// (lr, lg) holds rows 2lr, 2lr+1 of its wave's 32 at k = 2lg + q (q = 0, 1):
+// MFMA tile i = 0 takes the even rows, i = 1 the odd ones.  Two 16-byte
+// global loads per k-step: with A row-contiguous (opA = N) one per k (the
+// row pair), with A k-contiguous (opA = T) one per row (the k pair).  A runs three k-steps ahead in a ring of registers
 // (loaded after the last MFMA that reads the slot); B goes through the
 // 3-stage direct-to-LDS ring of dma_tile, whose fragments of step k+1 are read
 // in two halves around the barrier.  Loads in issue order: B(k+3) at the
 // barrier of step k, A(k+3) after its MFMAs, so the barrier's wait for B(k+2)
 // leaves the two loads of A(k+2) in flight.
-template <bool BKC>
+template <bool AK, bool BKC>
 __device__ __forceinline__ void dma_tile_ar(const Prob* __restrict__ probs, int nprob, const Seg* __restrict__ segs,
                                             int tile, char* lds) {
     using dmak::DBM; using dmak::DBN; using dmak::DBK; using dmak::DSTG;
@@ -701,7 +701,9 @@ __device__ __forceinline__ void dma_tile_ar(const Prob* __restrict__ probs, int 
     int iseg = P.seg0;
     Seg is = segs[iseg];
     int64_t left = is.klen / DBK;
-    const double* ap = is.a + arow + (int64_t)(2 * lg) * is.lda;  // A at k = 2lg of the next step to issue
+    // A at (arow, k = 2lg) of the next step to issue; the second load is one ld away
+    auto a_origin = [&](const Seg& g) { return AK ? g.a + arow * g.lda + 2 * lg : g.a + arow + (int64_t)(2 * lg) * g.lda; };
+    const double* ap = a_origin(is);
     int64_t alda = is.lda;
     dmak::OperandCursor cb = dmak::operand_cursor<BKC, false>(is.b, is.ldb, n0, P.n, 0, wave, lane);
     const double* apend = ap;  // A of the step issued last
@@ -712,16 +714,18 @@ __device__ __forceinline__ void dma_tile_ar(const Prob* __restrict__ probs, int 
         ist = ist + 1 == DSTG ? 0 : ist + 1;
         apend = ap;
         apld = alda;
-        ap += DBK * alda;
+        ap += AK ? DBK : DBK * alda;
         if (--left == 0 && ++iseg < P.seg0 + P.nseg) {
             is = segs[iseg];
             left = is.klen / DBK;
-            ap = is.a + arow + (int64_t)(2 * lg) * is.lda;
+            ap = a_origin(is);
             alda = is.lda;
             cb = dmak::operand_cursor<BKC, false>(is.b, is.ldb, n0, P.n, 0, wave, lane);
         }
     };
-    dv2 ra[3][2];  // ra[slot][q] = A(rows 2lr, 2lr + 1, k = 2lg + q) of the slot's step
+    // ra[slot][h] of the slot's step: opA = N, h = q: A(rows 2lr, 2lr + 1; k = 2lg + q);
+    // opA = T, h = i: A(row 2lr + i; k = 2lg, 2lg + 1)
+    dv2 ra[3][2];
     auto load_a = [&](dv2 (&r)[2]) {
         r[0] = gld2(apend);
         r[1] = gld2(apend + apld);
@@ -757,7 +761,8 @@ __device__ __forceinline__ void dma_tile_ar(const Prob* __restrict__ probs, int 
 #define AR_MFMAS(SLOT, CB, F0, F1)                                                                  \
     _Pragma("unroll") for (int f_ = F0; f_ < F1; ++f_) {                                            \
         const int q_ = f_ >> 4, i_ = (f_ >> 3) & 1, j_ = f_ & 7;                                    \
-        acc[i_][j_] = __builtin_amdgcn_mfma_f64_16x16x4f64(CB[j_ >> 2][j_ & 3][q_], ra[SLOT][q_][i_], acc[i_][j_], 0, 0, 0); \
+        acc[i_][j_] = __builtin_amdgcn_mfma_f64_16x16x4f64(CB[j_ >> 2][j_ & 3][q_],                \
+                                                           AK ? ra[SLOT][i_][q_] : ra[SLOT][q_][i_], acc[i_][j_], 0, 0, 0); \
     }
 // FULL: kt + 3 < nk is known (the barrier, the issue of B(kt+3) and the load of A(kt+3) all run)
 #define AR_STEP(FULL, SLOT, CB, NB)                                                                 \
@@ -842,15 +847,15 @@ __device__ __forceinline__ void dma_tile_ar(const Prob* __restrict__ probs, int 
     }
 }
 
-template <bool BKC, int MODE>
+template <bool AK, bool BKC, int MODE>
 __global__ __launch_bounds__(256, 2) void gemm_f64_dma_ar_kernel(const Prob* __restrict__ probs, int nprob,
                                                                   const Seg* __restrict__ segs, int total_tiles,
                                                                   int* __restrict__ counter) {
     extern __shared__ __attribute__((aligned(1024))) char lds[];
     if (MODE == 0) {
-        dma_tile_ar<BKC>(probs, nprob, segs, blockIdx.x, lds);
+        dma_tile_ar<AK, BKC>(probs, nprob, segs, blockIdx.x, lds);
     } else if (MODE == 1) {
-        dma_tile_ar<BKC>(probs, nprob, segs, xcd_remap(blockIdx.x, total_tiles), lds);
+        dma_tile_ar<AK, BKC>(probs, nprob, segs, xcd_remap(blockIdx.x, total_tiles), lds);
     } else {
         __shared__ int next;
         while (true) {
@@ -859,20 +864,20 @@ __global__ __launch_bounds__(256, 2) void gemm_f64_dma_ar_kernel(const Prob* __r
             __syncthreads();
             const int tile = next;
             if (tile >= total_tiles) break;
-            dma_tile_ar<BKC>(probs, nprob, segs, tile, lds);
+            dma_tile_ar<AK, BKC>(probs, nprob, segs, tile, lds);
         }
     }
 }
 
-template <bool BKC>
+template <bool AK, bool BKC>
 void launch_dma_ar(int mode, const Prob* dp, int np, const Seg* ds, int tiles, int grid, int* counter, hipStream_t st) {
     const size_t lb = (size_t)dmak::DSTG * dmak::part<BKC>();
     if (mode == DG_GEMM_MODE_ROUND_ROBIN)
-        hipLaunchKernelGGL((gemm_f64_dma_ar_kernel<BKC, 0>), dim3(tiles), dim3(256), lb, st, dp, np, ds, tiles, counter);
+        hipLaunchKernelGGL((gemm_f64_dma_ar_kernel<AK, BKC, 0>), dim3(tiles), dim3(256), lb, st, dp, np, ds, tiles, counter);
     else if (mode == DG_GEMM_MODE_XCD)
-        hipLaunchKernelGGL((gemm_f64_dma_ar_kernel<BKC, 1>), dim3(tiles), dim3(256), lb, st, dp, np, ds, tiles, counter);
+        hipLaunchKernelGGL((gemm_f64_dma_ar_kernel<AK, BKC, 1>), dim3(tiles), dim3(256), lb, st, dp, np, ds, tiles, counter);
     else
-        hipLaunchKernelGGL((gemm_f64_dma_ar_kernel<BKC, 2>), dim3(grid), dim3(256), lb, st, dp, np, ds, tiles, counter);
+        hipLaunchKernelGGL((gemm_f64_dma_ar_kernel<AK, BKC, 2>), dim3(grid), dim3(256), lb, st, dp, np, ds, tiles, counter);
 }
 
 template <bool AK, bool BKC, bool EDGE>
@@ -889,9 +894,7 @@ template <bool AK, bool BKC>
 void launch_dma(bool edge, int mode, const Prob* dp, int np, const Seg* ds, int tiles, int grid, int* counter,
                 hipStream_t st) {
     static const bool no_areg = getenv("DG_GEMM_NO_AREG") != nullptr;
-    if constexpr (!AK) {
-        if (!edge && !no_areg) return launch_dma_ar<BKC>(mode, dp, np, ds, tiles, grid, counter, st);
-    }
+    if (!edge && !no_areg) return launch_dma_ar<AK, BKC>(mode, dp, np, ds, tiles, grid, counter, st);
     if (edge) launch_dma_e<AK, BKC, true>(mode, dp, np, ds, tiles, grid, counter, st);
     else launch_dma_e<AK, BKC, false>(mode, dp, np, ds, tiles, grid, counter, st);
 }

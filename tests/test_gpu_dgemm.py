@@ -166,17 +166,18 @@ def test_invalid_arguments(dev):
     assert L.dg_dgemm(b"N", b"N", 4, 4, 4, 1.0, None, 3, None, 4, 0.0, None, 4, None) == dgamd.DG_ERR_INVALID
 
 
-@pytest.mark.parametrize("opB", ["N", "T"])
+@pytest.mark.parametrize("opA,opB", [("N", "N"), ("N", "T"), ("T", "N"), ("T", "T")])
 @pytest.mark.parametrize("steps", [2, 4, 6, 8, 10, 12, 14, 16, 18, 26, 52])
-def test_interior_step_counts_exact(dev, opB, steps):
-    """Whole 128x128 tiles with opA = N (A held in registers, three k-steps
-    ahead, in turns of six steps): k-step counts from the shortest prologue
+def test_interior_step_counts_exact(dev, opA, opB, steps):
+    """Whole 128x128 tiles (A held in registers, three k-steps ahead, in
+    turns of six steps; row pairs for opA = N, k pairs for opA = T): k-step
+    counts from the shortest prologue
     through whole turns and tails of 4, 6 and 8 steps, bit-exact on integer
     data.  The second shape has beta != 0 and an odd ldc (element stores);
     the padding rows of C stay untouched."""
     k = 8 * steps
     for m, n, beta, padc in [(256, 384, 0.0, 0), (384, 128, 0.5, 1)]:
-        got, ref = run(dev, "N", opB, m, n, k, 1.0, beta, pad=(0, 0, padc), integer=True, seed=steps)
+        got, ref = run(dev, opA, opB, m, n, k, 1.0, beta, pad=(0, 0, padc), integer=True, seed=steps)
         assert np.array_equal(got, ref), (m, n, k, beta, padc)
 
 
