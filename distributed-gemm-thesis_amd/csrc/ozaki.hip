@@ -731,6 +731,14 @@ __global__ void crt_reconstruct_kernel(const uint8_t* __restrict__ res, CrtTable
     }
 }
 
+typedef double pdv2 __attribute__((ext_vector_type(2)));
+
+// Rows contiguous in k whose every row starts 16-byte aligned: the scans read
+// them in 16-byte pairs, 16 lanes per row (host and device take the same test).
+__host__ __device__ inline bool pair_rows(const double* x, int64_t rs) {
+    return !(rs & 1) && !(reinterpret_cast<uintptr_t>(x) & 15);
+}
+
 // Row maxima of op(X): partial max |x| over a k-chunk for every row.
 // Element (row, k) of op(X) is at x[row*rs + k*ks].
 __global__ void oz_partial_max_kernel(const double* __restrict__ x, int64_t rows, int64_t K, int64_t rs, int64_t ks,
@@ -754,6 +762,26 @@ __global__ void oz_partial_max_kernel(const double* __restrict__ x, int64_t rows
             }
             for (; k < k1; ++k) m0 = fmax(m0, fabs(xr[k * ks]));
             part[blockIdx.y * rows + r] = fmax(fmax(m0, m1), fmax(m2, m3));
+        }
+    } else if (pair_rows(x, rs)) {
+        // 16 lanes per row, k pairs 32 apart: a load of the wave reads 256
+        // contiguous bytes of each of its four rows
+        const int l16 = threadIdx.x & 15;
+        const int64_t groups = (int64_t)gridDim.x * (blockDim.x >> 4);
+        for (int64_t r = blockIdx.x * (int64_t)(blockDim.x >> 4) + (threadIdx.x >> 4); r < rows; r += groups) {
+            const double* xr = x + r * rs;
+            double mx = 0.0;
+            for (int64_t k = k0 + 2 * l16; k < k1; k += 32) {
+                if (k + 1 < k1) {
+                    const pdv2 v = *reinterpret_cast<const pdv2*>(xr + k);
+                    mx = fmax(mx, fmax(fabs(v.x), fabs(v.y)));
+                } else {
+                    mx = fmax(mx, fabs(xr[k]));
+                }
+            }
+#pragma unroll
+            for (int off = 8; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off));
+            if (l16 == 0) part[blockIdx.y * rows + r] = mx;
         }
     } else {
         const int lane = threadIdx.x & 63;
@@ -821,6 +849,25 @@ __global__ void crt_partial_bound_kernel(const double* __restrict__ x, int64_t r
             }
             for (; k < k1; ++k) s0 += crt_bound_term(xr[k * ks], er);
             part[blockIdx.y * rows + r] = (s0 + s1) + (s2 + s3);
+        }
+    } else if (pair_rows(x, rs)) {
+        const int l16 = threadIdx.x & 15;  // as oz_partial_max_kernel
+        const int64_t groups = (int64_t)gridDim.x * (blockDim.x >> 4);
+        for (int64_t r = blockIdx.x * (int64_t)(blockDim.x >> 4) + (threadIdx.x >> 4); r < rows; r += groups) {
+            const double* xr = x + r * rs;
+            const int er = e[r];
+            uint64_t sum = 0;
+            for (int64_t k = k0 + 2 * l16; k < k1; k += 32) {
+                if (k + 1 < k1) {
+                    const pdv2 v = *reinterpret_cast<const pdv2*>(xr + k);
+                    sum += crt_bound_term(v.x, er) + crt_bound_term(v.y, er);
+                } else {
+                    sum += crt_bound_term(xr[k], er);
+                }
+            }
+#pragma unroll
+            for (int off = 8; off > 0; off >>= 1) sum += __shfl_xor(sum, off);
+            if (l16 == 0) part[blockIdx.y * rows + r] = sum;
         }
     } else {
         const int lane = threadIdx.x & 63;
@@ -1165,19 +1212,20 @@ int dg_ozaki_dgemm(char opA, char opB, int64_t m, int64_t n, int64_t k, double a
 
     // Row exponents of op(A), column exponents of op(B).
     {
-        // ks == 1: a wave per row (NT/64 rows per block); otherwise a thread per row
-        auto pm_grid = [](int64_t rows, int64_t ks) {
-            const int64_t per = ks == 1 ? NT / 64 : NT;
+        // ks == 1: 16 lanes per row when the rows pair up (NT/16 rows per block), else
+        // a wave per row (NT/64); otherwise a thread per row
+        auto pm_grid = [](int64_t rows, int64_t ks, const double* x, int64_t rs) {
+            const int64_t per = ks == 1 ? (pair_rows(x, rs) ? NT / 16 : NT / 64) : NT;
             const int64_t g = (rows + per - 1) / per;
             return (unsigned)(g < 65535 ? g : 65535);
         };
-        unsigned gx = pm_grid(m, a_ks);
+        unsigned gx = pm_grid(m, a_ks, A, a_rs);
         hipLaunchKernelGGL(oz_partial_max_kernel, dim3(gx, gy), dim3(NT), 0, st, A, m, k, a_rs, a_ks, p.kchunk_max, part);
         DG_LAUNCH_CHECK();
         hipLaunchKernelGGL(oz_exponent_kernel, dim3((unsigned)((p.Mp + NT - 1) / NT)), dim3(NT), 0, st, part, m, p.Mp,
                            nchunk, ea);
         DG_LAUNCH_CHECK();
-        gx = pm_grid(n, b_ks);
+        gx = pm_grid(n, b_ks, B, b_rs);
         hipLaunchKernelGGL(oz_partial_max_kernel, dim3(gx, gy), dim3(NT), 0, st, B, n, k, b_rs, b_ks, p.kchunk_max, part);
         DG_LAUNCH_CHECK();
         hipLaunchKernelGGL(oz_exponent_kernel, dim3((unsigned)((p.Np + NT - 1) / NT)), dim3(NT), 0, st, part, n, p.Np,
@@ -1186,12 +1234,12 @@ int dg_ozaki_dgemm(char opA, char opB, int64_t m, int64_t n, int64_t k, double a
         if (crt) {
             // line shifts from the Cauchy-Schwarz bound (the chunk buffer now holds bound sums)
             uint64_t* bp = reinterpret_cast<uint64_t*>(part);
-            gx = pm_grid(m, a_ks);
+            gx = pm_grid(m, a_ks, A, a_rs);
             hipLaunchKernelGGL(crt_partial_bound_kernel, dim3(gx, gy), dim3(NT), 0, st, A, m, k, a_rs, a_ks, p.kchunk_max,
                                ea, bp);
             hipLaunchKernelGGL(crt_shift_kernel, dim3((unsigned)((p.Mp + NT - 1) / NT)), dim3(NT), 0, st, bp, m, p.Mp,
                                nchunk, q, ea);
-            gx = pm_grid(n, b_ks);
+            gx = pm_grid(n, b_ks, B, b_rs);
             hipLaunchKernelGGL(crt_partial_bound_kernel, dim3(gx, gy), dim3(NT), 0, st, B, n, k, b_rs, b_ks, p.kchunk_max,
                                eb, bp);
             hipLaunchKernelGGL(crt_shift_kernel, dim3((unsigned)((p.Np + NT - 1) / NT)), dim3(NT), 0, st, bp, n, p.Np,
