@@ -10,8 +10,10 @@ local GEMM, all 32 panels, one launch).
 
 Prints one JSON line.  `roofline` is the dominant kernel against the FP64
 matrix peak (78.6 TFLOP/s; the FP32 matrix peak, 157.3, for sgemm), timed with HIP events on the launch stream;
-`cpu_baseline` is the host BLAS dgemm of the oracle on a bounded sample of
-the same work, on this host's cores.
+`hbm` times the HBM-bound pack/unpack, copy, layer-reduction and
+generation kernels at the configurations' shapes against the 8 TB/s peak;
+`cpu_baseline` is the host BLAS dgemm of the oracle on the workload's own
+global product and data, on this host's cores (median of 3).
 """
 import argparse
 import json
@@ -28,7 +30,7 @@ HBM_PEAK = 8000.0  # GB/s
 
 WORKLOADS = {
     "summa": "FP64 SUMMA 4x4 virtual grid, M=N=K=16384, block-cyclic nb=512, alpha=1, beta=0.5",
-    "cannon": "FP64 2.5D Cannon 4x4x2 virtual grid, M=16384 N=8192 K=32768, opA=N opB=N, layer reduce in HBM",
+    "cannon": "FP64 2.5D Cannon 4x4x2 virtual grid, M=16384 N=8192 K=32768, opA=N opB={opB}, layer reduce in HBM",
     "dgemm": "FP64 DGEMM NN M=N=K=16384, alpha=1, beta=0",
     "grouped": "FP64 grouped batch: 7x5 uneven tiles of M=N=K=20000 + 256 GEMMs with log-uniform M,N,K in "
                "[64,4096], one autotuned plan replayed as a HIP graph",
@@ -43,8 +45,10 @@ def parse():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", choices=sorted(WORKLOADS), default="summa")
+    ap.add_argument("--opB", choices=["N", "T"], default="N",
+                    help="cannon workload: B stored K x N (N) or N x K with ldb = N (T)")
+    ap.add_argument("--no-hbm", action="store_true", help="skip the HBM leg (pack, reduce, copy, generate)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="budget of the CPU baseline sample")
     ap.add_argument("--ozaki-slices", default="5,6,7,8,9",
                     help="INT8 slice counts of the Ozaki leg (empty: skip the leg)")
     ap.add_argument("--ozaki-bf16-slices", default="11", help="BF16 slice counts of the Ozaki leg")
@@ -66,7 +70,7 @@ def load_traffic(kernel_tag):
         return None
 
 
-def setup(workload, dev):
+def setup(workload, dev, opB="N"):
     import torch
     import dgamd
 
@@ -93,7 +97,8 @@ def setup(workload, dev):
     if workload == "cannon":
         M, N, K, p, c = 16384, 8192, 32768, 4, 2
         bufs = []
-        for seed, (r, cc) in zip((1, 2, 3), ((M, K), (K, N), (M, N))):
+        shapes = ((M, K), (K, N) if opB == "N" else (N, K), (M, N))
+        for seed, (r, cc) in zip((1, 2, 3), shapes):
             G = torch.empty(r * cc, **f64)
             dgamd.generate_t(seed, dgamd.DG_DIST_UNIFORM_PM1, G)
             Pk = torch.empty_like(G)
@@ -104,10 +109,11 @@ def setup(workload, dev):
         ws = torch.empty(dgamd.cannon25d_workspace(M, N, p, c), **f64)
 
         def step():
-            dgamd.cannon25d_t("N", "N", M, N, K, 1.0, Ap, Bp, 0.0, Cp, p, c, dgamd.DG_SCHED_AUTO, ws)
+            dgamd.cannon25d_t("N", opB, M, N, K, 1.0, Ap, Bp, 0.0, Cp, p, c, dgamd.DG_SCHED_AUTO, ws)
 
-        cfg = {"workload": WORKLOADS[workload], "M": M, "N": N, "K": K, "grid": "4x4x2", "alpha": 1.0, "beta": 0.0}
-        return step, (M, N, K), cfg, "cannon16384x8192x32768"
+        cfg = {"workload": WORKLOADS[workload].format(opB=opB), "M": M, "N": N, "K": K, "grid": "4x4x2",
+               "alpha": 1.0, "beta": 0.0, "opA": "N", "opB": opB}
+        return step, (M, N, K), cfg, "cannon16384x8192x32768" + ("nt" if opB == "T" else "")
     if workload == "grouped":
         import numpy as np
         n = 20000
@@ -287,41 +293,141 @@ def grouped_ozaki_leg(step, reps=3):
             "max_rel_err_vs_fp64": float("%.3e" % err)}
 
 
-def cpu_baseline(budget_s, single=False):
-    """Host BLAS dgemm (the oracle's numpy/OpenBLAS; sgemm with `single`) on
-    4096^3 products, repeated for about `budget_s` seconds;
-    TFLOP/s = 2*4096^3*reps / time."""
+DATA = {
+    "summa": "synthetic: SplitMix64-seeded uniform[-1,1) A, B, C (seeds 1, 2, 3), generated and packed on the GPU",
+    "cannon": "synthetic: SplitMix64-seeded uniform[-1,1) A, B, C (seeds 1, 2, 3), generated and packed on the GPU",
+    "dgemm": "synthetic: SplitMix64-seeded uniform[-1,1) A, B (seeds 1, 2), generated on the GPU",
+    "grouped": "synthetic: SplitMix64-seeded uniform[-1,1); 20000^2 A, B seeds 21, 22; extra problem t seeds "
+               "100+3t, 101+3t; shapes from numpy default_rng(5)",
+    "cannon2x2": "synthetic: SplitMix64-seeded uniform[-1,1) A, B (seeds 1, 2), generated and packed on the GPU",
+    "sgemm": "synthetic: SplitMix64-seeded uniform[-1,1) A, B (seeds 1, 2) rounded to FP32, generated on the GPU",
+}
+
+# The host product each workload's CPU baseline times: the workload's own
+# global product, on the same generated data (the oracle's generator is the
+# GPU's, bit for bit): (opB, M, N, K, seeds of A and B, beta with C's seed).
+CPU_PRODUCT = {
+    "summa": ("N", 16384, 16384, 16384, (1, 2), (0.5, 3)),
+    "dgemm": ("N", 16384, 16384, 16384, (1, 2), (0.0, None)),
+    "cannon": (None, 16384, 8192, 32768, (1, 2), (0.0, None)),
+    "cannon2x2": ("N", 2048, 2048, 2048, (1, 2), (0.0, None)),
+    "grouped": ("N", 20000, 20000, 20000, (21, 22), (0.0, None)),
+    "sgemm": ("N", 16384, 16384, 16384, (1, 2), (0.0, None)),
+}
+
+
+def cpu_baseline(workload, opB="N", runs=3):
+    """Host BLAS dgemm (the oracle's numpy BLAS; sgemm for the FP32
+    workload) on the workload's own global product and data: the operands
+    come from the oracle's C restatement of the generator (the GPU's values
+    bit for bit), one warm-up product of 2048^3, then the median of `runs`
+    timed products.  For the grouped batch the sample is the 7x5 grid's
+    product (20000^3), without the 256 extra problems."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import numpy as np
-    import oracle  # noqa: F401  (the checker's host dgemm is numpy's BLAS)
+    import oracle
 
-    threads = None
+    ob, M, N, K, (sa, sb), (beta, sc) = CPU_PRODUCT[workload]
+    ob = ob or opB
+    single = workload == "sgemm"
+    dt = np.float32 if single else np.float64
+    info = {}
     try:
         from threadpoolctl import threadpool_info
-        for info in threadpool_info():
-            if info.get("user_api") == "blas":
-                threads = info.get("num_threads")
+        for i in threadpool_info():
+            if i.get("user_api") == "blas":
+                info = i
                 break
     except Exception:
         pass
-    n = 4096
-    rng = np.random.default_rng(1)
-    dt = np.float32 if single else np.float64
-    a = rng.uniform(-1, 1, (n, n)).astype(dt)
-    b = rng.uniform(-1, 1, (n, n)).astype(dt)
-    c = np.empty((n, n), dtype=dt)
-    np.matmul(a, b, out=c)  # warm-up
-    reps, t0 = 0, time.perf_counter()
-    while True:
-        np.matmul(a, b, out=c)
-        reps += 1
-        el = time.perf_counter() - t0
-        if el >= budget_s:
-            break
-    tf = 2.0 * n ** 3 * reps / el / 1e12
-    return {"value": round(tf, 4), "unit": "TFLOP/s", "cores": threads or os.cpu_count(), "kind": "port",
-            "sample": "%d host-BLAS (numpy OpenBLAS) %s calls of 4096x4096x4096 in %.1f s; "
-                      "the 16384^3 step is 64 such products" % (reps, "sgemm" if single else "dgemm", el)}
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    # column-major flat arrays; .reshape(cols, rows).T is the matrix (a view)
+    a = oracle.generate(sa, 0, M * K).astype(dt).reshape(K, M).T
+    b = oracle.generate(sb, 0, K * N).astype(dt)
+    # op(B) as K x N: the stored K x N matrix (N), or the transpose of the stored N x K one (T)
+    b = b.reshape(N, K).T if ob == "N" else b.reshape(K, N)
+    c = oracle.generate(sc, 0, M * N).astype(dt).reshape(N, M).T if sc is not None else None
+    w = np.ones((2048, 2048), dtype=dt)
+    w @ w  # warm-up: thread pool and code paths
+    times = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        out = a @ b
+        if c is not None:
+            out += dt(beta) * c
+        times.append(time.perf_counter() - t0)
+        del out
+    t = sorted(times)[len(times) // 2]
+    lib = "%s %s" % (info.get("internal_api", "numpy BLAS"), info.get("version", "")) if info else "numpy BLAS"
+    return {"value": round(2.0 * M * N * K / t / 1e12, 4), "unit": "TFLOP/s",
+            "cores": info.get("num_threads") or os.cpu_count(), "kind": "port",
+            "nproc": os.cpu_count(), "cpu_model": model, "library": lib.strip(),
+            "sample": "host %s %s%s %dx%dx%d on the workload's generated data (seeds %d, %d%s), median of %d "
+                      "(%s s)" % ("sgemm" if single else "dgemm", "N", ob, M, N, K, sa, sb,
+                                   ", C seed %d, beta %g" % (sc, beta) if sc is not None else "", runs,
+                                   ", ".join("%.2f" % x for x in times))}
+
+
+def hbm_leg(dev, reps=5):
+    """The HBM-bound kernels at the configurations' shapes, each timed with
+    HIP events on the launch stream over `reps` calls after one warm-up:
+    bytes moved per call (16 B per element copied, 8 (c+1) B per element of a
+    (c-1)-partial layer reduction, 8 B per generated element), GB/s and the
+    fraction of the 8 TB/s HBM peak."""
+    import torch
+    import dgamd
+    f64 = dict(dtype=torch.float64, device=dev)
+    out = {}
+
+    def timed(name, bytes_moved, fn, shape):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        t = e0.elapsed_time(e1) / 1e3 / reps
+        gbs = bytes_moved / t / 1e9
+        out[name] = {"shape": shape, "ms": round(t * 1e3, 4), "bytes": int(bytes_moved), "GB_s": round(gbs, 1),
+                     "frac": round(gbs / HBM_PEAK, 4)}
+
+    n, nb, P = 16384, 512, 4
+    G = torch.empty(n * n, **f64)
+    Pk = torch.empty(n * n, **f64)
+    timed("generate", 8 * n * n, lambda: dgamd.generate_t(1, dgamd.DG_DIST_UNIFORM_PM1, G), "16384^2 FP64")
+    timed("pack_block_cyclic", 16 * n * n, lambda: dgamd.pack_block_cyclic_t(G, n, n, n, nb, nb, P, P, Pk),
+          "16384^2, 512x512 blocks on 4x4")
+    timed("unpack_block_cyclic", 16 * n * n, lambda: dgamd.unpack_block_cyclic_t(Pk, nb, nb, P, P, G, n, n, n),
+          "16384^2, 512x512 blocks on 4x4")
+    timed("copy_block", 16 * (n - 8) * (n - 8),
+          lambda: dgamd.copy_block_t(n - 8, n - 8, G, n, Pk, n), "16376^2 of ld 16384")
+    del G, Pk
+    M, N, K, p = 16384, 8192, 32768, 4
+    G = torch.empty(M * K, **f64)
+    Pk = torch.empty(M * K, **f64)
+    dgamd.generate_t(1, dgamd.DG_DIST_UNIFORM_PM1, G)
+    ro, ko = dgamd.block_splits(M, p), dgamd.block_splits(K, p)
+    timed("pack_block2d", 16 * M * K, lambda: dgamd.pack_block2d_t(G, M, M, K, ro, ko, Pk), "16384x32768 on 4x4")
+    timed("unpack_block2d", 16 * M * K, lambda: dgamd.unpack_block2d_t(Pk, ro, ko, G, M, M, K),
+          "16384x32768 on 4x4")
+    del G, Pk
+    C = torch.empty(M * N, **f64)
+    W = torch.empty(M * N, **f64)
+    dgamd.generate_t(3, dgamd.DG_DIST_UNIFORM_PM1, C)
+    dgamd.generate_t(4, dgamd.DG_DIST_UNIFORM_PM1, W)
+    timed("layer_reduce", 8 * 3 * M * N, lambda: dgamd.layer_reduce_t(C, W, M * N, 1),
+          "16384x8192, c=2 (one partial layer)")
+    return out
 
 
 def main():
@@ -337,7 +443,7 @@ def main():
     dgamd.lib()
     if dgamd.lib().dg_device_ok() != 1:
         raise SystemExit("no gfx950 device: " + dgamd.last_error())
-    step, (M, N, K), cfg, tag = setup(args.workload, dev)
+    step, (M, N, K), cfg, tag = setup(args.workload, dev, args.opB)
     flops = 2.0 * M * N * K
     fp32 = args.workload == "sgemm"
     peak = FP32_PEAK if fp32 else FP64_PEAK
@@ -373,22 +479,25 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32" if fp32 else "f64",
-        "data": "synthetic: SplitMix64-seeded uniform[-1,1) (seeds 1,2,3), generated and packed on the GPU",
+        "data": DATA[args.workload],
         "config": cfg,
         "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
                      "frac": round(achieved / peak, 4), "traffic": load_traffic(tag)},
     }
     if args.workload == "grouped":
         out["ozaki_form"] = grouped_ozaki_leg(step)
+    del step  # the workload's buffers
+    torch.cuda.empty_cache()
     int8_s = [int(x) for x in args.ozaki_slices.split(",") if x.strip()]
     bf16_s = [int(x) for x in args.ozaki_bf16_slices.split(",") if x.strip()]
     crt_m = [int(x) for x in args.ozaki_crt_moduli.split(",") if x.strip()]
     if int8_s or bf16_s or crt_m:
-        del step
-        torch.cuda.empty_cache()
         out["ozaki"] = ozaki_leg(dev, int8_s, bf16_s, crt_m)
+        torch.cuda.empty_cache()
+    if not args.no_hbm:
+        out["hbm"] = hbm_leg(dev)
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(args.cpu_seconds, single=fp32)
+        out["cpu_baseline"] = cpu_baseline(args.workload, args.opB)
     print(json.dumps(out), flush=True)
 
 

@@ -66,7 +66,31 @@ void test_cannon(char ta, char tb, long long M, long long N, long long K, int p,
     const double err = relativeFrobeniusError(ref.data(), out.data(), M * N);
     char name[96];
     snprintf(name, sizeof(name), "Summa25Decomposer %c%c %lldx%lldx%lld on %dx%dx%d", ta, tb, M, N, K, p, p, c);
-    report(name, err <= 1e-13 && Dtest_equality(ref.data(), out.data(), M * N) >= 0, err);
+    report(name, err <= 1e-13, err);
+}
+
+// The reference's digit-count check on a GPU result whose entries are bounded
+// away from zero (A, B in [0.5, 1], beta = 0: every C entry is a sum of K
+// products in [0.25, 1]), against the long-double host product: the FP64
+// result agrees to at least 13 digits everywhere.
+void test_validation_digits() {
+    const long long M = 256, N = 192, K = 320;
+    std::vector<double> A(M * K), B(K * N), C(M * N, 0.0), ref(M * N, 0.0);
+    setMatrixSeed(13);
+    MatrixInit(A.data(), M * K, -1);
+    MatrixInit(B.data(), K * N, -1);
+    for (double& x : A) x = 0.75 + 0.25 * x;  // [-1, 1) -> [0.5, 1)
+    for (double& x : B) x = 0.75 + 0.25 * x;
+    host_gemm('N', 'N', M, N, K, 1.0, A.data(), M, B.data(), K, 0.0, ref.data(), M);
+    Summa25Decomposer dec(M, N, K, 2, 2, 1, false);
+    dec.decompose2D(0, A.data(), B.data(), C.data(), dec.localA, dec.localB, dec.localC);
+    dec.multiply('N', 'N', A.data(), B.data(), C.data(), 1.0, 0.0);
+    std::vector<double> out(M * N, 0.0);
+    dec.reorderMatrix(0, out.data(), dec.localC);
+    const short digits = Dtest_equality(ref.data(), out.data(), M * N);
+    const double err = relativeFrobeniusError(ref.data(), out.data(), M * N);
+    printf("Dtest_equality digits on the 2x2x1 Cannon result: %d\n", (int)digits);
+    report("Dtest_equality >= 13 digits on a GPU result", digits >= 13 && err <= 1e-14, err);
 }
 
 void test_full_offload_driver() {
@@ -105,6 +129,34 @@ void test_pblas() {
         }
     HIP_CHECK(hipFree(packed));
     report("pblasDecomposer scatter/gather 203x117, 16x8 on 3x2", ok, 0.0);
+}
+
+// Row-major globals (ld = columns), as the reference's colMajor = false
+// branch: C = alpha A B + beta C with every piece row-major.
+void test_block_sequential_row_major() {
+    const long long M = 230, N = 170, K = 70;
+    BlockSequentialDecomposer dec(M, N, K, 6, false);
+    std::vector<double> A(M * K), B(K * N), C(M * N), ref;
+    setMatrixSeed(17);
+    MatrixInit(A.data(), M * K, -1);
+    MatrixInit(B.data(), K * N, -1);
+    MatrixInit(C.data(), M * N, -1);
+    ref = C;
+    // row-major C = A B is column-major C^T = B^T A^T
+    host_gemm('N', 'N', N, M, K, 0.5, B.data(), N, A.data(), K, -1.0, ref.data(), N);
+    dec.deliverMatrix(A.data(), B.data(), C.data());
+    // rank (1, 1)'s C tile is row-major with ld = its columns
+    std::vector<double> tile(dec.tileRows(1) * dec.tileColumns(1));
+    HIP_CHECK(hipMemcpy(tile.data(), dec.blockC(1, 1), sizeof(double) * tile.size(), hipMemcpyDeviceToHost));
+    bool ok = dec.dRow == 3 && dec.dCol == 2;
+    const long long r0 = M / 3, c0 = N / 2, tn = dec.tileColumns(1);
+    for (long long r = 0; r < dec.tileRows(1) && ok; ++r)
+        for (long long c = 0; c < tn && ok; ++c) ok = tile[r * tn + c] == C[(r0 + r) * N + c0 + c];
+    dec.multiply(0.5, -1.0);
+    std::vector<double> out(M * N);
+    dec.gatherResult(0, out.data());
+    const double err = relativeFrobeniusError(ref.data(), out.data(), M * N);
+    report("BlockSequentialDecomposer row-major 230x170x70 on 3x2", ok && err <= 1e-13, err);
 }
 
 void test_block_sequential() {
@@ -221,6 +273,8 @@ int main() {
     test_full_offload_driver();
     test_pblas();
     test_block_sequential();
+    test_block_sequential_row_major();
+    test_validation_digits();
     test_grid_layout();
     test_validation_and_errors();
     test_generation();

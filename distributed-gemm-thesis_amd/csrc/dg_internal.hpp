@@ -35,8 +35,22 @@ inline char norm_op(char op) {
 // Identical tables (same bytes) share one device buffer, so repeated calls
 // with the same arguments upload nothing.  Returns nullptr and sets the
 // error on failure.  Uploading while `stream` is being captured into a HIP
-// graph is refused: tables must be built before capture.
+// graph is refused: tables must be built before capture; a table looked up
+// during a capture stays resident for the life of the process (the graph
+// holds its address).  Beyond 256 tables per device, a new table reuses the
+// buffer of the least recently used one whose last use has completed on the
+// device, found by event queries (nothing waits, nothing is freed).
 const void* cached_table(const void* host, size_t bytes, hipStream_t stream);
+
+// Marks the kernels that read this thread's tables looked up since the last
+// mark as enqueued on `stream`: records the completion event that a later
+// reuse of those buffers waits for.  TableUseScope calls it when a call that
+// looks up tables returns, after its launches.
+void table_uses_enqueued(hipStream_t stream);
+struct TableUseScope {
+    hipStream_t stream;
+    ~TableUseScope() { table_uses_enqueued(stream); }
+};
 
 #define DG_HIP(call)                                                            \
     do {                                                                        \

@@ -12,6 +12,8 @@
 //   * SUMMA panel broadcasts: panel t of A is read from process column t mod Q
 //     and panel t of B from process row t mod P.
 // All problems of one call run in a single launch.
+#include <algorithm>
+
 #include "dg_internal.hpp"
 
 namespace dg {
@@ -156,8 +158,6 @@ int dg_summa(char opA, char opB, int64_t M, int64_t N, int64_t K, double alpha, 
     using namespace dg;
     const char oa = norm_op(opA), ob = norm_op(opB);
     if (!oa || !ob) return fail(DG_ERR_INVALID, "dg_summa: invalid op '%c%c'", opA, opB);
-    if (oa != 'N' || ob != 'N')
-        return fail(DG_ERR_UNSUPPORTED, "dg_summa: only opA = opB = 'N' on block-cyclic operands");
     if (M < 0 || N < 0 || K < 0) return fail(DG_ERR_INVALID, "dg_summa: negative dimension");
     if (mb <= 0 || nb <= 0 || kb <= 0) return fail(DG_ERR_INVALID, "dg_summa: block sizes must be positive");
     if (P < 1 || Q < 1 || P > 256 || Q > 256) return fail(DG_ERR_INVALID, "dg_summa: grid %dx%d", P, Q);
@@ -176,15 +176,19 @@ int dg_summa(char opA, char opB, int64_t M, int64_t N, int64_t K, double alpha, 
         for (int x = 0; x < np; ++x) { loc[x] = dg_numroc(n, b, x, 0, np); pre[x] = acc; acc += loc[x]; }
         pre[np] = acc;
     };
+    // Stored operands: A is M x K (opA = N) or K x M (opA = T) with blocks
+    // mb x kb or kb x mb; B is K x N or N x K with blocks kb x nb or nb x kb;
+    // all distributed block-cyclically over the same P x Q grid.
     std::vector<int64_t> rA, rApre, cA, cApre, rB, rBpre, cB, cBpre, rC, rCpre, cC, cCpre;
-    extents(M, mb, P, rA, rApre);
-    extents(K, kb, Q, cA, cApre);
-    extents(K, kb, P, rB, rBpre);
-    extents(N, nb, Q, cB, cBpre);
+    if (oa == 'N') { extents(M, mb, P, rA, rApre); extents(K, kb, Q, cA, cApre); }
+    else           { extents(K, kb, P, rA, rApre); extents(M, mb, Q, cA, cApre); }
+    if (ob == 'N') { extents(K, kb, P, rB, rBpre); extents(N, nb, Q, cB, cBpre); }
+    else           { extents(N, nb, P, rB, rBpre); extents(K, kb, Q, cB, cBpre); }
     extents(M, mb, P, rC, rCpre);
     extents(N, nb, Q, cC, cCpre);
-    auto baseA = [&](int pr, int pc) { return rApre[pr] * K + rA[pr] * cApre[pc]; };
-    auto baseB = [&](int pr, int pc) { return rBpre[pr] * N + rB[pr] * cBpre[pc]; };
+    const int64_t colsA = oa == 'N' ? K : M, colsB = ob == 'N' ? N : K;
+    auto baseA = [&](int pr, int pc) { return rApre[pr] * colsA + rA[pr] * cApre[pc]; };
+    auto baseB = [&](int pr, int pc) { return rBpre[pr] * colsB + rB[pr] * cBpre[pc]; };
     auto baseC = [&](int pr, int pc) { return rCpre[pr] * N + rC[pr] * cCpre[pc]; };
 
     if (K == 0 || alpha == 0.0) {
@@ -195,34 +199,68 @@ int dg_summa(char opA, char opB, int64_t M, int64_t N, int64_t K, double alpha, 
             }
         return DG_OK;
     }
+    // One problem per rank's local C.  A transposed operand's panel is not one
+    // block of one owner (the global row block ib of op(A) is the stored
+    // column block ib, owned by process column ib % Q), so with opA = T the
+    // local C splits into its row blocks, and with opB = T into its column
+    // blocks; each piece then reads one block of one owner per panel.
+    // Every element still sums the panels in order 0, 1, ...
     const int64_t npanels = (K + kb - 1) / kb;
     std::vector<Prob> probs;
     std::vector<Seg> segs;
     for (int pr = 0; pr < P; ++pr)
         for (int pc = 0; pc < Q; ++pc) {
             if (rC[pr] == 0 || cC[pc] == 0) continue;
-            Prob p{};
-            p.m = rC[pr];
-            p.n = cC[pc];
-            p.c = C_packed + baseC(pr, pc);
-            p.ldc = rC[pr];
-            p.alpha = alpha;
-            p.beta = beta;
-            p.seg0 = (int32_t)segs.size();
-            p.nseg = (int32_t)npanels;
-            for (int64_t t = 0; t < npanels; ++t) {
-                const int qa = (int)(t % Q), pb = (int)(t % P);
-                Seg s{};
-                s.a = A_packed + baseA(pr, qa) + (t / Q) * kb * rA[pr];
-                s.lda = rA[pr];
-                s.b = B_packed + baseB(pb, pc) + (t / P) * kb;
-                s.ldb = rB[pb];
-                s.klen = (K - t * kb) < kb ? (K - t * kb) : kb;
-                segs.push_back(s);
-            }
-            probs.push_back(p);
+            const int64_t nmb = oa == 'N' ? 1 : (rC[pr] + mb - 1) / mb;  // row pieces
+            const int64_t nnb = ob == 'N' ? 1 : (cC[pc] + nb - 1) / nb;  // column pieces
+            for (int64_t jm = 0; jm < nmb; ++jm)
+                for (int64_t jn = 0; jn < nnb; ++jn) {
+                    const int64_t r0 = oa == 'N' ? 0 : jm * mb, c0 = ob == 'N' ? 0 : jn * nb;
+                    const int64_t rows = oa == 'N' ? rC[pr] : std::min<int64_t>(mb, rC[pr] - r0);
+                    const int64_t cols = ob == 'N' ? cC[pc] : std::min<int64_t>(nb, cC[pc] - c0);
+                    const int64_t ib = (int64_t)pr + jm * P;   // global row block of op(A) (opA = T)
+                    const int64_t jb = (int64_t)pc + jn * Q;   // global column block of op(B) (opB = T)
+                    Prob p{};
+                    p.m = rows;
+                    p.n = cols;
+                    p.c = C_packed + baseC(pr, pc) + r0 + c0 * rC[pr];
+                    p.ldc = rC[pr];
+                    p.alpha = alpha;
+                    p.beta = beta;
+                    p.seg0 = (int32_t)segs.size();
+                    p.nseg = (int32_t)npanels;
+                    for (int64_t t = 0; t < npanels; ++t) {
+                        Seg s{};
+                        if (oa == 'N') {
+                            // local rows of rank (pr, t % Q), its local column block t / Q
+                            const int qa = (int)(t % Q);
+                            s.a = A_packed + baseA(pr, qa) + (t / Q) * kb * rA[pr];
+                            s.lda = rA[pr];
+                        } else {
+                            // stored row block t (owner row t % P), column block ib (owner column ib % Q)
+                            const int pa = (int)(t % P), qa = (int)(ib % Q);
+                            s.a = A_packed + baseA(pa, qa) + (t / P) * kb + (ib / Q) * mb * rA[pa];
+                            s.lda = rA[pa];
+                        }
+                        if (ob == 'N') {
+                            const int pb = (int)(t % P);
+                            s.b = B_packed + baseB(pb, pc) + (t / P) * kb;
+                            s.ldb = rB[pb];
+                        } else {
+                            // stored row block jb (owner row jb % P), column block t (owner column t % Q)
+                            const int pb = (int)(jb % P), qb = (int)(t % Q);
+                            s.b = B_packed + baseB(pb, qb) + (jb / P) * nb + (t / Q) * kb * rB[pb];
+                            s.ldb = rB[pb];
+                        }
+                        s.klen = (K - t * kb) < kb ? (K - t * kb) : kb;
+                        segs.push_back(s);
+                    }
+                    probs.push_back(p);
+                }
         }
-    return launch_gemm_f64('N', 'N', probs, segs, st, DG_GEMM_MODE_XCD);
+    if (probs.size() > (size_t)INT32_MAX || segs.size() > (size_t)INT32_MAX)
+        return fail(DG_ERR_UNSUPPORTED, "dg_summa: too many blocks");
+    return launch_gemm_f64(oa, ob, probs, segs, st, DG_GEMM_MODE_XCD);
 }
 
 }  // extern "C"

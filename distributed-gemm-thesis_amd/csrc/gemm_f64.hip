@@ -1070,6 +1070,7 @@ int launch_gemm_f64(char opA, char opB, const std::vector<Prob>& probs_in, const
     if (rc) return rc;
     if (hb.dma.empty() && hb.gen.empty()) return DG_OK;
     GemmBatchDev b;
+    TableUseScope uses{stream};  // after the launch below returns
     auto table = [&](const auto& v, auto** dst) -> bool {
         using T = typename std::decay_t<decltype(v)>::value_type;
         *dst = nullptr;

@@ -31,7 +31,20 @@ __device__ __forceinline__ double generated_value(uint64_t seed, int dist, int64
     return v;
 }
 
+typedef double gdv2 __attribute__((ext_vector_type(2)));
+
+// Two consecutive elements per thread and pass, one 16-byte store (the
+// buffer is 16-byte aligned: checked by the host); an odd last element is
+// written by thread 0.
 __global__ void generate_kernel(uint64_t seed, int dist, double* __restrict__ out, int64_t n) {
+    const int64_t np = n / 2;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < np; i += (int64_t)gridDim.x * blockDim.x)
+        *reinterpret_cast<gdv2*>(out + 2 * i) = gdv2{generated_value(seed, dist, 2 * i), generated_value(seed, dist, 2 * i + 1)};
+    if ((n & 1) && blockIdx.x == 0 && threadIdx.x == 0) out[n - 1] = generated_value(seed, dist, n - 1);
+}
+
+// Unaligned buffers: one element per thread and pass.
+__global__ void generate_unaligned_kernel(uint64_t seed, int dist, double* __restrict__ out, int64_t n) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         out[i] = generated_value(seed, dist, i);
 }
@@ -58,10 +71,16 @@ static int dg_generate_common(uint64_t seed, int dist, T* ptr, int64_t n, void* 
     if (rc != DG_OK) return rc;
     int64_t blocks = (n + 255) / 256;
     if (blocks > 8192) blocks = 8192;
-    if constexpr (sizeof(T) == 8)
-        hipLaunchKernelGGL(generate_kernel, dim3((unsigned)blocks), dim3(256), 0, as_stream(stream), seed, dist, ptr, n);
-    else
+    if constexpr (sizeof(T) == 8) {
+        if (reinterpret_cast<uintptr_t>(ptr) & 15)
+            hipLaunchKernelGGL(generate_unaligned_kernel, dim3((unsigned)blocks), dim3(256), 0, as_stream(stream), seed,
+                               dist, ptr, n);
+        else
+            hipLaunchKernelGGL(generate_kernel, dim3((unsigned)blocks), dim3(256), 0, as_stream(stream), seed, dist,
+                               ptr, n);
+    } else {
         hipLaunchKernelGGL(generate_f32_kernel, dim3((unsigned)blocks), dim3(256), 0, as_stream(stream), seed, dist, ptr, n);
+    }
     DG_LAUNCH_CHECK();
     return DG_OK;
 }

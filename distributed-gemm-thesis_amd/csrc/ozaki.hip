@@ -382,9 +382,8 @@ struct CrtTables {
     int32_t n;                          // moduli in use
     int32_t cinv[kMaxModuli];           // M_t^-1 mod p_t
     double mt[kMaxModuli][3];           // M_t in 40-bit limbs (as doubles), least significant first
-    uint32_t P[4];                      // P = prod p_t
-    double Pd;                          // P rounded to double
-    double invPd;                       // 1 / Pd rounded to double
+    double Pl[3];                       // P = prod p_t in 40-bit limbs (as doubles), least significant first
+    double invPd;                       // 1 / fl(P) rounded to double
 };
 
 // p_t as a constant expression, so that x % crt_modulus(t) in an unrolled
@@ -395,28 +394,20 @@ __host__ __device__ constexpr int crt_modulus(int t) {
          : t == 13 ? 199 : t == 14 ? 197 : 193;
 }
 
-// x mod p in [0, p) for an integer-valued double |x| < 2^53 (exact steps).
-__device__ __forceinline__ int mod_exact(double x, int p, double inv_p) {
-    const double q = floor(x * inv_p);
-    double r = fma(-q, (double)p, x);  // exact: q*p and the difference are integers below 2^53
-    if (r < 0) r += p;
-    if (r >= p) r -= p;
-    return (int)r;
-}
-
 // Balanced residues of A' = rint(op(X)(row, k) * 2^(q - e_row)) for every
 // modulus, [t][row][Kp] int8 (zeros in the padding).  A workgroup stages a
 // 16-row x 128-k tile of op(X) through LDS (loaded along whichever direction
 // is contiguous in memory; 16.5 KB, so eight workgroups share a CU and the
 // load and arithmetic phases of different workgroups overlap); each thread
 // then turns 8 consecutive k of one row into one 8-byte store per modulus,
-// so a wave writes whole 128-byte lines.  Residue: q = rint(a/p) from a*fl(1/p), r = a - q*p exact by FMA
-// (|r| <= p/2 + p), then one step into [-floor(p/2), ceil(p/2) - 1].
+// so a wave writes whole 128-byte lines.  Residues in [-floor(p/2),
+// ceil(p/2) - 1] from an exact split of A' (below), five FP64 operations
+// per element and modulus.
 constexpr int RT_R = 16, RT_K = 128, RT_T = 256 / RT_R, RT_KPT = RT_K / RT_T;  // threads per row, k per thread
+template <int NMOD>
 __global__ __launch_bounds__(256) void crt_residue_kernel(const double* __restrict__ x, int64_t rows, int64_t K,
                                                           int64_t rs, int64_t ks, const int32_t* __restrict__ e, int q,
-                                                          int nmod, int8_t* __restrict__ out, int64_t Kp,
-                                                          int64_t slice_bytes) {
+                                                          int8_t* __restrict__ out, int64_t Kp, int64_t slice_bytes) {
     __shared__ double tile[RT_R][RT_K + 1];
     const int64_t r0 = (int64_t)blockIdx.y * RT_R;
     const int64_t k0 = (int64_t)blockIdx.x * RT_K;
@@ -438,25 +429,34 @@ __global__ __launch_bounds__(256) void crt_residue_kernel(const double* __restri
     const int rr = t / RT_T;
     const int kq = (t % RT_T) * RT_KPT;
     const int64_t r = r0 + rr;
-    double a[RT_KPT];
+    // a = ah 2^24 + al exactly (al in [0, 2^24)), so for each modulus
+    // y = ah (2^24 mod p) + al = a (mod p) with |y| < 2^38, whose quotient
+    // estimate fma(y, fl(1/p), 1/2) is within 2^-22 of y/p + 1/2 (never closer
+    // than 1/(2p) to an integer for odd p; exact for p = 256, where y = al):
+    // r = y - p floor(y/p + 1/2) is the balanced residue, computed exactly.
+    double ah[RT_KPT], al[RT_KPT];
     const int sh = r < rows ? q - e[r] : 0;
 #pragma unroll
-    for (int c = 0; c < RT_KPT; ++c) a[c] = r < rows ? rint(ldexp(tile[rr][kq + c], sh)) : 0.0;
+    for (int c = 0; c < RT_KPT; ++c) {
+        const double a = r < rows ? rint(ldexp(tile[rr][kq + c], sh)) : 0.0;
+        ah[c] = floor(a * 0x1p-24);
+        al[c] = fma(-ah[c], 0x1p24, a);
+    }
     int8_t* dst = out + r * Kp + k0 + kq;
 #pragma unroll
-    for (int m = 0; m < kMaxModuli; ++m) {
-        if (m < nmod) {
+    for (int m = 0; m < NMOD; ++m) {
+        {
             constexpr double one = 1.0;
             const int p = crt_modulus(m);
             const double pd = p, inv_p = one / crt_modulus(m);
+            const double cp = (double)((1 << 24) % crt_modulus(m));
             static_assert(RT_KPT == 8, "one 8-byte store per modulus");
             uint32_t w[2] = {0, 0};
 #pragma unroll
             for (int c = 0; c < RT_KPT; ++c) {
-                const double qd = rint(a[c] * inv_p);
-                int v = (int)fma(-qd, pd, a[c]);
-                if (2 * v >= p) v -= p;
-                if (2 * v < -p) v += p;
+                const double y = m == 0 ? al[c] : fma(ah[c], cp, al[c]);
+                const double qd = floor(fma(y, inv_p, 0.5));
+                const int v = (int)fma(-qd, pd, y);
                 w[c >> 2] |= (uint32_t)(v & 255) << (8 * (c & 3));
             }
             *reinterpret_cast<uint2*>(dst + m * slice_bytes) = uint2{w[0], w[1]};
@@ -513,7 +513,7 @@ __device__ __forceinline__ void crt_half(iv4 (&acc)[FM][FN], iv4 (&fa)[FM], cons
 #pragma unroll
     for (int i = 0; i < FM; ++i) {
 #pragma unroll
-        for (int j = 0; j < FN; ++j) acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(cb[j], fa[i], acc[i][j], 0, 0, 0);
+        for (int j = 0; j < FN; ++j) acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa[i], cb[j], acc[i][j], 0, 0, 0);
         if (i == 0) {
 #pragma unroll
             for (int j = 0; j < FN; ++j) nb[j] = *reinterpret_cast<const iv4*>(nbs + 2048 * j);
@@ -603,59 +603,71 @@ __global__ __launch_bounds__(CNT, OCC) void crt_gemm_kernel(CrtArgs g) {
         crt_half(acc, fa, nb, cb, lds + asl1 * CSLOT + wm * CK2 + roff0, lds + bsl1 * CSLOT + wn * CK2 + roff0);
         csl = asl1;
     }
-    // acc[i][j][v] = V_t(m0 + wm + 16i + lr, n0 + wn + 16j + 4lg + v)
+    // acc[i][j][v] = V_t(m0 + wm + 16i + 4lg + v, n0 + wn + 16j + lr): four
+    // consecutive rows of one column, stored as one 32-bit word of digits.
+    // x = acc * cinv is exact (|x| < 2^39) and floor(fl(x fl(1/p) + 2^-10)) is
+    // floor(x/p) exactly (the estimate errs by < 2^-20 and the fractional
+    // part of x/p lies in [0, 1 - 1/p]), so x - p floor(x/p) is the digit.
     const int p = c_moduli[tmod];
-    const int cinv = g.cinv[tmod];
-    const double inv_p = 1.0 / p;
+    const double pd = p, inv_p = 1.0 / p, cinv = g.cinv[tmod];
     uint8_t* out = g.res + (int64_t)tmod * g.res_plane;
 #pragma unroll
-    for (int j = 0; j < FN; ++j)
+    for (int j = 0; j < FN; ++j) {
+        const int64_t n = n0 + wn + 16 * j + lr;
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            const int64_t n = n0 + wn + 16 * j + 4 * lg + v;
+        for (int i = 0; i < FM; ++i) {
+            const int64_t m = m0 + wm + 16 * i + 4 * lg;
+            uint32_t w = 0;
 #pragma unroll
-            for (int i = 0; i < FM; ++i) {
-                const int64_t m = m0 + wm + 16 * i + lr;
-                const int r = mod_exact((double)acc[i][j][v], p, inv_p);
-                out[n * g.Mp + m] = (uint8_t)mod_exact((double)(r * cinv), p, inv_p);
+            for (int v = 0; v < 4; ++v) {
+                const double x = (double)acc[i][j][v] * cinv;
+                const double qd = floor(fma(x, inv_p, 0x1p-10));
+                w |= (uint32_t)(int)fma(-qd, pd, x) << (8 * v);
             }
+            *reinterpret_cast<uint32_t*>(out + n * g.Mp + m) = w;
         }
+    }
 }
 
-// The balanced CRT value of one element, correctly rounded to double, from
-// Le[l] = sum over 32-bit limb positions l of S = sum_t x_t M_t (each Le[l]
-// below 2^40, so the carries fit 64 bits).  S = V + jP with j in [0, 16] and
-// |V| <= P/4 (the scaling bound), so S/P lies within 1/4 of the integer j and
-// j = rint(S * fl(1/P)) exactly (the floating estimate is off by ~1e-14);
-// V = S - jP follows exactly in limbs.  Branch-free.
-__host__ __device__ inline double crt_value(const uint64_t (&Le)[4], const CrtTables& tb) {
-    // carry-propagate into 32-bit limbs s0..s4
-    const uint64_t c0 = Le[0];
-    const uint64_t c1 = Le[1] + (c0 >> 32);
-    const uint64_t c2 = Le[2] + (c1 >> 32);
-    const uint64_t c3 = Le[3] + (c2 >> 32);
-    const uint32_t s0 = (uint32_t)c0, s1 = (uint32_t)c1, s2 = (uint32_t)c2, s3 = (uint32_t)c3;
-    const uint32_t s4 = (uint32_t)(c3 >> 32);
-    const double Sd = (((((double)s4 * 4294967296.0 + (double)s3) * 4294967296.0) + (double)s2) * 4294967296.0 +
-                       (double)s1) * 4294967296.0 + (double)s0;
-    const uint32_t j = (uint32_t)rint(Sd * tb.invPd);
-    // V = S - j P over 32-bit limbs; |V| < 2^127, so limbs 0..3 hold it in two's complement
-    const uint64_t b0 = (uint64_t)j * tb.P[0];
-    const uint64_t b1 = (uint64_t)j * tb.P[1] + (b0 >> 32);
-    const uint64_t b2 = (uint64_t)j * tb.P[2] + (b1 >> 32);
-    const uint64_t b3 = (uint64_t)j * tb.P[3] + (b2 >> 32);
-    const uint64_t lo_s = ((uint64_t)s1 << 32) | s0, hi_s = ((uint64_t)s3 << 32) | s2;
-    const uint64_t lo_b = (b1 << 32) | (uint32_t)b0, hi_b = (b3 << 32) | (uint32_t)b2;
-    uint64_t lo = lo_s - lo_b;
-    uint64_t hi = hi_s - hi_b - (lo_s < lo_b ? 1u : 0u);
+// The balanced CRT value V of one element, correctly rounded to double, from
+// the limb sums L_l = sum_t x_t * limb_l(M_t) over 40-bit limbs (exact
+// doubles below 2^52), S = L0 + L1 2^40 + L2 2^80 = V + jP with j in [0, 16].
+// Since |V| <= P/4 (the scaling bound), S/P lies within 1/4 of the integer j
+// and j = rint(fl(S) fl(1/P)) exactly (the estimate errs by ~2^-47).  Then
+// in FP64, every step exact: D_l = L_l - j P_l (|D_l| < 2^52), and two carries
+// bring D0, D1 into [0, 2^40), so V = D2 2^80 + D1 2^40 + D0 with D2 signed.
+// The three limbs split into 32-bit words that make V in 128-bit two's
+// complement, whose magnitude is rounded to double once: the top 64 bits
+// with a sticky bit for everything below, converted with one rounding.
+__host__ __device__ inline double crt_value(double L0, double L1, double L2, const CrtTables& tb) {
+    const double Sd = fma(L2, 0x1p80, fma(L1, 0x1p40, L0));
+    const double jd = rint(Sd * tb.invPd);
+    double D0 = fma(-jd, tb.Pl[0], L0);
+    double D1 = fma(-jd, tb.Pl[1], L1);
+    double D2 = fma(-jd, tb.Pl[2], L2);
+    double c = floor(D0 * 0x1p-40);
+    D0 = fma(-c, 0x1p40, D0);
+    D1 = D1 + c;
+    c = floor(D1 * 0x1p-40);
+    D1 = fma(-c, 0x1p40, D1);
+    D2 = D2 + c;
+    // 32-bit halves: D0, D1 in [0, 2^40) (high halves below 2^8), D2 signed below 2^44
+    const double h0 = floor(D0 * 0x1p-32), h1 = floor(D1 * 0x1p-32), h2 = floor(D2 * 0x1p-32);
+    const uint32_t lo0 = (uint32_t)fma(-h0, 0x1p32, D0), lo1 = (uint32_t)fma(-h1, 0x1p32, D1);
+    const uint32_t lo2 = (uint32_t)fma(-h2, 0x1p32, D2);
+    const uint32_t hi0 = (uint32_t)h0, hi1 = (uint32_t)h1, hi2 = (uint32_t)(int32_t)h2;
+    const uint32_t w0 = lo0;
+    const uint32_t w1 = hi0 | (lo1 << 8);
+    const uint32_t w2 = (lo1 >> 24) | (hi1 << 8) | (lo2 << 16);
+    const uint32_t w3 = (lo2 >> 16) | (hi2 << 16);
+    uint64_t hi = ((uint64_t)w3 << 32) | w2;
+    uint64_t lo = ((uint64_t)w1 << 32) | w0;
     const bool neg = (int64_t)hi < 0;
     // magnitude
     const uint64_t nlo = ~lo + 1;
     const uint64_t nhi = ~hi + (nlo == 0 ? 1u : 0u);
     lo = neg ? nlo : lo;
     hi = neg ? nhi : hi;
-    // correctly rounded double of the 128-bit magnitude (hi, lo): the top 64
-    // bits with a sticky bit for everything below, converted once
     const int z = hi ? __builtin_clzll(hi) : 64;
     const uint64_t top = z == 64 ? lo : (z ? (hi << z) | (lo >> (64 - z)) : hi);
     const uint64_t rest = z == 64 ? 0 : (z ? (lo << z) : lo);
@@ -666,11 +678,13 @@ __host__ __device__ inline double crt_value(const uint64_t (&Le)[4], const CrtTa
 // Four consecutive rows per thread (one 32-bit load of CRT digits per modulus).
 // The limb sums L_l = sum_t x_t * limb_l(M_t) over 40-bit limbs of M_t are
 // exact FP64 FMAs (x_t < 256, limbs < 2^40, at most 16 terms: below 2^52);
-// they are regrouped into 32-bit limb positions and crt_value reduces
-// S = sum_t x_t M_t < 16 P modulo P with a floating quotient estimate and one
-// correction, balances it and rounds it to double once.
+// crt_value reduces S = sum_t x_t M_t < 16 P modulo P to the balanced V and
+// rounds it to double once.
 // C = alpha * ldexp(V, e_i + f_j - 2q) + beta*C, four rows per 32-byte access
 // when C allows it.
+// NMOD (the moduli count) is a template argument so that every digit load
+// of a thread is issued before the first is used.
+template <int NMOD>
 __global__ void crt_reconstruct_kernel(const uint8_t* __restrict__ res, CrtTables tb, int64_t m, int64_t n,
                                        int64_t Mp, int64_t plane, const int32_t* __restrict__ ea,
                                        const int32_t* __restrict__ eb, int q, double alpha, double beta,
@@ -681,34 +695,30 @@ __global__ void crt_reconstruct_kernel(const uint8_t* __restrict__ res, CrtTable
         const int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;  // one thread per 4 rows
         if (x >= total) return;
         const int64_t j = x / qm, i0 = (x - j * qm) * 4;
+        const uint8_t* rp = res + j * Mp + i0;
+        uint32_t w[NMOD];
+#pragma unroll
+        for (int t = 0; t < NMOD; ++t) w[t] = *reinterpret_cast<const uint32_t*>(rp + (int64_t)t * plane);
         double L[4][3];
 #pragma unroll
         for (int e = 0; e < 4; ++e)
 #pragma unroll
             for (int l = 0; l < 3; ++l) L[e][l] = 0.0;
-        const uint8_t* rp = res + j * Mp + i0;
 #pragma unroll
-        for (int t = 0; t < kMaxModuli; ++t) {
-            if (t < tb.n) {
-                const uint32_t w = *reinterpret_cast<const uint32_t*>(rp + (int64_t)t * plane);
+        for (int t = 0; t < NMOD; ++t) {
 #pragma unroll
-                for (int e = 0; e < 4; ++e) {
-                    const double xd = (double)((w >> (8 * e)) & 255u);
+            for (int e = 0; e < 4; ++e) {
+                const double xd = (double)((w[t] >> (8 * e)) & 255u);
 #pragma unroll
-                    for (int l = 0; l < 3; ++l) L[e][l] = fma(xd, tb.mt[t][l], L[e][l]);
-                }
+                for (int l = 0; l < 3; ++l) L[e][l] = fma(xd, tb.mt[t][l], L[e][l]);
             }
         }
         const int jf = eb[j];
         double val[4];
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
-            // S = a0 + a1 2^40 + a2 2^80 regrouped at 32-bit positions
-            const uint64_t a0 = (uint64_t)L[e][0], a1 = (uint64_t)L[e][1], a2 = (uint64_t)L[e][2];
-            const uint64_t Le[4] = {a0 & 0xffffffffull, (a0 >> 32) + ((a1 << 8) & 0xffffffffull),
-                                    (a1 >> 24) + ((a2 << 16) & 0xffffffffull), a2 >> 16};
             const int64_t i = i0 + e < m ? i0 + e : m - 1;  // rows past m are computed but never stored
-            val[e] = __dmul_rn(alpha, ldexp(crt_value(Le, tb), ea[i] + jf - 2 * q));
+            val[e] = __dmul_rn(alpha, ldexp(crt_value(L[e][0], L[e][1], L[e][2], tb), ea[i] + jf - 2 * q));
         }
         double* cp = C + i0 + j * ldc;
         if (i0 + 3 < m && (reinterpret_cast<uintptr_t>(cp) & 31) == 0) {
@@ -814,92 +824,139 @@ __global__ void oz_exponent_kernel(const double* __restrict__ part, int64_t rows
 
 // Ozaki scheme II scaling (Cauchy-Schwarz bound).  Entry bound: |x| = m 2^ex
 // (m in [0.5, 1)) counts ceil(256 m)^2 2^(16 - 2d), d = e - ex >= 0 for the
-// line's max exponent e, rounded up to an integer (at least 1) when 2d > 16:
-// an exact integer upper bound of 2^32 (x 2^-e)^2 whose sums do not depend on
-// the summation order.
-__device__ __forceinline__ uint64_t crt_bound_term(double x, int e) {
-    if (x == 0.0) return 0;
+// exponent e of the chunk's maximum, rounded up to an integer (at least 1)
+// when 2d > 16: an exact integer upper bound of 2^32 (x 2^-e)^2 whose sums do
+// not depend on the summation order.  Computed in FP64 without branches:
+// M^2 <= 2^16 scaled by 2^-min(2d - 16, 64) and rounded up is the integer
+// exactly (a zero entry gives m = 0, hence 0), and chunk sums of at most
+// 64 terms of at most 2^32 stay exact.
+__device__ __forceinline__ double crt_bound_term(double x, int e) {
     int ex;
     const double m = frexp(fabs(x), &ex);
-    const uint64_t M = (uint64_t)ceil(m * 256.0);
-    const uint64_t M2 = M * M;
-    const int sh = 2 * (e - ex) - 16;
-    if (sh <= 0) return M2 << (-sh);
-    if (sh >= 17) return 1;
-    return (M2 + ((uint64_t)1 << sh) - 1) >> sh;
+    const double M = ceil(m * 256.0);
+    return ceil(ldexp(M * M, -min(2 * (e - ex) - 16, 64)));
 }
 
-// Per k-chunk sums of the entry bounds of each line (same access patterns
-// as oz_partial_max_kernel); e holds the lines' max exponents.
-__global__ void crt_partial_bound_kernel(const double* __restrict__ x, int64_t rows, int64_t K, int64_t rs, int64_t ks,
-                                         int64_t kchunk, const int32_t* __restrict__ e, uint64_t* __restrict__ part) {
-    const int64_t k0 = blockIdx.y * kchunk;
-    const int64_t k1 = min(K, k0 + kchunk);
+// The scheme II line scan, one read of op(X) per call.  Each line is cut
+// into k-chunks of CRT_KC entries; chunk c gets its own max exponent e_c
+// (max |x| < 2^e_c) and the exact integer bound S_c = sum crt_bound_term(x,
+// e_c) (at most CRT_KC * 2^32 = 2^38), stored as one word
+// ((e_c + 2048) << 48) | S_c, or 0 for an all-zero chunk.  crt_shift_kernel
+// rescales the chunk bounds to the line's max exponent.  Rows strided in
+// memory (ks != 1): a thread per line, consecutive lanes on consecutive
+// lines, the chunk's CRT_KC values held in registers between the max and
+// the sum.  Lines contiguous in k: 16 lanes per line with 16-byte pair
+// loads when every line starts 16-byte aligned, else a wave per line.
+constexpr int CRT_KC = 64;
+
+__device__ __forceinline__ uint64_t crt_chunk_word(double amax, uint64_t sum) {
+    if (amax == 0.0) return 0;
+    int ec;
+    frexp(amax, &ec);
+    return ((uint64_t)(ec + 2048) << 48) | sum;
+}
+
+__device__ __forceinline__ int crt_chunk_exp(double amax) {
+    int ec = 0;
+    if (amax != 0.0) frexp(amax, &ec);
+    return ec;
+}
+
+__global__ __launch_bounds__(256) void crt_scan_kernel(const double* __restrict__ x, int64_t rows, int64_t K,
+                                                       int64_t rs, int64_t ks, uint64_t* __restrict__ part) {
+    const int64_t k0 = (int64_t)blockIdx.y * CRT_KC;
     if (ks != 1) {
-        for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < rows; r += (int64_t)gridDim.x * blockDim.x) {
-            const double* xr = x + r * rs;
-            const int er = e[r];
-            uint64_t s0 = 0, s1 = 0, s2 = 0, s3 = 0;
-            int64_t k = k0;
-            for (; k + 4 <= k1; k += 4) {
-                s0 += crt_bound_term(xr[k * ks], er);
-                s1 += crt_bound_term(xr[(k + 1) * ks], er);
-                s2 += crt_bound_term(xr[(k + 2) * ks], er);
-                s3 += crt_bound_term(xr[(k + 3) * ks], er);
-            }
-            for (; k < k1; ++k) s0 += crt_bound_term(xr[k * ks], er);
-            part[blockIdx.y * rows + r] = (s0 + s1) + (s2 + s3);
-        }
-    } else if (pair_rows(x, rs)) {
-        const int l16 = threadIdx.x & 15;  // as oz_partial_max_kernel
-        const int64_t groups = (int64_t)gridDim.x * (blockDim.x >> 4);
-        for (int64_t r = blockIdx.x * (int64_t)(blockDim.x >> 4) + (threadIdx.x >> 4); r < rows; r += groups) {
-            const double* xr = x + r * rs;
-            const int er = e[r];
-            uint64_t sum = 0;
-            for (int64_t k = k0 + 2 * l16; k < k1; k += 32) {
-                if (k + 1 < k1) {
-                    const pdv2 v = *reinterpret_cast<const pdv2*>(xr + k);
-                    sum += crt_bound_term(v.x, er) + crt_bound_term(v.y, er);
-                } else {
-                    sum += crt_bound_term(xr[k], er);
-                }
-            }
+        const int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+        if (r >= rows) return;
+        const double* xr = x + r * rs;
+        double v[CRT_KC];
+        double mx = 0.0;
+        if (k0 + CRT_KC <= K) {
 #pragma unroll
-            for (int off = 8; off > 0; off >>= 1) sum += __shfl_xor(sum, off);
-            if (l16 == 0) part[blockIdx.y * rows + r] = sum;
+            for (int c = 0; c < CRT_KC; ++c) v[c] = xr[(k0 + c) * ks];
+        } else {
+#pragma unroll
+            for (int c = 0; c < CRT_KC; ++c) v[c] = k0 + c < K ? xr[(k0 + c) * ks] : 0.0;
         }
+#pragma unroll
+        for (int c = 0; c < CRT_KC; ++c) mx = fmax(mx, fabs(v[c]));
+        const int ec = crt_chunk_exp(mx);
+        double s0 = 0.0, s1 = 0.0;
+#pragma unroll
+        for (int c = 0; c < CRT_KC; c += 2) {
+            s0 += crt_bound_term(v[c], ec);
+            s1 += crt_bound_term(v[c + 1], ec);
+        }
+        part[blockIdx.y * rows + r] = crt_chunk_word(mx, (uint64_t)(s0 + s1));
+    } else if (pair_rows(x, rs)) {
+        // 16 lanes per line, two 16-byte pairs each (k0 + 2 l16 and 32 further)
+        const int l16 = threadIdx.x & 15;
+        const int64_t r = blockIdx.x * (int64_t)(blockDim.x >> 4) + (threadIdx.x >> 4);
+        if (r >= rows) return;  // whole 16-lane groups leave together
+        const double* xr = x + r * rs;
+        double v[4];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int64_t k = k0 + 32 * h + 2 * l16;
+            if (k + 1 < K) {
+                const pdv2 w = *reinterpret_cast<const pdv2*>(xr + k);
+                v[2 * h] = w.x;
+                v[2 * h + 1] = w.y;
+            } else {
+                v[2 * h] = k < K ? xr[k] : 0.0;
+                v[2 * h + 1] = 0.0;
+            }
+        }
+        double mx = fmax(fmax(fabs(v[0]), fabs(v[1])), fmax(fabs(v[2]), fabs(v[3])));
+#pragma unroll
+        for (int off = 8; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off));
+        const int ec = crt_chunk_exp(mx);
+        double sum = (crt_bound_term(v[0], ec) + crt_bound_term(v[1], ec)) +
+                     (crt_bound_term(v[2], ec) + crt_bound_term(v[3], ec));
+#pragma unroll
+        for (int off = 8; off > 0; off >>= 1) sum += __shfl_xor(sum, off);
+        if (l16 == 0) part[blockIdx.y * rows + r] = crt_chunk_word(mx, (uint64_t)sum);
     } else {
         const int lane = threadIdx.x & 63;
-        const int64_t waves = (int64_t)gridDim.x * (blockDim.x >> 6);
-        for (int64_t r = blockIdx.x * (int64_t)(blockDim.x >> 6) + (threadIdx.x >> 6); r < rows; r += waves) {
-            const double* xr = x + r * rs;
-            const int er = e[r];
-            uint64_t sum = 0;
-            for (int64_t k = k0 + lane; k < k1; k += 64) sum += crt_bound_term(xr[k], er);
+        const int64_t r = blockIdx.x * (int64_t)(blockDim.x >> 6) + (threadIdx.x >> 6);
+        if (r >= rows) return;
+        const double* xr = x + r * rs;
+        const double v = k0 + lane < K ? xr[k0 + lane] : 0.0;
+        double mx = fabs(v);
 #pragma unroll
-            for (int off = 32; off > 0; off >>= 1) sum += __shfl_xor(sum, off);
-            if (lane == 0) part[blockIdx.y * rows + r] = sum;
-        }
+        for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off));
+        const int ec = crt_chunk_exp(mx);
+        double sum = crt_bound_term(v, ec);
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) sum += __shfl_xor(sum, off);
+        if (lane == 0) part[blockIdx.y * rows + r] = crt_chunk_word(mx, (uint64_t)sum);
     }
 }
 
-// Line shifts: S = sum of the chunk bounds, sigma the largest shift with
+// Line shifts: e = the largest chunk exponent, S = sum over the nonzero
+// chunks of ceil(S_c / 4^(e - e_c)), sigma the largest shift with
 // 2^(2(sigma + e)) S <= 2^(2Q + 32) (sigma = Q + 16 - e - ceil(log4 S)),
-// capped at 53 - e; an all-zero line keeps sigma = 0.  Stored in place of the
-// max exponent as e' = Q - sigma, the form the residue and reconstruction
-// kernels take.
+// capped at 53 - e; an all-zero line keeps sigma = 0.  Stored as
+// e' = Q - sigma, the form the residue and reconstruction kernels take.
 __global__ void crt_shift_kernel(const uint64_t* __restrict__ part, int64_t rows, int64_t rows_p, int nchunk, int Q,
                                  int32_t* __restrict__ e) {
     for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < rows_p; r += (int64_t)gridDim.x * blockDim.x) {
         int sig = 0;
         if (r < rows) {
-            uint64_t S = 0;
-            for (int c = 0; c < nchunk; ++c) S += part[(int64_t)c * rows + r];
-            if (S != 0) {
+            uint64_t top = 0;
+            for (int c = 0; c < nchunk; ++c) top = max(top, part[(int64_t)c * rows + r] >> 48);
+            if (top != 0) {
+                uint64_t S = 0;
+                for (int c = 0; c < nchunk; ++c) {
+                    const uint64_t w = part[(int64_t)c * rows + r];
+                    if (w == 0) continue;
+                    const uint64_t sc = w & ((1ull << 48) - 1);
+                    const int d2 = 2 * (int)(top - (w >> 48));
+                    S += d2 >= 62 ? 1 : (sc + (1ull << d2) - 1) >> d2;
+                }
                 const int bl = S > 1 ? 64 - __builtin_clzll(S - 1) : 0;  // bit length of S - 1
                 const int j = (bl + 1) >> 1;                              // ceil(log4 S)
-                const int er = e[r];
+                const int er = (int)top - 2048;
                 sig = min(53 - er, Q + 16 - er - j);
             }
         }
@@ -991,9 +1048,6 @@ int crt_setup(int64_t k, int nmod, CrtTables* tb, int* q_out) {
     if (q < 1) return fail(DG_ERR_UNSUPPORTED, "dg_ozaki_dgemm: %d moduli leave no bits for k=%lld", nmod, (long long)k);
     *q_out = q;
     tb->n = nmod;
-    auto limbs = [](unsigned __int128 v, uint32_t (&out)[4]) {
-        for (int l = 0; l < 4; ++l) out[l] = (uint32_t)(v >> (32 * l));
-    };
     for (int t = 0; t < nmod; ++t) {
         const int p = crt_modulus(t);
         const unsigned __int128 M = P / (unsigned)p;
@@ -1008,9 +1062,8 @@ int crt_setup(int64_t k, int nmod, CrtTables* tb, int* q_out) {
         }
         tb->cinv[t] = (int32_t)(((x0 % p) + p) % p);
     }
-    limbs(P, tb->P);
-    tb->Pd = (double)P;
-    tb->invPd = 1.0 / tb->Pd;
+    for (int l = 0; l < 3; ++l) tb->Pl[l] = (double)(uint64_t)((P >> (40 * l)) & (l < 2 ? (((unsigned __int128)1 << 40) - 1) : ~(unsigned __int128)0));
+    tb->invPd = 1.0 / (double)P;
     return DG_OK;
 }
 
@@ -1038,9 +1091,8 @@ Plan make_crt_plan(int64_t m, int64_t n, int64_t k, int nmod) {
     p.a_slice = p.Mp * p.Kp;
     p.b_slice = p.Np * p.Kp;
     p.res_plane = p.Mp * p.Np;
-    p.kchunk_max = kMaxChunk;
-    while ((k + p.kchunk_max - 1) / p.kchunk_max > 4096) p.kchunk_max *= 2;
-    p.nchunk_a = (k + p.kchunk_max - 1) / p.kchunk_max;
+    p.kchunk_max = CRT_KC;  // the line scan's k-chunk (k < 2^17 keeps grid.y below 2^11)
+    p.nchunk_a = (k + CRT_KC - 1) / CRT_KC;
     if (p.nchunk_a < 1) p.nchunk_a = 1;
     p.nchunk_b = p.nchunk_a;
     size_t off = 0;
@@ -1210,8 +1262,25 @@ int dg_ozaki_dgemm(char opA, char opB, int64_t m, int64_t n, int64_t k, double a
     const int nchunk = (int)p.nchunk_a;
     const unsigned gy = (unsigned)nchunk;
 
-    // Row exponents of op(A), column exponents of op(B).
-    {
+    if (crt) {
+        // Line exponents and Cauchy-Schwarz shifts: one scan per operand
+        // (chunk exponents and chunk bounds), then the per-line combination.
+        auto scan_grid = [](int64_t rows, int64_t ks, const double* x, int64_t rs) {
+            const int64_t per = ks == 1 ? (pair_rows(x, rs) ? NT / 16 : NT / 64) : NT;
+            return (unsigned)((rows + per - 1) / per);
+        };
+        uint64_t* bp = reinterpret_cast<uint64_t*>(part);
+        hipLaunchKernelGGL(crt_scan_kernel, dim3(scan_grid(m, a_ks, A, a_rs), gy), dim3(NT), 0, st, A, m, k, a_rs,
+                           a_ks, bp);
+        hipLaunchKernelGGL(crt_shift_kernel, dim3((unsigned)((p.Mp + NT - 1) / NT)), dim3(NT), 0, st, bp, m, p.Mp,
+                           nchunk, q, ea);
+        hipLaunchKernelGGL(crt_scan_kernel, dim3(scan_grid(n, b_ks, B, b_rs), gy), dim3(NT), 0, st, B, n, k, b_rs,
+                           b_ks, bp);
+        hipLaunchKernelGGL(crt_shift_kernel, dim3((unsigned)((p.Np + NT - 1) / NT)), dim3(NT), 0, st, bp, n, p.Np,
+                           nchunk, q, eb);
+        DG_LAUNCH_CHECK();
+    } else {
+        // Row exponents of op(A), column exponents of op(B).
         // ks == 1: 16 lanes per row when the rows pair up (NT/16 rows per block), else
         // a wave per row (NT/64); otherwise a thread per row
         auto pm_grid = [](int64_t rows, int64_t ks, const double* x, int64_t rs) {
@@ -1231,29 +1300,22 @@ int dg_ozaki_dgemm(char opA, char opB, int64_t m, int64_t n, int64_t k, double a
         hipLaunchKernelGGL(oz_exponent_kernel, dim3((unsigned)((p.Np + NT - 1) / NT)), dim3(NT), 0, st, part, n, p.Np,
                            nchunk, eb);
         DG_LAUNCH_CHECK();
-        if (crt) {
-            // line shifts from the Cauchy-Schwarz bound (the chunk buffer now holds bound sums)
-            uint64_t* bp = reinterpret_cast<uint64_t*>(part);
-            gx = pm_grid(m, a_ks, A, a_rs);
-            hipLaunchKernelGGL(crt_partial_bound_kernel, dim3(gx, gy), dim3(NT), 0, st, A, m, k, a_rs, a_ks, p.kchunk_max,
-                               ea, bp);
-            hipLaunchKernelGGL(crt_shift_kernel, dim3((unsigned)((p.Mp + NT - 1) / NT)), dim3(NT), 0, st, bp, m, p.Mp,
-                               nchunk, q, ea);
-            gx = pm_grid(n, b_ks, B, b_rs);
-            hipLaunchKernelGGL(crt_partial_bound_kernel, dim3(gx, gy), dim3(NT), 0, st, B, n, k, b_rs, b_ks, p.kchunk_max,
-                               eb, bp);
-            hipLaunchKernelGGL(crt_shift_kernel, dim3((unsigned)((p.Np + NT - 1) / NT)), dim3(NT), 0, st, bp, n, p.Np,
-                               nchunk, q, eb);
-            DG_LAUNCH_CHECK();
-        }
     }
     if (crt) {
         dim3 ga((unsigned)(p.Kp / RT_K), (unsigned)(p.Mp / RT_R));
         dim3 gb((unsigned)(p.Kp / RT_K), (unsigned)(p.Np / RT_R));
-        hipLaunchKernelGGL(crt_residue_kernel, ga, dim3(256), 0, st, A, m, k, a_rs, a_ks, ea, q, slices, sa, p.Kp,
-                           p.a_slice);
-        hipLaunchKernelGGL(crt_residue_kernel, gb, dim3(256), 0, st, B, n, k, b_rs, b_ks, eb, q, slices, sb, p.Kp,
-                           p.b_slice);
+        decltype(&crt_residue_kernel<16>) rsk = crt_residue_kernel<16>;
+        decltype(&crt_reconstruct_kernel<16>) rk = crt_reconstruct_kernel<16>;
+        switch (slices) {
+#define DG_CRT_CASE(N) case N: rsk = crt_residue_kernel<N>; rk = crt_reconstruct_kernel<N>; break;
+            DG_CRT_CASE(2) DG_CRT_CASE(3) DG_CRT_CASE(4) DG_CRT_CASE(5) DG_CRT_CASE(6) DG_CRT_CASE(7)
+            DG_CRT_CASE(8) DG_CRT_CASE(9) DG_CRT_CASE(10) DG_CRT_CASE(11) DG_CRT_CASE(12) DG_CRT_CASE(13)
+            DG_CRT_CASE(14) DG_CRT_CASE(15)
+#undef DG_CRT_CASE
+            default: break;
+        }
+        hipLaunchKernelGGL(rsk, ga, dim3(256), 0, st, A, m, k, a_rs, a_ks, ea, q, sa, p.Kp, p.a_slice);
+        hipLaunchKernelGGL(rsk, gb, dim3(256), 0, st, B, n, k, b_rs, b_ks, eb, q, sb, p.Kp, p.b_slice);
         DG_LAUNCH_CHECK();
         CrtArgs cg{};
         cg.a = sa; cg.b = sb; cg.res = reinterpret_cast<uint8_t*>(ws + p.off_w);
@@ -1267,8 +1329,8 @@ int dg_ozaki_dgemm(char opA, char opB, int64_t m, int64_t n, int64_t k, double a
         DG_LAUNCH_CHECK();
         const int64_t blocks = (((m + 3) / 4) * n + 255) / 256;
         if (blocks > INT32_MAX) return fail(DG_ERR_INVALID, "dg_ozaki_dgemm: too many output elements");
-        hipLaunchKernelGGL(crt_reconstruct_kernel, dim3((unsigned)blocks), dim3(256), 0, st, cg.res, tables, m, n, p.Mp,
-                           p.res_plane, ea, eb, q, alpha, beta, C, ldc);
+        hipLaunchKernelGGL(rk, dim3((unsigned)blocks), dim3(256), 0, st, cg.res, tables, m, n, p.Mp, p.res_plane, ea,
+                           eb, q, alpha, beta, C, ldc);
         DG_LAUNCH_CHECK();
         return DG_OK;
     }
@@ -1289,6 +1351,7 @@ int dg_ozaki_dgemm(char opA, char opB, int64_t m, int64_t n, int64_t k, double a
         }
         DG_LAUNCH_CHECK();
     }
+    TableUseScope uses{st};  // after the launch below returns
     const OzSeg* dsched = static_cast<const OzSeg*>(cached_table(sched.data(), sched.size() * sizeof(OzSeg), st));
     if (!dsched) return DG_ERR_ALLOC;
     OzArgs g{};

@@ -1,7 +1,8 @@
 // Bit-exact layout kernels: 2D-block and block-cyclic pack/unpack, strided
-// block copy and the 2.5D layer reduction.  All are HBM-bound copies; each
-// moves whole column runs so that both the strided global side and the
-// contiguous packed side are accessed in 512-byte wavefront runs.
+// block copy and the 2.5D layer reduction.  All are HBM-bound copies; a
+// workgroup moves whole column runs of four columns at once, so that both
+// the strided global side and the contiguous packed side are accessed in
+// 1 KB wavefront runs of 16-byte vectors where alignment allows.
 #include "dg_internal.hpp"
 
 namespace dg {
@@ -16,20 +17,70 @@ typedef __attribute__((address_space(1))) const double gcdouble;
 typedef __attribute__((address_space(1))) double gdouble;
 typedef double dv2 __attribute__((ext_vector_type(2)));
 
-// Copies one run of `len` elements; both sides contiguous.
+// Copies runs of `len` elements, both sides contiguous, for nc <= 4
+// columns at once: every load of a thread's pass over the columns is issued
+// before the first store.  When every run starts 16-byte aligned on both
+// sides, the runs move in 16-byte vectors (E elements each) and threads
+// 0..len%E-1 copy the tail elements of each run; otherwise element by element.
 template <typename T>
-__device__ __forceinline__ void copy_run(const T* __restrict__ src, T* __restrict__ dst, int64_t len) {
+__device__ __forceinline__ void copy_cols(const T* const (&src)[kColsPerBlock], T* const (&dst)[kColsPerBlock],
+                                          int nc, int64_t len) {
     typedef __attribute__((address_space(1))) const T gcT;
     typedef __attribute__((address_space(1))) T gT;
-    gcT* s = (gcT*)src;
-    gT* d = (gT*)dst;
-    int64_t x = threadIdx.x;
-    // Four independent loads in flight per thread.
-    for (; x + 3 * kThreads < len; x += 4 * kThreads) {
-        T a = s[x], b = s[x + kThreads], c = s[x + 2 * kThreads], e = s[x + 3 * kThreads];
-        d[x] = a; d[x + kThreads] = b; d[x + 2 * kThreads] = c; d[x + 3 * kThreads] = e;
+    typedef T vT __attribute__((ext_vector_type(16 / sizeof(T))));
+    typedef __attribute__((address_space(1))) const vT gcV;
+    typedef __attribute__((address_space(1))) vT gV;
+    constexpr int E = 16 / sizeof(T);
+    constexpr int U = 2;  // vectors per column and pass
+    bool vec = true;
+#pragma unroll
+    for (int i = 0; i < kColsPerBlock; ++i)
+        if (i < nc) vec = vec && !((reinterpret_cast<uintptr_t>(src[i]) | reinterpret_cast<uintptr_t>(dst[i])) & 15);
+    const int64_t tid = threadIdx.x;
+    if (vec) {
+        const int64_t nv = len / E;
+        for (int64_t x0 = 0; x0 < nv; x0 += U * kThreads) {
+            vT v[kColsPerBlock][U];
+#pragma unroll
+            for (int i = 0; i < kColsPerBlock; ++i)
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const int64_t x = x0 + u * kThreads + tid;
+                    if (i < nc && x < nv) v[i][u] = ((gcV*)src[i])[x];
+                }
+#pragma unroll
+            for (int i = 0; i < kColsPerBlock; ++i)
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const int64_t x = x0 + u * kThreads + tid;
+                    if (i < nc && x < nv) ((gV*)dst[i])[x] = v[i][u];
+                }
+        }
+        const int64_t t0 = nv * E;
+        if (tid < len - t0) {
+#pragma unroll
+            for (int i = 0; i < kColsPerBlock; ++i)
+                if (i < nc) ((gT*)dst[i])[t0 + tid] = ((gcT*)src[i])[t0 + tid];
+        }
+    } else {
+        for (int64_t x0 = 0; x0 < len; x0 += U * kThreads) {
+            T v[kColsPerBlock][U];
+#pragma unroll
+            for (int i = 0; i < kColsPerBlock; ++i)
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const int64_t x = x0 + u * kThreads + tid;
+                    if (i < nc && x < len) v[i][u] = ((gcT*)src[i])[x];
+                }
+#pragma unroll
+            for (int i = 0; i < kColsPerBlock; ++i)
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const int64_t x = x0 + u * kThreads + tid;
+                    if (i < nc && x < len) ((gT*)dst[i])[x] = v[i][u];
+                }
+        }
     }
-    for (; x < len; x += kThreads) d[x] = s[x];
 }
 
 struct Bands {
@@ -49,13 +100,18 @@ __global__ void block2d_kernel(const double* __restrict__ src, double* __restric
     if (rb <= 0) return;
     const int64_t band_base = r0 * cols;
     for (int64_t c = (int64_t)blockIdx.y * kColsPerBlock; c < cols; c += (int64_t)gridDim.y * kColsPerBlock) {
-        const int64_t cend = c + kColsPerBlock < cols ? c + kColsPerBlock : cols;
-        for (int64_t cc = c; cc < cend; ++cc) {
+        const int nc = (int)(c + kColsPerBlock < cols ? kColsPerBlock : cols - c);
+        const double* s[kColsPerBlock];
+        double* d[kColsPerBlock];
+#pragma unroll
+        for (int i = 0; i < kColsPerBlock; ++i) {
+            const int64_t cc = c + (i < nc ? i : 0);
             const int64_t g = r0 + cc * ldg;
             const int64_t p = band_base + cc * rb;
-            if (PACK) copy_run(src + g, dst + p, rb);
-            else      copy_run(src + p, dst + g, rb);
+            s[i] = PACK ? src + g : src + p;
+            d[i] = PACK ? dst + p : dst + g;
         }
+        copy_cols(s, d, nc, rb);
     }
 }
 
@@ -81,17 +137,22 @@ __global__ void cyclic_kernel(const double* __restrict__ src, double* __restrict
     const int64_t lrow0 = (ib / geo.P) * geo.mb;
     const int64_t lrows = geo.lrows[pr];
     for (int64_t c = (int64_t)blockIdx.y * kColsPerBlock; c < geo.cols; c += (int64_t)gridDim.y * kColsPerBlock) {
-        const int64_t cend = c + kColsPerBlock < geo.cols ? c + kColsPerBlock : geo.cols;
-        for (int64_t cc = c; cc < cend; ++cc) {
+        const int nc = (int)(c + kColsPerBlock < geo.cols ? kColsPerBlock : geo.cols - c);
+        const double* s[kColsPerBlock];
+        double* d[kColsPerBlock];
+#pragma unroll
+        for (int i = 0; i < kColsPerBlock; ++i) {
+            const int64_t cc = c + (i < nc ? i : 0);
             const int64_t jb = cc / geo.nb;
             const int pc = (int)(jb % geo.Q);
             const int64_t lcol = (jb / geo.Q) * geo.nb + (cc - jb * geo.nb);
             const int64_t base = geo.rowpre[pr] * geo.cols + lrows * colpre[pc];
             const int64_t p = base + lrow0 + lcol * lrows;
             const int64_t g = row0 + cc * ldg;
-            if (PACK) copy_run(src + g, dst + p, len);
-            else      copy_run(src + p, dst + g, len);
+            s[i] = PACK ? src + g : src + p;
+            d[i] = PACK ? dst + p : dst + g;
         }
+        copy_cols(s, d, nc, len);
     }
 }
 
@@ -99,8 +160,16 @@ template <typename T>
 __global__ void copy_block_kernel(int64_t rows, int64_t cols, const T* __restrict__ src, int64_t lds,
                                   T* __restrict__ dst, int64_t ldd) {
     for (int64_t c = (int64_t)blockIdx.x * kColsPerBlock; c < cols; c += (int64_t)gridDim.x * kColsPerBlock) {
-        const int64_t cend = c + kColsPerBlock < cols ? c + kColsPerBlock : cols;
-        for (int64_t cc = c; cc < cend; ++cc) copy_run(src + cc * lds, dst + cc * ldd, rows);
+        const int nc = (int)(c + kColsPerBlock < cols ? kColsPerBlock : cols - c);
+        const T* s[kColsPerBlock];
+        T* d[kColsPerBlock];
+#pragma unroll
+        for (int i = 0; i < kColsPerBlock; ++i) {
+            const int64_t cc = c + (i < nc ? i : 0);
+            s[i] = src + cc * lds;
+            d[i] = dst + cc * ldd;
+        }
+        copy_cols(s, d, nc, rows);
     }
 }
 
@@ -115,14 +184,23 @@ __global__ void layer_reduce_kernel(double* __restrict__ C, const double* __rest
     if (VEC) {
         typedef __attribute__((address_space(1))) dv2 gdv2;
         typedef __attribute__((address_space(1))) const dv2 gcdv2;
-        for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x < n / 2; x += stride) {
-            dv2 acc = *(gdv2*)(c + 2 * x);
+        // two 16-byte vectors per thread and pass, `stride` apart: every
+        // load of a pass is issued before the adds
+        const int64_t nv = n / 2;
+        for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x < nv; x += 2 * stride) {
+            const bool two = x + stride < nv;
+            dv2 acc0 = *(gdv2*)(c + 2 * x);
+            dv2 acc1 = two ? *(gdv2*)(c + 2 * (x + stride)) : dv2{0.0, 0.0};
             for (int l = 0; l < nlayers; ++l) {
-                const dv2 v = *(gcdv2*)(p + (int64_t)l * n + 2 * x);
-                acc.x += v.x;
-                acc.y += v.y;
+                const dv2 v0 = *(gcdv2*)(p + (int64_t)l * n + 2 * x);
+                const dv2 v1 = two ? *(gcdv2*)(p + (int64_t)l * n + 2 * (x + stride)) : dv2{0.0, 0.0};
+                acc0.x += v0.x;
+                acc0.y += v0.y;
+                acc1.x += v1.x;
+                acc1.y += v1.y;
             }
-            *(gdv2*)(c + 2 * x) = acc;
+            *(gdv2*)(c + 2 * x) = acc0;
+            if (two) *(gdv2*)(c + 2 * (x + stride)) = acc1;
         }
     } else {
         for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x < n; x += stride) {
@@ -199,6 +277,7 @@ int cyclic(const char* who, const double* src, double* dst, int64_t ldg, int64_t
     acc = 0;
     for (int pc = 0; pc < Q; ++pc) { colpre[pc] = acc; acc += dg_numroc(cols, nb, pc, 0, Q); }
     colpre[Q] = acc;
+    TableUseScope uses{as_stream(stream)};  // after the launch below returns
     const CyclicGeom* dg_ = static_cast<const CyclicGeom*>(cached_table(&g, sizeof(g), as_stream(stream)));
     if (!dg_) return DG_ERR_ALLOC;
     const int64_t* dcp = static_cast<const int64_t*>(cached_table(colpre.data(), colpre.size() * sizeof(int64_t), as_stream(stream)));

@@ -171,7 +171,6 @@ void pblasDecomposer::gatherMatrix(int receiverRank, double* globalMatrix, doubl
 BlockSequentialDecomposer::BlockSequentialDecomposer(long long M_, long long N_, long long K_, int size_,
                                                      bool colMajor_, int rank_)
     : M(M_), N(N_), K(K_), numberOfDevices(size_), colMajor(colMajor_), rank(rank_), size(size_) {
-    if (!colMajor_) dgHostError(DG_ABORT_GRID, "BlockSequentialDecomposer: only column-major storage is implemented");
     if (size_ < 1 || rank_ < 0 || rank_ >= size_) dgHostError(DG_ABORT_GRID, "BlockSequentialDecomposer: rank %d of %d", rank_, size_);
     calculateVirtualDeviceGrid();
     rowOff_.resize(dRow + 1);
@@ -197,28 +196,49 @@ BlockSequentialDecomposer::~BlockSequentialDecomposer() {
 void BlockSequentialDecomposer::calculateVirtualDeviceGrid() { DG_CHECK(dg_virtual_grid(numberOfDevices, &dRow, &dCol)); }
 
 // Row panel i of A (rows rowOff_[i].., all K columns) at rowOff_[i]*K; column
-// panel j of B (all K rows, columns colOff_[j]..) at colOff_[j]*K; block (i,j)
-// of C at rowOff_[i]*N + rows_i*colOff_[j]: the dg_pack_block2d layouts of A
-// over a dRow x 1 grid, B over 1 x dCol and C over dRow x dCol.
+// panel j of B (all K rows, columns colOff_[j]..) at colOff_[j]*K.
+// Column-major (ld of a piece = its rows): block (i,j) of C at
+// rowOff_[i]*N + rows_i*colOff_[j] -- the dg_pack_block2d layouts of A over
+// a dRow x 1 grid, B over 1 x dCol and C over dRow x dCol.
+// Row-major (ld of a piece = its columns, as the reference's colMajor = false
+// branch, src/Decompositions/2DBlockSequentialDecomposition.cpp:89-100, 190-200):
+// a row-major matrix is the column-major transpose, so A^T (K x M) packs over
+// 1 x dRow, B^T (N x K) over dCol x 1 and C^T (N x M) over dCol x dRow; block
+// (i,j) of C is block (j,i) of C^T, at colOff_[j]*M + columns_j*rowOff_[i].
 double* BlockSequentialDecomposer::panelA(int i) const { return localA + rowOff_[i] * K; }
 double* BlockSequentialDecomposer::panelB(int j) const { return localB + colOff_[j] * K; }
 double* BlockSequentialDecomposer::blockC(int i, int j) const {
+    if (!colMajor) return localC + colOff_[j] * M + tileColumns(j) * rowOff_[i];
     return localC + rowOff_[i] * N + tileRows(i) * colOff_[j];
 }
 
 void BlockSequentialDecomposer::deliverMatrix(double* gA, double* gB, double* gC, double** la, double** lb,
                                               double** lc) {
     const int64_t kOff[2] = {0, K};
-    with_device_matrix(gA, M, K, true, false, [&](double* d) {
-        DG_CHECK(dg_pack_block2d(d, M, M, K, rowOff_.data(), dRow, kOff, 1, localA, nullptr));
-    });
-    with_device_matrix(gB, K, N, true, false, [&](double* d) {
-        DG_CHECK(dg_pack_block2d(d, K, K, N, kOff, 1, colOff_.data(), dCol, localB, nullptr));
-    });
-    if (gC) {
-        with_device_matrix(gC, M, N, true, false, [&](double* d) {
-            DG_CHECK(dg_pack_block2d(d, M, M, N, rowOff_.data(), dRow, colOff_.data(), dCol, localC, nullptr));
+    if (colMajor) {
+        with_device_matrix(gA, M, K, true, false, [&](double* d) {
+            DG_CHECK(dg_pack_block2d(d, M, M, K, rowOff_.data(), dRow, kOff, 1, localA, nullptr));
         });
+        with_device_matrix(gB, K, N, true, false, [&](double* d) {
+            DG_CHECK(dg_pack_block2d(d, K, K, N, kOff, 1, colOff_.data(), dCol, localB, nullptr));
+        });
+        if (gC) {
+            with_device_matrix(gC, M, N, true, false, [&](double* d) {
+                DG_CHECK(dg_pack_block2d(d, M, M, N, rowOff_.data(), dRow, colOff_.data(), dCol, localC, nullptr));
+            });
+        }
+    } else {
+        with_device_matrix(gA, K, M, true, false, [&](double* d) {
+            DG_CHECK(dg_pack_block2d(d, K, K, M, kOff, 1, rowOff_.data(), dRow, localA, nullptr));
+        });
+        with_device_matrix(gB, N, K, true, false, [&](double* d) {
+            DG_CHECK(dg_pack_block2d(d, N, N, K, colOff_.data(), dCol, kOff, 1, localB, nullptr));
+        });
+        if (gC) {
+            with_device_matrix(gC, N, M, true, false, [&](double* d) {
+                DG_CHECK(dg_pack_block2d(d, N, N, M, colOff_.data(), dCol, rowOff_.data(), dRow, localC, nullptr));
+            });
+        }
     }
     HIP_CHECK(hipDeviceSynchronize());
     if (la) *la = localA;
@@ -229,10 +249,17 @@ void BlockSequentialDecomposer::deliverMatrix(double* gA, double* gB, double* gC
 void BlockSequentialDecomposer::gatherResult(int gatherRank, double* C, double* lc) {
     if (gatherRank < 0 || gatherRank >= size) dgHostError(DG_ABORT_GRID, "gatherResult: rank %d", gatherRank);
     double* src = lc ? lc : localC;
-    with_device_matrix(C, M, N, false, true, [&](double* d) {
-        DG_CHECK(dg_unpack_block2d(src, rowOff_.data(), dRow, colOff_.data(), dCol, d, M, M, N, nullptr));
-        HIP_CHECK(hipDeviceSynchronize());
-    });
+    if (colMajor) {
+        with_device_matrix(C, M, N, false, true, [&](double* d) {
+            DG_CHECK(dg_unpack_block2d(src, rowOff_.data(), dRow, colOff_.data(), dCol, d, M, M, N, nullptr));
+            HIP_CHECK(hipDeviceSynchronize());
+        });
+    } else {
+        with_device_matrix(C, N, M, false, true, [&](double* d) {
+            DG_CHECK(dg_unpack_block2d(src, colOff_.data(), dCol, rowOff_.data(), dRow, d, N, N, M, nullptr));
+            HIP_CHECK(hipDeviceSynchronize());
+        });
+    }
 }
 
 void BlockSequentialDecomposer::multiply(double alpha, double beta, hipStream_t stream) {
@@ -243,16 +270,26 @@ void BlockSequentialDecomposer::multiply(double alpha, double beta, hipStream_t 
             dg_gemm_desc d{};
             d.opA = 'N';
             d.opB = 'N';
-            d.m = tileRows(i);
-            d.n = tileColumns(j);
             d.k = K;
             d.alpha = alpha;
             d.beta = beta;
-            d.A = panelA(i);
-            d.lda = d.m > 0 ? d.m : 1;
-            d.B = panelB(j);
-            d.ldb = K > 0 ? K : 1;
             d.C = blockC(i, j);
+            if (colMajor) {
+                d.m = tileRows(i);
+                d.n = tileColumns(j);
+                d.A = panelA(i);
+                d.lda = d.m > 0 ? d.m : 1;
+                d.B = panelB(j);
+                d.ldb = K > 0 ? K : 1;
+            } else {
+                // row-major C_ij = A_i B_j is column-major C_ij^T = B_j^T A_i^T
+                d.m = tileColumns(j);
+                d.n = tileRows(i);
+                d.A = panelB(j);
+                d.lda = d.m > 0 ? d.m : 1;
+                d.B = panelA(i);
+                d.ldb = K > 0 ? K : 1;
+            }
             d.ldc = d.m > 0 ? d.m : 1;
             descs.push_back(d);
         }

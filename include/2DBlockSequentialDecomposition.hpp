@@ -26,8 +26,9 @@ class BlockSequentialDecomposer {
     // Device buffers (owned): every rank's A row panel, B column panel, C block.
     double *localA = nullptr, *localB = nullptr, *localC = nullptr;
 
-    // `size` virtual ranks; the scalar fields describe `rank`.  Only
-    // column-major storage is implemented (colMajor = false is rejected).
+    // `size` virtual ranks; the scalar fields describe `rank`.  colMajor =
+    // false takes row-major global matrices (ld = columns) and keeps every
+    // piece row-major with ld = its columns, as the reference's branch.
     BlockSequentialDecomposer(long long M, long long N, long long K, int size, bool colMajor = true, int rank = 0);
     ~BlockSequentialDecomposer();
     BlockSequentialDecomposer(const BlockSequentialDecomposer&) = delete;
@@ -36,8 +37,9 @@ class BlockSequentialDecomposer {
     // dRow x dCol = most-square factorisation of `size` (:27-54).
     void calculateVirtualDeviceGrid();
 
-    // Global A (M x K, ld M), B (K x N, ld K), C (M x N, ld M), host or device,
-    // -> every rank's pieces (deliverMatrix :56-162).  The localX arguments
+    // Global A (M x K), B (K x N), C (M x N), host or device, column-major
+    // (ld M, K, M) or row-major (ld K, N, N) -> every rank's pieces
+    // (deliverMatrix :56-162).  The localX arguments
     // receive the owned packed buffers (may be nullptr).
     void deliverMatrix(double* globalA, double* globalB, double* globalC, double** localA_out = nullptr,
                        double** localB_out = nullptr, double** localC_out = nullptr);

@@ -187,12 +187,16 @@ int dg_cannon25d(char opA, char opB, int64_t M, int64_t N, int64_t K, double alp
 int dg_cannon_m_index(int p, int c, int schedule, int i, int j, int k, int r);
 
 /* SUMMA on a virtual P x Q grid over block-cyclic local matrices
- * (dg_pack_block_cyclic layout).  A is M x K with blocks mb x kb, B is K x N
- * with blocks kb x nb, C is M x N with blocks mb x nb.  Panel t of width kb is
- * read in place from its owners (process column t mod Q of A, process row
- * t mod P of B): the panel broadcasts are index rotations.  opA = opB = 'N'.
+ * (dg_pack_block_cyclic layout): C = alpha op(A) op(B) + beta C.  C is M x N
+ * with blocks mb x nb.  The stored A is M x K with blocks mb x kb (opA = 'N')
+ * or K x M with blocks kb x mb (opA = 'T'); the stored B is K x N with blocks
+ * kb x nb (opB = 'N') or N x K with blocks nb x kb (opB = 'T'); all three are
+ * distributed over the same P x Q grid.  Panel t of width kb is read in place
+ * from its owners (for 'N' operands process column t mod Q of A and process
+ * row t mod P of B): the panel broadcasts are index rotations.
  * New relative to the reference (include/summa.hpp:1-28 is empty; SUMMA
- * exists there only inside cublasMpGemm, src/Wrappers/cuBLASMp/cuBLASMpWrapped.cpp:198-223). */
+ * exists there only inside cublasMpGemm, whose TransA/TransB arguments this
+ * follows: src/Wrappers/cuBLASMp/cuBLASMpWrapped.cpp:198-223, 565-574, 703). */
 int dg_summa(char opA, char opB, int64_t M, int64_t N, int64_t K, double alpha,
              const double* A_packed, const double* B_packed, double beta, double* C_packed,
              int64_t mb, int64_t nb, int64_t kb, int P, int Q, void* stream);

@@ -79,6 +79,45 @@ void or_pack_block2d(const double* G, int64_t ldg, int64_t rows, int64_t cols, c
         }
 }
 
+/* copyBlock with colMajor = false (sharedTransfers.cpp:29-63): rows x cols
+ * of a row-major source (ld = lds) into a row-major destination (ld = ldd),
+ * row by row. */
+static void or_copy_block_rows(int64_t rows, int64_t cols, const double* src, int64_t lds, double* dst, int64_t ldd) {
+    int64_t r;
+    if (rows * cols <= 0) return;
+    for (r = 0; r < rows; ++r) memcpy(dst + ldd * r, src + lds * r, sizeof(double) * (size_t)cols);
+}
+
+/* BlockSequentialDecomposer::deliverMatrix, 2DBlockSequentialDecomposition.cpp:56-162:
+ * every rank's tiles A_r (tileM x K), B_r (K x tileN), C_r (tileM x tileN), in
+ * rank order r = i*dCol + j, each tile in the storage order of the globals:
+ * column-major (lld = tile rows; offsets :100-108) or row-major (lld = tile
+ * columns; offsets :89-99).  Tiles of the last grid row / column absorb the
+ * remainder (:79-88).  Offsets are 64-bit here (the reference's are int). */
+void or_block_seq_tiles(int col_major, int64_t M, int64_t N, int64_t K, int dRow, int dCol, const double* gA,
+                        const double* gB, const double* gC, double* tA, double* tB, double* tC) {
+    int i, j;
+    int64_t pa = 0, pb = 0, pc = 0;
+    for (i = 0; i < dRow; ++i) {
+        int64_t tileM = M / dRow + (i + 1 == dRow ? M % dRow : 0);
+        for (j = 0; j < dCol; ++j) {
+            int64_t tileN = N / dCol + (j + 1 == dCol ? N % dCol : 0);
+            if (col_major) {
+                or_copy_block(tileM, K, gA + (M / dRow) * i, M, tA + pa, tileM);
+                or_copy_block(K, tileN, gB + (N / dCol) * K * j, K, tB + pb, K);
+                or_copy_block(tileM, tileN, gC + (N / dCol) * M * j + (M / dRow) * i, M, tC + pc, tileM);
+            } else {
+                or_copy_block_rows(tileM, K, gA + (M / dRow) * K * i, K, tA + pa, K);
+                or_copy_block_rows(K, tileN, gB + (N / dCol) * j, N, tB + pb, tileN);
+                or_copy_block_rows(tileM, tileN, gC + (M / dRow) * N * i + (N / dCol) * j, N, tC + pc, tileN);
+            }
+            pa += tileM * K;
+            pb += K * tileN;
+            pc += tileM * tileN;
+        }
+    }
+}
+
 void or_unpack_block2d(const double* packed, const int64_t* row_off, int nrb, const int64_t* col_off, int ncb,
                        double* G, int64_t ldg, int64_t rows, int64_t cols) {
     /* reorderMatrix 25DCannon.cpp:254-271: copyBlock(localM, localN, cTile, &C[offset], localM, M) */
@@ -278,6 +317,37 @@ void or_summa(int64_t M, int64_t N, int64_t K, double alpha, const double* Ap, c
             }
         }
     free(bA); free(bB); free(bC);
+}
+
+/* SUMMA with either operand transposed, restated on the global matrices:
+ * unpack the stored operands (A is M x K or K x M, B is K x N or N x K, on
+ * the same P x Q grid with the blocks of the matching dimensions), add the
+ * panels of width kb in order 0, 1, ... (beta on the first), pack C.  The
+ * result per element is the same panel-ordered sum as or_summa's. */
+void or_summa_op(char opA, char opB, int64_t M, int64_t N, int64_t K, double alpha, const double* Ap,
+                 const double* Bp, double beta, double* Cp, int64_t mb, int64_t nb, int64_t kb, int P, int Q) {
+    int ta = opA == 'T' || opA == 't' || opA == 'C' || opA == 'c';
+    int tb = opB == 'T' || opB == 't' || opB == 'C' || opB == 'c';
+    int64_t rA = ta ? K : M, cA = ta ? M : K, rB = tb ? N : K, cB = tb ? K : N;
+    int64_t t, npanels = (K + kb - 1) / kb, x;
+    double* Ag = (double*)malloc((size_t)(rA * cA > 0 ? rA * cA : 1) * sizeof(double));
+    double* Bg = (double*)malloc((size_t)(rB * cB > 0 ? rB * cB : 1) * sizeof(double));
+    double* Cg = (double*)malloc((size_t)(M * N > 0 ? M * N : 1) * sizeof(double));
+    if (M == 0 || N == 0) { free(Ag); free(Bg); free(Cg); return; }
+    or_unpack_block_cyclic(Ap, ta ? kb : mb, ta ? mb : kb, P, Q, Ag, rA > 0 ? rA : 1, rA, cA);
+    or_unpack_block_cyclic(Bp, tb ? nb : kb, tb ? kb : nb, P, Q, Bg, rB > 0 ? rB : 1, rB, cB);
+    or_unpack_block_cyclic(Cp, mb, nb, P, Q, Cg, M, M, N);
+    if (npanels == 0)
+        for (x = 0; x < M * N; ++x) Cg[x] = beta == 0.0 ? 0.0 : beta * Cg[x];
+    for (t = 0; t < npanels; ++t) {
+        int64_t k0 = t * kb, w = (K - k0) < kb ? (K - k0) : kb;
+        const double* Ak = ta ? Ag + k0 : Ag + k0 * M;   /* op(A)(:, k0:k0+w) */
+        const double* Bk = tb ? Bg + k0 * N : Bg + k0;   /* op(B)(k0:k0+w, :) */
+        or_dgemm(ta ? 'T' : 'N', tb ? 'T' : 'N', M, N, w, alpha, Ak, ta ? K : M, Bk, tb ? N : K,
+                 t == 0 ? beta : 1.0, Cg, M);
+    }
+    or_pack_block_cyclic(Cg, M, M, N, mb, nb, P, Q, Cp);
+    free(Ag); free(Bg); free(Cg);
 }
 
 void or_layer_reduce(double* C, const double* partial, int64_t n, int nlayers) {

@@ -46,6 +46,11 @@ void or_unpack_block2d(const double* packed, const int64_t* row_off, int nrb, co
  * running rowOffset/columnOffset bookkeeping, kept per rank). */
 void or_pack_block_cyclic(const double* G, int64_t ldg, int64_t rows, int64_t cols, int64_t mb, int64_t nb, int P,
                           int Q, double* packed);
+/* BlockSequentialDecomposer::deliverMatrix (2DBlockSequentialDecomposition.cpp:56-162):
+ * every rank's A, B, C tiles in rank order, column-major (col_major != 0) or
+ * row-major, each with the reference's local leading dimension. */
+void or_block_seq_tiles(int col_major, int64_t M, int64_t N, int64_t K, int dRow, int dCol, const double* gA,
+                        const double* gB, const double* gC, double* tA, double* tB, double* tC);
 /* gatherMatrix src/Decompositions/pblasDecomposition.cpp:150-221 */
 void or_unpack_block_cyclic(const double* packed, int64_t mb, int64_t nb, int P, int Q, double* G, int64_t ldg,
                             int64_t rows, int64_t cols);
@@ -73,6 +78,11 @@ void or_cannon25d(char opA, char opB, int64_t M, int64_t N, int64_t K, double al
  * panels t of A_loc(pr, t mod Q)[:, panel] * B_loc(t mod P, pc)[panel, :]. */
 void or_summa(int64_t M, int64_t N, int64_t K, double alpha, const double* Ap, const double* Bp, double beta,
               double* Cp, int64_t mb, int64_t nb, int64_t kb, int P, int Q);
+/* The same with opA / opB = 'T': the stored A is K x M (blocks kb x mb),
+ * the stored B is N x K (blocks nb x kb); TransA/TransB of cublasMpGemm,
+ * src/Wrappers/cuBLASMp/cuBLASMpWrapped.cpp:565-574, 703. */
+void or_summa_op(char opA, char opB, int64_t M, int64_t N, int64_t K, double alpha, const double* Ap,
+                 const double* Bp, double beta, double* Cp, int64_t mb, int64_t nb, int64_t kb, int P, int Q);
 
 /* C[x] += sum_l partial[l*n + x], layer order (MPI_Reduce SUM,
  * src/25DCannon.cpp:203-228). */

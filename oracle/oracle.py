@@ -41,7 +41,9 @@ _SIGS = {
     "or_dgemm": (None, [_chr, _chr, _i64, _i64, _i64, _dbl, _dp, _i64, _dp, _i64, _dbl, _dp, _i64]),
     "or_cannon25d": (None, [_chr, _chr, _i64, _i64, _i64, _dbl, _dp, _dp, _dbl, _dp, _int, _int, _int]),
     "or_summa": (None, [_i64, _i64, _i64, _dbl, _dp, _dp, _dbl, _dp, _i64, _i64, _i64, _int, _int]),
+    "or_summa_op": (None, [_chr, _chr, _i64, _i64, _i64, _dbl, _dp, _dp, _dbl, _dp, _i64, _i64, _i64, _int, _int]),
     "or_layer_reduce": (None, [_dp, _dp, _i64, _int]),
+    "or_block_seq_tiles": (None, [_int, _i64, _i64, _i64, _int, _int, _dp, _dp, _dp, _dp, _dp, _dp]),
     "or_Derror": (_dbl, [_dbl, _dbl]),
     "or_Dvec_diff": (ctypes.c_long, [_dp, _dp, _i64, _dbl]),
     "or_Dtest_equality": (ctypes.c_short, [_dp, _dp, _i64]),
@@ -152,6 +154,19 @@ def unpack_block_cyclic(packed, mb, nb, P, Q, G, ldg, rows, cols):
     return G
 
 
+def block_seq_tiles(col_major, M, N, K, size, gA, gB, gC):
+    """deliverMatrix's tiles of every rank (rank order, concatenated) for
+    flat global matrices in column-major or row-major storage."""
+    dRow, dCol = virtual_grid(size)
+    sizes = [(M // dRow + (M % dRow if i == dRow - 1 else 0), N // dCol + (N % dCol if j == dCol - 1 else 0))
+             for i in range(dRow) for j in range(dCol)]
+    tA = np.zeros(max(1, sum(tm * K for tm, _ in sizes)))
+    tB = np.zeros(max(1, sum(K * tn for _, tn in sizes)))
+    tC = np.zeros(max(1, sum(tm * tn for tm, tn in sizes)))
+    lib().or_block_seq_tiles(int(col_major), M, N, K, dRow, dCol, _p(gA), _p(gB), _p(gC), _p(tA), _p(tB), _p(tC))
+    return tA, tB, tC, sizes
+
+
 def layer_reduce(C, partial, n, nlayers):
     lib().or_layer_reduce(_p(C), _p(partial), n, nlayers)
     return C
@@ -181,8 +196,11 @@ def cannon25d(opA, opB, M, N, K, alpha, Ap, Bp, beta, Cp, p, c, sched_std=False)
     return Cp
 
 
-def summa(M, N, K, alpha, Ap, Bp, beta, Cp, mb, nb, kb, P, Q):
-    lib().or_summa(M, N, K, alpha, _p(Ap), _p(Bp), beta, _p(Cp), mb, nb, kb, P, Q)
+def summa(M, N, K, alpha, Ap, Bp, beta, Cp, mb, nb, kb, P, Q, opA="N", opB="N"):
+    if opA == "N" and opB == "N":
+        lib().or_summa(M, N, K, alpha, _p(Ap), _p(Bp), beta, _p(Cp), mb, nb, kb, P, Q)
+    else:
+        lib().or_summa_op(_op(opA), _op(opB), M, N, K, alpha, _p(Ap), _p(Bp), beta, _p(Cp), mb, nb, kb, P, Q)
     return Cp
 
 
@@ -379,6 +397,39 @@ def crt_bound_sums(X, e, axis):
     return t.sum(axis=axis).astype(np.int64)
 
 
+CRT_KCHUNK = 64   # k-chunk of the line bound scan
+
+
+def crt_line_bounds(X, axis, kc=CRT_KCHUNK):
+    """Line exponent e and norm bound S per row (axis=1) or column (axis=0),
+    from one pass over k-chunks of kc entries: chunk c gets its own max
+    exponent e_c (max|x| < 2^e_c) and the exact integer bound S_c of
+    crt_bound_sums at e_c; the line takes e = max e_c over its nonzero
+    chunks and S = sum_c ceil(S_c / 4^(e - e_c)), still an integer upper
+    bound of 2^32 * sum_k (x_k 2^-e)^2 whatever the summation order.  An
+    all-zero line gets e = 0, S = 0.  Returns (e int32, S int64)."""
+    Xl = np.asarray(X, dtype=np.float64)
+    if axis == 0:
+        Xl = Xl.T
+    lines, K = Xl.shape
+    nch = max(1, -(-K // kc))
+    Xc = np.zeros((lines, nch * kc))
+    Xc[:, :K] = Xl
+    Xc = Xc.reshape(lines * nch, kc)
+    amax = np.max(np.abs(Xc), axis=1)
+    _, ec = np.frexp(amax)
+    ec = ec.astype(np.int64)
+    Sc = crt_bound_sums(Xc, ec, 1).reshape(lines, nch)
+    nz = (amax != 0).reshape(lines, nch)
+    ec = ec.reshape(lines, nch)
+    e = np.where(nz.any(axis=1), np.max(np.where(nz, ec, np.iinfo(np.int64).min), axis=1), 0)
+    d2 = np.where(nz, 2 * (e.reshape(-1, 1) - ec), 0)
+    small = np.right_shift(Sc + np.left_shift(np.int64(1), np.minimum(d2, 62)) - 1, np.minimum(d2, 62))
+    t = np.where(d2 >= 62, np.where(Sc > 0, 1, 0), small)
+    S = np.where(nz, t, 0).sum(axis=1).astype(np.int64)
+    return e.astype(np.int32), S
+
+
 def crt_shifts(e, S, Q):
     """The scaling as an exponent e' = Q - sigma per line (the form crt_scale
     and the reconstruction take): sigma is the largest shift with
@@ -448,8 +499,8 @@ def ozaki2_dgemm(a, b, nmod, alpha=1.0, beta=0.0, c=None):
     q = crt_Q(K, nmod)
     if q is None:
         raise ValueError("%d moduli leave no bits for K=%d" % (nmod, K))
-    e = crt_shifts(oz_exponents(a, 1), crt_bound_sums(a, oz_exponents(a, 1), 1), q)
-    f = crt_shifts(oz_exponents(b, 0), crt_bound_sums(b, oz_exponents(b, 0), 0), q)
+    e = crt_shifts(*crt_line_bounds(a, 1), q)
+    f = crt_shifts(*crt_line_bounds(b, 0), q)
     Ai = crt_scale(a, e, 1, q)
     Bi = crt_scale(b, f, 0, q)
     ps = CRT_MODULI[:nmod]

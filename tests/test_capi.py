@@ -97,7 +97,9 @@ def test_argument_errors_need_no_device():
     st = L.dg_cannon25d(b"N", b"N", 8, 8, 8, 1.0, None, None, 0.0, None, 4, 2, dgamd.DG_SCHED_AUTO, None, 0, None)
     assert st == dgamd.DG_ERR_WORKSPACE
     st = L.dg_summa(b"N", b"T", 8, 8, 8, 1.0, None, None, 0.0, None, 2, 2, 2, 2, 2, None)
-    assert st == dgamd.DG_ERR_UNSUPPORTED
+    assert st == dgamd.DG_ERR_INVALID   # transposed operands are supported; the operands are null
+    st = L.dg_summa(b"N", b"Z", 8, 8, 8, 1.0, None, None, 0.0, None, 2, 2, 2, 2, 2, None)
+    assert st == dgamd.DG_ERR_INVALID and "op" in dgamd.last_error()
     st = L.dg_generate(1, 7, None, 10, None)
     assert st == dgamd.DG_ERR_INVALID
     import ctypes

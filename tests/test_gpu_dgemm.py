@@ -160,6 +160,39 @@ def test_config1_shape_2048(dev):
     print("Dtest_equality digits:", digits, "rel Frobenius:", err)
 
 
+def test_dtest_equality_on_gpu_result(dev):
+    """The reference's digit count (Validation.cpp:57-83) on a GPU DGEMM
+    result whose entries are bounded away from zero (A, B in [0.5, 1)):
+    the oracle's restatement, the reference's own Validation.cpp built from
+    its sources (oracle/_ref, when built) and the host API's drop-in
+    Dtest_equality (libdghost.so) report the same digit count, and the FP64
+    result agrees with the host dgemm to at least 13 digits everywhere."""
+    import ctypes
+    import os
+    m, n, k = 2048, 1536, 2048
+    rng = np.random.default_rng(17)
+    a = rng.uniform(0.5, 1.0, (m, k))
+    b = rng.uniform(0.5, 1.0, (k, n))
+    up = lambda x: torch.from_numpy(np.asfortranarray(x).ravel(order="F").copy()).to(dev)
+    A, B = up(a), up(b)
+    C = torch.empty(m * n, dtype=torch.float64, device=dev)
+    dgamd.dgemm_t("N", "N", m, n, k, 1.0, A, m, B, k, 0.0, C, m)
+    got = np.ascontiguousarray(C.cpu().numpy())
+    ref = np.ascontiguousarray((a @ b).T).ravel()    # column-major
+    digits = oracle.Dtest_equality(ref, got)
+    assert digits >= 13, digits
+    R = oracle.ref_lib()
+    if R is not None:
+        dp = ctypes.POINTER(ctypes.c_double)
+        assert R.ref_Dtest_equality(ref.ctypes.data_as(dp), got.ctypes.data_as(dp), ref.size) == digits
+    host = ctypes.CDLL(os.path.join(os.path.dirname(dgamd.__file__), "libdghost.so"))
+    f = host._Z14Dtest_equalityPdS_x   # short Dtest_equality(double*, double*, long long)
+    f.restype = ctypes.c_short
+    f.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double), ctypes.c_longlong]
+    dp = ctypes.POINTER(ctypes.c_double)
+    assert f(ref.ctypes.data_as(dp), got.ctypes.data_as(dp), ref.size) == digits
+
+
 def test_invalid_arguments(dev):
     L = dgamd.lib()
     assert L.dg_dgemm(b"X", b"N", 4, 4, 4, 1.0, None, 4, None, 4, 0.0, None, 4, None) == dgamd.DG_ERR_INVALID

@@ -154,6 +154,34 @@ def test_summa_exact_vs_oracle(dev, M, N, K, mb, nb, kb, P, Q):
     assert np.array_equal(G.reshape(N, M).T, full)
 
 
+@pytest.mark.parametrize("opA,opB", [("T", "N"), ("N", "T"), ("T", "T")])
+@pytest.mark.parametrize("M,N,K,mb,nb,kb,P,Q", [
+    (64, 64, 64, 8, 8, 8, 4, 4),
+    (100, 70, 50, 16, 8, 7, 3, 2),      # P != Q: a transposed panel spans several owners
+    (33, 65, 129, 4, 16, 32, 2, 4),
+    (40, 40, 40, 64, 64, 64, 2, 2),     # ranks with no data
+    (512, 256, 144, 128, 64, 24, 2, 2),
+])
+def test_summa_transposed_exact_vs_oracle(dev, opA, opB, M, N, K, mb, nb, kb, P, Q):
+    """op(A) = A^T with A stored K x M (blocks kb x mb), op(B) = B^T with B
+    stored N x K (blocks nb x kb), as cublasMpGemm's TransA / TransB."""
+    rng = np.random.default_rng(M * N + K + ord(opA) + 2 * ord(opB))
+    rA, cA = (M, K) if opA == "N" else (K, M)
+    rB, cB = (K, N) if opB == "N" else (N, K)
+    A = rng.integers(-4, 5, rA * cA).astype(np.float64)
+    B = rng.integers(-4, 5, rB * cB).astype(np.float64)
+    C = rng.integers(-4, 5, M * N).astype(np.float64)
+    bA = (mb, kb) if opA == "N" else (kb, mb)
+    bB = (kb, nb) if opB == "N" else (nb, kb)
+    Ap = oracle.pack_block_cyclic(A, rA, rA, cA, bA[0], bA[1], P, Q)
+    Bp = oracle.pack_block_cyclic(B, rB, rB, cB, bB[0], bB[1], P, Q)
+    Cp = oracle.pack_block_cyclic(C, M, M, N, mb, nb, P, Q)
+    ref = oracle.summa(M, N, K, -1.0, Ap, Bp, 0.5, Cp.copy(), mb, nb, kb, P, Q, opA, opB)
+    Ad, Bd, Cd = to_dev(Ap, dev), to_dev(Bp, dev), to_dev(Cp, dev)
+    dgamd.summa_t(opA, opB, M, N, K, -1.0, Ad, Bd, 0.5, Cd, mb, nb, kb, P, Q)
+    assert np.array_equal(Cd.cpu().numpy(), ref)
+
+
 def test_config2_summa_scaled(dev):
     """SUMMA 4x4, nb=512, alpha=1, beta=0.5 at M=N=K=4096 vs host dgemm."""
     n, nb, P = 4096, 512, 4
@@ -183,4 +211,6 @@ def test_grid_argument_errors(dev):
     st = L.dg_cannon25d(b"N", b"N", 10, 9, 9, 1.0, None, None, 0.0, None, 3, 1, dgamd.DG_SCHED_STANDARD, None, 0, None)
     assert st == dgamd.DG_ERR_INVALID
     st = L.dg_summa(b"T", b"N", 8, 8, 8, 1.0, None, None, 0.0, None, 2, 2, 2, 2, 2, None)
-    assert st == dgamd.DG_ERR_UNSUPPORTED
+    assert st == dgamd.DG_ERR_INVALID   # null operands
+    st = L.dg_summa(b"X", b"N", 8, 8, 8, 1.0, None, None, 0.0, None, 2, 2, 2, 2, 2, None)
+    assert st == dgamd.DG_ERR_INVALID

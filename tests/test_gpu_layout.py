@@ -42,6 +42,41 @@ def test_block2d_pack_unpack(dev, rows, cols, ldg, pr, pc):
     assert np.all(g2[:, rows:] == 3.5)  # rows beyond the matrix untouched
 
 
+@pytest.mark.parametrize("col_major", [True, False])
+@pytest.mark.parametrize("M,N,K,size", [(23, 17, 5, 6), (300, 260, 90, 35), (16, 16, 3, 4), (9, 30, 1, 7)])
+def test_block_sequential_tiles(dev, col_major, M, N, K, size):
+    """BlockSequentialDecomposer's layouts, as its host code packs them
+    (row-major globals through their column-major transposes), hold every
+    rank's deliverMatrix tiles bit for bit (reference branch per storage order)."""
+    rng = np.random.default_rng(M + N + K + size + int(col_major))
+    gA, gB, gC = (rng.standard_normal(n) for n in (M * K, K * N, M * N))
+    tA, tB, tC, sizes = oracle.block_seq_tiles(col_major, M, N, K, size, gA, gB, gC)
+    dRow, dCol = oracle.virtual_grid(size)
+    ro, co, ko = oracle.block_splits(M, dRow), oracle.block_splits(N, dCol), [0, K]
+    PA = torch.empty(M * K, dtype=torch.float64, device=dev)
+    PB = torch.empty(K * N, dtype=torch.float64, device=dev)
+    PC = torch.empty(M * N, dtype=torch.float64, device=dev)
+    if col_major:
+        dgamd.pack_block2d_t(dev_copy(gA, dev), M, M, K, ro, ko, PA)
+        dgamd.pack_block2d_t(dev_copy(gB, dev), K, K, N, ko, co, PB)
+        dgamd.pack_block2d_t(dev_copy(gC, dev), M, M, N, ro, co, PC)
+    else:
+        dgamd.pack_block2d_t(dev_copy(gA, dev), K, K, M, ko, ro, PA)
+        dgamd.pack_block2d_t(dev_copy(gB, dev), N, N, K, co, ko, PB)
+        dgamd.pack_block2d_t(dev_copy(gC, dev), N, N, M, co, ro, PC)
+    pa, pb, pc = PA.cpu().numpy(), PB.cpu().numpy(), PC.cpu().numpy()
+    oa = ob = oc = 0
+    for r, (tm, tn) in enumerate(sizes):
+        i, j = divmod(r, dCol)
+        a = pa[ro[i] * K: ro[i] * K + tm * K]
+        b = pb[co[j] * K: co[j] * K + K * tn]
+        cb = (ro[i] * N + tm * co[j]) if col_major else (co[j] * M + tn * ro[i])
+        assert np.array_equal(a, tA[oa: oa + tm * K]), r
+        assert np.array_equal(b, tB[ob: ob + K * tn]), r
+        assert np.array_equal(pc[cb: cb + tm * tn], tC[oc: oc + tm * tn]), r
+        oa, ob, oc = oa + tm * K, ob + K * tn, oc + tm * tn
+
+
 @pytest.mark.parametrize("rows,cols,mb,nb,P,Q", [(23, 17, 3, 2, 7, 5), (64, 64, 8, 8, 4, 4), (1000, 777, 64, 100, 3, 2),
                                                  (5, 5, 8, 8, 2, 2), (512, 520, 512, 512, 4, 4)])
 def test_block_cyclic_pack_unpack(dev, rows, cols, mb, nb, P, Q):

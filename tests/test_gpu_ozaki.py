@@ -162,6 +162,41 @@ def test_crt_ragged_and_zero_lines(dev):
     assert np.array_equal(got, a @ b)
 
 
+def upload_padded(x2d, ld, offset, dev):
+    """x2d stored column-major with leading dimension ld >= rows, starting
+    `offset` doubles into a NaN-filled device buffer; returns (buffer, view)."""
+    r, c = x2d.shape
+    buf = np.full(offset + ld * c + 1, np.nan)
+    for j in range(c):
+        buf[offset + j * ld: offset + j * ld + r] = x2d[:, j]
+    t = torch.from_numpy(buf).to(dev)
+    return t, t[offset:]
+
+
+@pytest.mark.parametrize("opA,opB,m,n,k,pad,offset", [
+    ("T", "N", 70, 50, 201, 1, 0),    # k-contiguous lines, odd k, even ld (16-byte pair loads, scalar tail)
+    ("T", "N", 70, 50, 201, 1, 1),    # base 8 bytes off 16-byte alignment: a wave per line
+    ("N", "T", 65, 33, 130, 3, 1),    # strided lines, partial last chunk, odd ld
+    ("T", "T", 40, 90, 64, 2, 0),     # one whole chunk
+    ("N", "N", 33, 47, 257, 0, 1),    # a single entry in the last chunk
+])
+def test_crt_scan_layouts_bit_exact(dev, opA, opB, m, n, k, pad, offset):
+    rng = np.random.default_rng(m * n + k + pad + offset)
+    a = wide(rng, (m, k) if opA == "N" else (k, m))
+    b = wide(rng, (k, n) if opB == "N" else (n, k))
+    a[:, 1] *= 2.0 ** -35   # a column / row far below the others
+    lda, ldb = a.shape[0] + pad, b.shape[0] + pad
+    _, A = upload_padded(a, lda, offset, dev)
+    _, B = upload_padded(b, ldb, offset, dev)
+    C = torch.full((m * n,), float("nan"), dtype=torch.float64, device=dev)
+    dgamd.ozaki_dgemm_t(opA, opB, m, n, k, 1.0, A, lda, B, ldb, 0.0, C, m, 14, dgamd.DG_OZAKI_INT8_CRT)
+    got = C.cpu().numpy().reshape(n, m).T
+    aa = np.ascontiguousarray(a if opA == "N" else a.T)
+    bb = np.ascontiguousarray(b if opB == "N" else b.T)
+    ref = oracle.ozaki2_dgemm(aa, bb, 14)
+    assert np.array_equal(got.view(np.uint64), ref.view(np.uint64))
+
+
 @pytest.mark.parametrize("dist", ["uniform", "wide"])
 def test_crt_error_per_moduli(dev, dist):
     """1024 x 1024 x 16384: error vs the host FP64 dgemm falls ~2^-8 per

@@ -274,6 +274,60 @@ def test_summa_restatement():
     assert np.array_equal(G.reshape(N, M).T, ref)
 
 
+@pytest.mark.parametrize("opA,opB", [("N", "N"), ("T", "N"), ("N", "T"), ("T", "T")])
+@pytest.mark.parametrize("P,Q", [(3, 2), (2, 2), (1, 3)])
+def test_summa_op_restatement(opA, opB, P, Q):
+    """or_summa_op against the textbook product on exact integers; with
+    'N' operands it equals or_summa bit for bit."""
+    rng = np.random.default_rng(P * 10 + Q + ord(opA) + 3 * ord(opB))
+    M, N, K, mb, nb, kb = 23, 17, 19, 3, 2, 4
+    rA, cA = (M, K) if opA == "N" else (K, M)
+    rB, cB = (K, N) if opB == "N" else (N, K)
+    A = rng.integers(-3, 4, rA * cA).astype(float)
+    B = rng.integers(-3, 4, rB * cB).astype(float)
+    C = rng.integers(-3, 4, M * N).astype(float)
+    Ap = oracle.pack_block_cyclic(A, rA, rA, cA, *((mb, kb) if opA == "N" else (kb, mb)), P, Q)
+    Bp = oracle.pack_block_cyclic(B, rB, rB, cB, *((kb, nb) if opB == "N" else (nb, kb)), P, Q)
+    Cp = oracle.pack_block_cyclic(C, M, M, N, mb, nb, P, Q)
+    got = oracle.lib()
+    out = Cp.copy()
+    got.or_summa_op(opA.encode(), opB.encode(), M, N, K, 1.0, oracle._p(Ap), oracle._p(Bp), -0.5, oracle._p(out),
+                    mb, nb, kb, P, Q)
+    G = np.zeros(M * N)
+    oracle.unpack_block_cyclic(out, mb, nb, P, Q, G, M, M, N)
+    ref = oracle.host_dgemm(opA, opB, 1.0, A.reshape(cA, rA).T, B.reshape(cB, rB).T, -0.5, C.reshape(N, M).T)
+    assert np.array_equal(G.reshape(N, M).T, ref)
+    if opA == opB == "N":
+        assert np.array_equal(out, oracle.summa(M, N, K, 1.0, Ap, Bp, -0.5, Cp.copy(), mb, nb, kb, P, Q))
+
+
+@pytest.mark.parametrize("col_major", [True, False])
+def test_block_seq_tiles_definition(col_major):
+    """Every rank's tile element equals the global element it stands for
+    (uneven 3x2 grid, remainders in the last row and column)."""
+    M, N, K, size = 23, 17, 5, 6
+    gA = np.arange(M * K, dtype=float)
+    gB = np.arange(K * N, dtype=float) + 0.5
+    gC = np.arange(M * N, dtype=float) + 0.25
+    tA, tB, tC, sizes = oracle.block_seq_tiles(col_major, M, N, K, size, gA, gB, gC)
+    dRow, dCol = oracle.virtual_grid(size)
+    assert (dRow, dCol) == (3, 2)
+    A = gA.reshape(K, M).T if col_major else gA.reshape(M, K)
+    B = gB.reshape(N, K).T if col_major else gB.reshape(K, N)
+    C = gC.reshape(N, M).T if col_major else gC.reshape(M, N)
+    oa = ob = oc = 0
+    for r, (tm, tn) in enumerate(sizes):
+        i, j = divmod(r, dCol)
+        r0, c0 = (M // dRow) * i, (N // dCol) * j
+        assert tm == (M // dRow if i < dRow - 1 else M - r0) and tn == (N // dCol if j < dCol - 1 else N - c0)
+        def tile(flat, rows, cols):
+            return flat.reshape(cols, rows).T if col_major else flat.reshape(rows, cols)
+        assert np.array_equal(tile(tA[oa: oa + tm * K], tm, K), A[r0: r0 + tm, :])
+        assert np.array_equal(tile(tB[ob: ob + K * tn], K, tn), B[:, c0: c0 + tn])
+        assert np.array_equal(tile(tC[oc: oc + tm * tn], tm, tn), C[r0: r0 + tm, c0: c0 + tn])
+        oa, ob, oc = oa + tm * K, ob + K * tn, oc + tm * tn
+
+
 # ------------------------------------------------------------------ validation
 
 def planted(base, digits_err):

@@ -169,6 +169,55 @@ def test_crt_bound_sums_is_a_tight_upper_bound():
         assert Fraction(int(S[i])) <= exact * Fraction(102, 100) + 300
 
 
+def test_crt_line_bounds_one_chunk_is_the_line_sum():
+    # K <= the chunk: the chunk exponent is the line's and nothing is rescaled
+    rng = np.random.default_rng(4)
+    X = wide(rng, (9, oracle.CRT_KCHUNK))
+    X[3] = 0.0
+    e, S = oracle.crt_line_bounds(X, 1)
+    assert e.tolist() == oracle.oz_exponents(X, 1).tolist()
+    assert S.tolist() == oracle.crt_bound_sums(X, oracle.oz_exponents(X, 1), 1).tolist()
+    e0, S0 = oracle.crt_line_bounds(X.T, 0)
+    assert e0.tolist() == e.tolist() and S0.tolist() == S.tolist()
+
+
+def test_crt_line_bounds_known_answers():
+    kc = oracle.CRT_KCHUNK
+    # chunk 0: 0.75 (e_c = 0); chunk 1: 0.1 = 0.8 * 2^-3 (e_c = -3): ceil(204.8) = 205,
+    # 205^2 2^16 rescaled by 4^-3 exactly
+    x = np.zeros((1, 2 * kc))
+    x[0, 5] = 0.75
+    x[0, kc + 7] = 0.1
+    e, S = oracle.crt_line_bounds(x, 1)
+    assert e.tolist() == [0] and S.tolist() == [192 ** 2 * 2 ** 16 + 205 ** 2 * 2 ** 10]
+    # chunk 1 holds two entries 0.6 * 2^-40 (e_c = -40): 2 * 154^2 * 2^16 / 4^40 rounds up to 1
+    # (the per-entry bound at the line exponent counts each as 1)
+    x[0, kc + 7] = 0.6 * 2.0 ** -40
+    x[0, kc + 9] = -0.6 * 2.0 ** -40
+    e, S = oracle.crt_line_bounds(x, 1)
+    assert S.tolist() == [192 ** 2 * 2 ** 16 + 1]
+    assert oracle.crt_bound_sums(x, e, 1).tolist() == [192 ** 2 * 2 ** 16 + 2]
+    # a zero chunk does not set the line exponent; an all-zero line is (0, 0)
+    y = np.zeros((2, 3 * kc))
+    y[0, 2 * kc + 1] = 0.3 * 2.0 ** -12
+    e, S = oracle.crt_line_bounds(y, 1)
+    _, ex = np.frexp(0.3 * 2.0 ** -12)
+    assert e.tolist() == [ex, 0] and S.tolist() == [int(np.ceil(0.6 * 256)) ** 2 * 2 ** 16, 0]
+
+
+def test_crt_line_bounds_tight_upper_bound():
+    rng = np.random.default_rng(8)
+    X = wide(rng, (12, 300))
+    X[:, 64:128] *= 2.0 ** -30       # a chunk far below the line maximum
+    e, S = oracle.crt_line_bounds(X, 1)
+    assert e.tolist() == oracle.oz_exponents(X, 1).tolist()
+    nch = -(-300 // oracle.CRT_KCHUNK)
+    for i in range(12):
+        exact = sum((Fraction(float(x)) * Fraction(2) ** (-int(e[i]))) ** 2 for x in X[i]) * 2 ** 32
+        assert Fraction(int(S[i])) >= exact
+        assert Fraction(int(S[i])) <= exact * Fraction(102, 100) + nch
+
+
 @pytest.mark.parametrize("nmod", [6, 10, 14, 16])
 def test_crt_shifts_bound_the_product(nmod):
     # |A'B'| <= P/4 with exact integers, and each shift is the largest the norm bound allows
@@ -178,15 +227,14 @@ def test_crt_shifts_bound_the_product(nmod):
     a = wide(rng, (6, K))
     b = rng.uniform(-1, 1, (K, 5))
     Q = oracle.crt_Q(K, nmod)
-    ea = oracle.crt_shifts(oracle.oz_exponents(a, 1), oracle.crt_bound_sums(a, oracle.oz_exponents(a, 1), 1), Q)
-    fb = oracle.crt_shifts(oracle.oz_exponents(b, 0), oracle.crt_bound_sums(b, oracle.oz_exponents(b, 0), 0), Q)
+    ea = oracle.crt_shifts(*oracle.crt_line_bounds(a, 1), Q)
+    fb = oracle.crt_shifts(*oracle.crt_line_bounds(b, 0), Q)
     A = oracle.crt_scale(a, ea, 1, Q)
     B = oracle.crt_scale(b, fb, 0, Q)
     P = math.prod(oracle.CRT_MODULI[:nmod])
     V = [[sum(int(A[i, k]) * int(B[k, j]) for k in range(K)) for j in range(5)] for i in range(6)]
     assert max(abs(v) for r in V for v in r) <= P // 4
-    e = oracle.oz_exponents(a, 1)
-    S = oracle.crt_bound_sums(a, e, 1)
+    e, S = oracle.crt_line_bounds(a, 1)
     for i in range(6):
         sig = Q - int(ea[i])
         assert 2 ** (2 * (sig + int(e[i]))) * int(S[i]) <= 2 ** (2 * Q + 32) or sig == 53 - int(e[i])
@@ -222,8 +270,8 @@ def test_crt_error_within_bound(nmod):
     Q = oracle.crt_Q(40, nmod)
     e = oracle.oz_exponents(a, 1)
     f = oracle.oz_exponents(b, 0)
-    ea = oracle.crt_shifts(e, oracle.crt_bound_sums(a, e, 1), Q)
-    fb = oracle.crt_shifts(f, oracle.crt_bound_sums(b, f, 0), Q)
+    ea = oracle.crt_shifts(*oracle.crt_line_bounds(a, 1), Q)
+    fb = oracle.crt_shifts(*oracle.crt_line_bounds(b, 0), Q)
     for i in range(5):
         for j in range(6):
             exact = sum(Fraction(a[i, k]) * Fraction(b[k, j]) for k in range(40))
