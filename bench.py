@@ -411,6 +411,7 @@ def hbm_leg(dev, reps=5):
           "16384^2, 512x512 blocks on 4x4")
     timed("copy_block", 16 * (n - 8) * (n - 8),
           lambda: dgamd.copy_block_t(n - 8, n - 8, G, n, Pk, n), "16376^2 of ld 16384")
+    timed("torch_copy (calibration)", 16 * n * n, lambda: Pk.copy_(G), "16384^2, torch's device copy")
     del G, Pk
     M, N, K, p = 16384, 8192, 32768, 4
     G = torch.empty(M * K, **f64)

@@ -8,8 +8,10 @@ namespace dg {
 
 namespace {
 
-__device__ __forceinline__ uint64_t splitmix64_at(uint64_t seed, uint64_t i) {
-    uint64_t z = seed + (i + 1) * 0x9E3779B97F4A7C15ull;
+constexpr uint64_t kGamma = 0x9E3779B97F4A7C15ull;
+
+// The output function applied to the generator state z = seed + (i+1) gamma.
+__device__ __forceinline__ uint64_t splitmix64_mix(uint64_t z) {
     z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
     z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
     return z ^ (z >> 31);
@@ -18,33 +20,30 @@ __device__ __forceinline__ uint64_t splitmix64_at(uint64_t seed, uint64_t i) {
 // Second, independent stream for the exponent of the wide distribution.
 constexpr uint64_t kExpSeedXor = 0xD1B54A32D192ED03ull;
 
-__device__ __forceinline__ double generated_value(uint64_t seed, int dist, int64_t i) {
-    const uint64_t z = splitmix64_at(seed, (uint64_t)i);
+// Element value from the states of the value stream (z) and, for the wide
+// distribution, of the exponent stream (ze).
+__device__ __forceinline__ double value_of(uint64_t zs, int dist, uint64_t zes) {
+    const uint64_t z = splitmix64_mix(zs);
     const double u01 = (double)(z >> 11) * 0x1.0p-53;  // exact: 53-bit integer times 2^-53
     if (dist == DG_DIST_UNIFORM_01) return u01;
     double v = 2.0 * u01 - 1.0;  // exact in [-1, 1)
     if (dist == DG_DIST_WIDE) {
-        const uint64_t ze = splitmix64_at(seed ^ kExpSeedXor, (uint64_t)i);
+        const uint64_t ze = splitmix64_mix(zes);
         const int e = (int)((ze >> 32) % 41ull) - 20;
         v = ldexp(v, e);  // exact power-of-two scaling
     }
     return v;
 }
 
-typedef double gdv2 __attribute__((ext_vector_type(2)));
-
-// Two consecutive elements per thread and pass, one 16-byte store (the
-// buffer is 16-byte aligned: checked by the host); an odd last element is
-// written by thread 0.
-__global__ void generate_kernel(uint64_t seed, int dist, double* __restrict__ out, int64_t n) {
-    const int64_t np = n / 2;
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < np; i += (int64_t)gridDim.x * blockDim.x)
-        *reinterpret_cast<gdv2*>(out + 2 * i) = gdv2{generated_value(seed, dist, 2 * i), generated_value(seed, dist, 2 * i + 1)};
-    if ((n & 1) && blockIdx.x == 0 && threadIdx.x == 0) out[n - 1] = generated_value(seed, dist, n - 1);
+__device__ __forceinline__ double generated_value(uint64_t seed, int dist, int64_t i) {
+    const uint64_t step = ((uint64_t)i + 1) * kGamma;
+    return value_of(seed + step, dist, (seed ^ kExpSeedXor) + step);
 }
 
-// Unaligned buffers: one element per thread and pass.
-__global__ void generate_unaligned_kernel(uint64_t seed, int dist, double* __restrict__ out, int64_t n) {
+// One element per thread and pass: generation is bound by the 64-bit
+// multiplies of the output function (about 10 32-bit multiplies per
+// element), not by the 8-byte stores.
+__global__ void generate_kernel(uint64_t seed, int dist, double* __restrict__ out, int64_t n) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         out[i] = generated_value(seed, dist, i);
 }
@@ -72,12 +71,7 @@ static int dg_generate_common(uint64_t seed, int dist, T* ptr, int64_t n, void* 
     int64_t blocks = (n + 255) / 256;
     if (blocks > 8192) blocks = 8192;
     if constexpr (sizeof(T) == 8) {
-        if (reinterpret_cast<uintptr_t>(ptr) & 15)
-            hipLaunchKernelGGL(generate_unaligned_kernel, dim3((unsigned)blocks), dim3(256), 0, as_stream(stream), seed,
-                               dist, ptr, n);
-        else
-            hipLaunchKernelGGL(generate_kernel, dim3((unsigned)blocks), dim3(256), 0, as_stream(stream), seed, dist,
-                               ptr, n);
+        hipLaunchKernelGGL(generate_kernel, dim3((unsigned)blocks), dim3(256), 0, as_stream(stream), seed, dist, ptr, n);
     } else {
         hipLaunchKernelGGL(generate_f32_kernel, dim3((unsigned)blocks), dim3(256), 0, as_stream(stream), seed, dist, ptr, n);
     }

@@ -206,6 +206,55 @@ int build_groups(dg_grouped_plan* p, const dg_gemm_desc* descs, int count, const
 // Output bytes of problem i (its C footprint).
 size_t c_bytes(const dg_gemm_desc& d) { return sizeof(double) * (size_t)(d.ldc * (d.n - 1) + d.m); }
 
+// Timing trials write the outputs; this scope saves them first and, on every
+// exit path (a failed launch included), writes them back, waits, frees the
+// snapshot and destroys the two timing events.  `saved` is set only once the
+// whole snapshot is enqueued, so a failure while saving restores nothing.
+struct TrialScope {
+    hipStream_t st = nullptr;
+    std::vector<std::pair<double*, size_t>> spans;
+    char* snap = nullptr;
+    bool saved = false;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+
+    int save(hipStream_t stream, std::vector<std::pair<double*, size_t>> outs) {
+        st = stream;
+        spans = std::move(outs);
+        size_t bytes = 0;
+        for (auto& sp : spans) bytes += sp.second;
+        if (bytes) {
+            if (hipMalloc(&snap, bytes) != hipSuccess) {
+                snap = nullptr;
+                return fail(DG_ERR_ALLOC, "autotune: snapshot of %zu bytes", bytes);
+            }
+            size_t off = 0;
+            for (auto& sp : spans) {
+                DG_HIP(hipMemcpyAsync(snap + off, sp.first, sp.second, hipMemcpyDeviceToDevice, st));
+                off += sp.second;
+            }
+            saved = true;
+        }
+        DG_HIP(hipEventCreate(&e0));
+        DG_HIP(hipEventCreate(&e1));
+        return DG_OK;
+    }
+    ~TrialScope() {
+        if (saved) {
+            size_t off = 0;
+            for (auto& sp : spans) {
+                (void)hipMemcpyAsync(sp.first, snap + off, sp.second, hipMemcpyDeviceToDevice, st);
+                off += sp.second;
+            }
+        }
+        if (snap) {
+            (void)hipStreamSynchronize(st);
+            (void)hipFree(snap);
+        }
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+    }
+};
+
 // Times one problem alone (best of three): FP64 MFMA, or Ozaki with the
 // plan's workspace.
 int time_problem(const dg_grouped_plan* p, int i, bool ozaki, hipStream_t st, hipEvent_t e0, hipEvent_t e1,
@@ -249,9 +298,6 @@ int choose_forms(dg_grouped_plan* p, hipStream_t st) {
         if (hipMalloc(&p->oz_ws, (size_t)ws) != hipSuccess) return fail(DG_ERR_ALLOC, "plan: Ozaki workspace of %lld bytes", (long long)ws);
         p->oz_ws_bytes = ws;
         TuneCache& c = tune_cache();
-        hipEvent_t e0, e1;
-        DG_HIP(hipEventCreate(&e0));
-        DG_HIP(hipEventCreate(&e1));
         for (int i : cand) {
             const dg_gemm_desc& d = p->descs[i];
             char key[160];
@@ -265,26 +311,22 @@ int choose_forms(dg_grouped_plan* p, hipStream_t st) {
                 if (it != c.best.end()) choice = it->second;
             }
             if (choice < 0) {
-                // snapshot C, time both forms, restore C
-                double* snap = nullptr;
-                const size_t cb = c_bytes(d);
-                if (hipMalloc(&snap, cb) != hipSuccess) return fail(DG_ERR_ALLOC, "autotune: snapshot of %zu bytes", cb);
-                DG_HIP(hipMemcpyAsync(snap, d.C, cb, hipMemcpyDeviceToDevice, st));
+                // time both forms on this problem's own operands; C is restored
+                // when `trial` ends, whether or not the trials succeed
                 float t64 = 0.f, toz = 0.f;
-                int rc = time_problem(p, i, false, st, e0, e1, &t64);
-                if (!rc) rc = time_problem(p, i, true, st, e0, e1, &toz);
-                DG_HIP(hipMemcpyAsync(d.C, snap, cb, hipMemcpyDeviceToDevice, st));
-                DG_HIP(hipStreamSynchronize(st));
-                (void)hipFree(snap);
-                if (rc) return rc;
+                {
+                    TrialScope trial;
+                    int rc = trial.save(st, {{d.C, c_bytes(d)}});
+                    if (!rc) rc = time_problem(p, i, false, st, trial.e0, trial.e1, &t64);
+                    if (!rc) rc = time_problem(p, i, true, st, trial.e0, trial.e1, &toz);
+                    if (rc) return rc;
+                }
                 choice = toz < t64 ? DG_FORM_OZAKI_INT8 : DG_FORM_FP64;
                 std::lock_guard<std::mutex> lock(c.mu);
                 tune_store_locked(c, key, choice);
             }
             p->prob_form[i] = choice;
         }
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
     }
     std::vector<char> use(count, 1);
     p->oz_idx.clear();
@@ -308,7 +350,8 @@ int choose_forms(dg_grouped_plan* p, hipStream_t st) {
 
 // Times each scheduling variant (best of three) on the plan's own operands
 // and keeps the fastest.  The outputs are saved before the trials and
-// restored after them, so tuning leaves C as it found it.
+// restored after them, also when a trial fails, so tuning leaves C as it
+// found it.
 int autotune(dg_grouped_plan* p, hipStream_t st) {
     TuneCache& c = tune_cache();
     {
@@ -319,47 +362,28 @@ int autotune(dg_grouped_plan* p, hipStream_t st) {
     }
     const int variants[3] = {DG_GEMM_MODE_XCD, DG_GEMM_MODE_ROUND_ROBIN, DG_GEMM_MODE_PERSISTENT};
     const int cfgs[2] = {DG_GEMM_TILE_128x128, DG_GEMM_TILE_256x128};
-    size_t snap_bytes = 0;
-    for (auto& sp : p->c_spans) snap_bytes += sp.second;
-    char* snap = nullptr;
-    if (snap_bytes) {
-        if (hipMalloc(&snap, snap_bytes) != hipSuccess) return fail(DG_ERR_ALLOC, "autotune: snapshot of %zu bytes", snap_bytes);
-        size_t off = 0;
-        for (auto& sp : p->c_spans) {
-            DG_HIP(hipMemcpyAsync(snap + off, sp.first, sp.second, hipMemcpyDeviceToDevice, st));
-            off += sp.second;
-        }
-    }
-    hipEvent_t e0, e1;
-    DG_HIP(hipEventCreate(&e0));
-    DG_HIP(hipEventCreate(&e1));
     float best_ms = 1e30f;
     int best = DG_GEMM_MODE_XCD;  // encoded cfg * 4 + scheduling mode
-    for (int cfg : cfgs)
-        for (int v : variants) {
-            float vbest = 1e30f;
-            for (int rep = 0; rep < 3; ++rep) {
-                DG_HIP(hipEventRecord(e0, st));
-                int rc = plan_launch(p, v, cfg, st, false);
-                if (rc) return rc;
-                DG_HIP(hipEventRecord(e1, st));
-                DG_HIP(hipEventSynchronize(e1));
-                float ms = 0.f;
-                DG_HIP(hipEventElapsedTime(&ms, e0, e1));
-                vbest = std::min(vbest, ms);
+    {
+        // the outputs are restored when `trial` ends, whether or not the trials succeed
+        TrialScope trial;
+        int rc = trial.save(st, p->c_spans);
+        if (rc) return rc;
+        for (int cfg : cfgs)
+            for (int v : variants) {
+                float vbest = 1e30f;
+                for (int rep = 0; rep < 3; ++rep) {
+                    DG_HIP(hipEventRecord(trial.e0, st));
+                    rc = plan_launch(p, v, cfg, st, false);
+                    if (rc) return rc;
+                    DG_HIP(hipEventRecord(trial.e1, st));
+                    DG_HIP(hipEventSynchronize(trial.e1));
+                    float ms = 0.f;
+                    DG_HIP(hipEventElapsedTime(&ms, trial.e0, trial.e1));
+                    vbest = std::min(vbest, ms);
+                }
+                if (vbest < best_ms) { best_ms = vbest; best = cfg * 4 + v; }
             }
-            if (vbest < best_ms) { best_ms = vbest; best = cfg * 4 + v; }
-        }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    if (snap) {
-        size_t off = 0;
-        for (auto& sp : p->c_spans) {
-            DG_HIP(hipMemcpyAsync(sp.first, snap + off, sp.second, hipMemcpyDeviceToDevice, st));
-            off += sp.second;
-        }
-        DG_HIP(hipStreamSynchronize(st));
-        (void)hipFree(snap);
     }
     p->variant = best % 4;
     p->cfg = best / 4;

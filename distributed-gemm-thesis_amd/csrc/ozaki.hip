@@ -404,10 +404,20 @@ __host__ __device__ constexpr int crt_modulus(int t) {
 // ceil(p/2) - 1] from an exact split of A' (below), five FP64 operations
 // per element and modulus.
 constexpr int RT_R = 16, RT_K = 128, RT_T = 256 / RT_R, RT_KPT = RT_K / RT_T;  // threads per row, k per thread
+// Per-modulus constants of the residue pass, passed as kernel arguments so
+// that they sit in scalar registers (a VOP3 FMA takes them directly; gfx9
+// VOP3 encodings take no 64-bit literals).
+struct ResidueConsts {
+    double c24[kMaxModuli];   // 2^24 mod p
+    double inv_p[kMaxModuli]; // fl(1/p)
+    double pd[kMaxModuli];    // p
+};
+
 template <int NMOD>
 __global__ __launch_bounds__(256) void crt_residue_kernel(const double* __restrict__ x, int64_t rows, int64_t K,
                                                           int64_t rs, int64_t ks, const int32_t* __restrict__ e, int q,
-                                                          int8_t* __restrict__ out, int64_t Kp, int64_t slice_bytes) {
+                                                          int8_t* __restrict__ out, int64_t Kp, int64_t slice_bytes,
+                                                          ResidueConsts rc) {
     __shared__ double tile[RT_R][RT_K + 1];
     const int64_t r0 = (int64_t)blockIdx.y * RT_R;
     const int64_t k0 = (int64_t)blockIdx.x * RT_K;
@@ -445,22 +455,20 @@ __global__ __launch_bounds__(256) void crt_residue_kernel(const double* __restri
     int8_t* dst = out + r * Kp + k0 + kq;
 #pragma unroll
     for (int m = 0; m < NMOD; ++m) {
-        {
-            constexpr double one = 1.0;
-            const int p = crt_modulus(m);
-            const double pd = p, inv_p = one / crt_modulus(m);
-            const double cp = (double)((1 << 24) % crt_modulus(m));
-            static_assert(RT_KPT == 8, "one 8-byte store per modulus");
-            uint32_t w[2] = {0, 0};
+        static_assert(RT_KPT == 8, "one 8-byte store per modulus");
+        uint32_t v[RT_KPT];
 #pragma unroll
-            for (int c = 0; c < RT_KPT; ++c) {
-                const double y = m == 0 ? al[c] : fma(ah[c], cp, al[c]);
-                const double qd = floor(fma(y, inv_p, 0.5));
-                const int v = (int)fma(-qd, pd, y);
-                w[c >> 2] |= (uint32_t)(v & 255) << (8 * (c & 3));
-            }
-            *reinterpret_cast<uint2*>(dst + m * slice_bytes) = uint2{w[0], w[1]};
+        for (int c = 0; c < RT_KPT; ++c) {
+            const double y = m == 0 ? al[c] : fma(ah[c], rc.c24[m], al[c]);
+            const double qd = floor(fma(y, rc.inv_p[m], 0.5));
+            v[c] = (uint32_t)(int)fma(-qd, rc.pd[m], y);
         }
+        // bytes 0 of four words into one word: two selections of two bytes, then one of the two halves
+        const uint32_t w0 = __builtin_amdgcn_perm(__builtin_amdgcn_perm(v[3], v[2], 0x0c0c0400u),
+                                                  __builtin_amdgcn_perm(v[1], v[0], 0x0c0c0400u), 0x05040100u);
+        const uint32_t w1 = __builtin_amdgcn_perm(__builtin_amdgcn_perm(v[7], v[6], 0x0c0c0400u),
+                                                  __builtin_amdgcn_perm(v[5], v[4], 0x0c0c0400u), 0x05040100u);
+        *reinterpret_cast<uint2*>(dst + m * slice_bytes) = uint2{w0, w1};
     }
 }
 
@@ -843,10 +851,10 @@ __device__ __forceinline__ double crt_bound_term(double x, int e) {
 // e_c) (at most CRT_KC * 2^32 = 2^38), stored as one word
 // ((e_c + 2048) << 48) | S_c, or 0 for an all-zero chunk.  crt_shift_kernel
 // rescales the chunk bounds to the line's max exponent.  Rows strided in
-// memory (ks != 1): a thread per line, consecutive lanes on consecutive
-// lines, the chunk's CRT_KC values held in registers between the max and
-// the sum.  Lines contiguous in k: 16 lanes per line with 16-byte pair
-// loads when every line starts 16-byte aligned, else a wave per line.
+// memory (ks != 1): a 64-line x 64-k tile through LDS, loaded along the
+// lines.  Lines contiguous in k: 16 lanes per line with 16-byte pair loads
+// when every line starts 16-byte aligned, else a wave per line.  Every term
+// and chunk sum is an integer below 2^39, so the sums are exact in any order.
 constexpr int CRT_KC = 64;
 
 __device__ __forceinline__ uint64_t crt_chunk_word(double amax, uint64_t sum) {
@@ -862,33 +870,11 @@ __device__ __forceinline__ int crt_chunk_exp(double amax) {
     return ec;
 }
 
-__global__ __launch_bounds__(256) void crt_scan_kernel(const double* __restrict__ x, int64_t rows, int64_t K,
-                                                       int64_t rs, int64_t ks, uint64_t* __restrict__ part) {
-    const int64_t k0 = (int64_t)blockIdx.y * CRT_KC;
-    if (ks != 1) {
-        const int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-        if (r >= rows) return;
-        const double* xr = x + r * rs;
-        double v[CRT_KC];
-        double mx = 0.0;
-        if (k0 + CRT_KC <= K) {
-#pragma unroll
-            for (int c = 0; c < CRT_KC; ++c) v[c] = xr[(k0 + c) * ks];
-        } else {
-#pragma unroll
-            for (int c = 0; c < CRT_KC; ++c) v[c] = k0 + c < K ? xr[(k0 + c) * ks] : 0.0;
-        }
-#pragma unroll
-        for (int c = 0; c < CRT_KC; ++c) mx = fmax(mx, fabs(v[c]));
-        const int ec = crt_chunk_exp(mx);
-        double s0 = 0.0, s1 = 0.0;
-#pragma unroll
-        for (int c = 0; c < CRT_KC; c += 2) {
-            s0 += crt_bound_term(v[c], ec);
-            s1 += crt_bound_term(v[c + 1], ec);
-        }
-        part[blockIdx.y * rows + r] = crt_chunk_word(mx, (uint64_t)(s0 + s1));
-    } else if (pair_rows(x, rs)) {
+// Lines contiguous in k: 16 lanes per line with 16-byte pair loads when every
+// line starts 16-byte aligned, else a wave per line.
+__device__ __forceinline__ void scan_contiguous(const double* __restrict__ x, int64_t rows, int64_t K, int64_t rs,
+                                                int64_t k0, uint64_t* __restrict__ part) {
+    if (pair_rows(x, rs)) {
         // 16 lanes per line, two 16-byte pairs each (k0 + 2 l16 and 32 further)
         const int l16 = threadIdx.x & 15;
         const int64_t r = blockIdx.x * (int64_t)(blockDim.x >> 4) + (threadIdx.x >> 4);
@@ -933,35 +919,142 @@ __global__ __launch_bounds__(256) void crt_scan_kernel(const double* __restrict_
     }
 }
 
+// Lines strided in memory with 16-byte aligned line pairs (rs == 1, even ks,
+// 16-byte aligned x): a 128-line x 64-k tile through LDS, loaded as line
+// pairs (one 16-byte load covers lines 2j, 2j + 1 at one k, so a wave's load
+// reads 1 KB of a column), then two lanes per line, 32 k each.
+__global__ __launch_bounds__(256) void crt_scan_pairs_kernel(const double* __restrict__ x, int64_t rows, int64_t K,
+                                                             int64_t ks, uint64_t* __restrict__ part) {
+    __shared__ double tile[128][CRT_KC + 1];
+    const int64_t k0 = (int64_t)blockIdx.y * CRT_KC;
+    const int t = threadIdx.x;
+    const int64_t r0 = (int64_t)blockIdx.x * 128;
+    {
+        const int j = t & 63;
+        const int64_t r = r0 + 2 * j;
+#pragma unroll
+        for (int i = 0; i < CRT_KC / 4; ++i) {
+            const int kk = (t >> 6) + 4 * i;
+            double a = 0.0, b = 0.0;
+            if (k0 + kk < K) {
+                if (r + 1 < rows) {
+                    const pdv2 w = *reinterpret_cast<const pdv2*>(x + r + (k0 + kk) * ks);
+                    a = w.x;
+                    b = w.y;
+                } else if (r < rows) {
+                    a = x[r + (k0 + kk) * ks];
+                }
+            }
+            tile[2 * j][kk] = a;
+            tile[2 * j + 1][kk] = b;
+        }
+    }
+    __syncthreads();
+    const int rr = t >> 1, h = t & 1;
+    const int64_t r = r0 + rr;
+    double v[CRT_KC / 2];
+    double mx = 0.0;
+#pragma unroll
+    for (int i = 0; i < CRT_KC / 2; ++i) {
+        v[i] = tile[rr][32 * h + i];
+        mx = fmax(mx, fabs(v[i]));
+    }
+    mx = fmax(mx, __shfl_xor(mx, 1));
+    const int ec = crt_chunk_exp(mx);
+    double sum = 0.0;
+#pragma unroll
+    for (int i = 0; i < CRT_KC / 2; ++i) sum += crt_bound_term(v[i], ec);
+    sum += __shfl_xor(sum, 1);
+    if (h == 0 && r < rows) part[blockIdx.y * rows + r] = crt_chunk_word(mx, (uint64_t)sum);
+}
+
+// STRIDED: lines strided in memory (ks != 1); the static LDS tile of that
+// path would otherwise limit the occupancy of the k-contiguous paths too.
+template <bool STRIDED>
+__global__ __launch_bounds__(256) void crt_scan_kernel(const double* __restrict__ x, int64_t rows, int64_t K,
+                                                       int64_t rs, int64_t ks, uint64_t* __restrict__ part) {
+    const int64_t k0 = (int64_t)blockIdx.y * CRT_KC;
+    if constexpr (STRIDED) {
+        // 64 lines x 64 k through LDS: loads along the lines (consecutive
+        // lanes on consecutive lines, 16 per thread in flight), then 4 lanes
+        // per line, 16 k each, reduce the chunk with two shuffles.
+        __shared__ double tile[64][CRT_KC + 1];
+        const int t = threadIdx.x;
+        const int64_t r0 = (int64_t)blockIdx.x * 64;
+        {
+            const int64_t r = r0 + (t & 63);
+            const bool rv = r < rows;
+#pragma unroll
+            for (int i = 0; i < CRT_KC / 4; ++i) {
+                const int kk = (t >> 6) + 4 * i;
+                tile[t & 63][kk] = (rv && k0 + kk < K) ? x[r * rs + (k0 + kk) * ks] : 0.0;
+            }
+        }
+        __syncthreads();
+        const int rr = t >> 2, q = t & 3;
+        const int64_t r = r0 + rr;
+        double v[CRT_KC / 4];
+        double mx = 0.0;
+#pragma unroll
+        for (int i = 0; i < CRT_KC / 4; ++i) {
+            v[i] = tile[rr][16 * q + i];
+            mx = fmax(mx, fabs(v[i]));
+        }
+        mx = fmax(mx, __shfl_xor(mx, 1));
+        mx = fmax(mx, __shfl_xor(mx, 2));
+        const int ec = crt_chunk_exp(mx);
+        double sum = 0.0;
+#pragma unroll
+        for (int i = 0; i < CRT_KC / 4; ++i) sum += crt_bound_term(v[i], ec);
+        sum += __shfl_xor(sum, 1);
+        sum += __shfl_xor(sum, 2);
+        if (q == 0 && r < rows) part[blockIdx.y * rows + r] = crt_chunk_word(mx, (uint64_t)sum);
+    } else {
+        scan_contiguous(x, rows, K, rs, k0, part);
+    }
+}
+
 // Line shifts: e = the largest chunk exponent, S = sum over the nonzero
 // chunks of ceil(S_c / 4^(e - e_c)), sigma the largest shift with
 // 2^(2(sigma + e)) S <= 2^(2Q + 32) (sigma = Q + 16 - e - ceil(log4 S)),
 // capped at 53 - e; an all-zero line keeps sigma = 0.  Stored as
 // e' = Q - sigma, the form the residue and reconstruction kernels take.
-__global__ void crt_shift_kernel(const uint64_t* __restrict__ part, int64_t rows, int64_t rows_p, int nchunk, int Q,
-                                 int32_t* __restrict__ e) {
-    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < rows_p; r += (int64_t)gridDim.x * blockDim.x) {
-        int sig = 0;
-        if (r < rows) {
-            uint64_t top = 0;
-            for (int c = 0; c < nchunk; ++c) top = max(top, part[(int64_t)c * rows + r] >> 48);
-            if (top != 0) {
-                uint64_t S = 0;
-                for (int c = 0; c < nchunk; ++c) {
-                    const uint64_t w = part[(int64_t)c * rows + r];
-                    if (w == 0) continue;
-                    const uint64_t sc = w & ((1ull << 48) - 1);
-                    const int d2 = 2 * (int)(top - (w >> 48));
-                    S += d2 >= 62 ? 1 : (sc + (1ull << d2) - 1) >> d2;
-                }
-                const int bl = S > 1 ? 64 - __builtin_clzll(S - 1) : 0;  // bit length of S - 1
-                const int j = (bl + 1) >> 1;                              // ceil(log4 S)
-                const int er = (int)top - 2048;
-                sig = min(53 - er, Q + 16 - er - j);
-            }
+// A workgroup takes 64 lines; its four waves split the chunks (lanes on
+// consecutive lines, so every load is coalesced) and combine through LDS.
+__global__ __launch_bounds__(256) void crt_shift_kernel(const uint64_t* __restrict__ part, int64_t rows, int64_t rows_p,
+                                                        int nchunk, int Q, int32_t* __restrict__ e) {
+    __shared__ uint64_t red[4][64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int64_t r = (int64_t)blockIdx.x * 64 + lane;
+    const bool valid = r < rows;
+    uint64_t top = 0;
+    if (valid)
+        for (int c = w; c < nchunk; c += 4) top = max(top, part[(int64_t)c * rows + r] >> 48);
+    red[w][lane] = top;
+    __syncthreads();
+    top = max(max(red[0][lane], red[1][lane]), max(red[2][lane], red[3][lane]));
+    __syncthreads();
+    uint64_t S = 0;
+    if (valid && top != 0)
+        for (int c = w; c < nchunk; c += 4) {
+            const uint64_t wd = part[(int64_t)c * rows + r];
+            if (wd == 0) continue;
+            const uint64_t sc = wd & ((1ull << 48) - 1);
+            const int d2 = 2 * (int)(top - (wd >> 48));
+            S += d2 >= 62 ? 1 : (sc + (1ull << d2) - 1) >> d2;
         }
-        e[r] = Q - sig;
+    red[w][lane] = S;
+    __syncthreads();
+    if (w != 0 || r >= rows_p) return;
+    S = (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
+    int sig = 0;
+    if (valid && top != 0) {
+        const int bl = S > 1 ? 64 - __builtin_clzll(S - 1) : 0;  // bit length of S - 1
+        const int j = (bl + 1) >> 1;                              // ceil(log4 S)
+        const int er = (int)top - 2048;
+        sig = min(53 - er, Q + 16 - er - j);
     }
+    e[r] = Q - sig;
 }
 
 // Writes the s digits of op(X)(row, k) into slices[t][row][Kp] (zeros in the
@@ -1266,17 +1359,34 @@ int dg_ozaki_dgemm(char opA, char opB, int64_t m, int64_t n, int64_t k, double a
         // Line exponents and Cauchy-Schwarz shifts: one scan per operand
         // (chunk exponents and chunk bounds), then the per-line combination.
         auto scan_grid = [](int64_t rows, int64_t ks, const double* x, int64_t rs) {
-            const int64_t per = ks == 1 ? (pair_rows(x, rs) ? NT / 16 : NT / 64) : NT;
+            const int64_t per = ks == 1 ? (pair_rows(x, rs) ? NT / 16 : NT / 64) : 64;
             return (unsigned)((rows + per - 1) / per);
         };
         uint64_t* bp = reinterpret_cast<uint64_t*>(part);
-        hipLaunchKernelGGL(crt_scan_kernel, dim3(scan_grid(m, a_ks, A, a_rs), gy), dim3(NT), 0, st, A, m, k, a_rs,
-                           a_ks, bp);
-        hipLaunchKernelGGL(crt_shift_kernel, dim3((unsigned)((p.Mp + NT - 1) / NT)), dim3(NT), 0, st, bp, m, p.Mp,
+        auto pairs = [](const double* x, int64_t rs, int64_t ks) {
+            return rs == 1 && !(ks & 1) && !(reinterpret_cast<uintptr_t>(x) & 15);
+        };
+        if (a_ks != 1 && pairs(A, a_rs, a_ks))
+            hipLaunchKernelGGL(crt_scan_pairs_kernel, dim3((unsigned)((m + 127) / 128), gy), dim3(NT), 0, st, A, m, k,
+                               a_ks, bp);
+        else if (a_ks != 1)
+            hipLaunchKernelGGL(crt_scan_kernel<true>, dim3(scan_grid(m, a_ks, A, a_rs), gy), dim3(NT), 0, st, A, m, k,
+                               a_rs, a_ks, bp);
+        else
+            hipLaunchKernelGGL(crt_scan_kernel<false>, dim3(scan_grid(m, a_ks, A, a_rs), gy), dim3(NT), 0, st, A, m, k,
+                               a_rs, a_ks, bp);
+        hipLaunchKernelGGL(crt_shift_kernel, dim3((unsigned)((p.Mp + 63) / 64)), dim3(NT), 0, st, bp, m, p.Mp,
                            nchunk, q, ea);
-        hipLaunchKernelGGL(crt_scan_kernel, dim3(scan_grid(n, b_ks, B, b_rs), gy), dim3(NT), 0, st, B, n, k, b_rs,
-                           b_ks, bp);
-        hipLaunchKernelGGL(crt_shift_kernel, dim3((unsigned)((p.Np + NT - 1) / NT)), dim3(NT), 0, st, bp, n, p.Np,
+        if (b_ks != 1 && pairs(B, b_rs, b_ks))
+            hipLaunchKernelGGL(crt_scan_pairs_kernel, dim3((unsigned)((n + 127) / 128), gy), dim3(NT), 0, st, B, n, k,
+                               b_ks, bp);
+        else if (b_ks != 1)
+            hipLaunchKernelGGL(crt_scan_kernel<true>, dim3(scan_grid(n, b_ks, B, b_rs), gy), dim3(NT), 0, st, B, n, k,
+                               b_rs, b_ks, bp);
+        else
+            hipLaunchKernelGGL(crt_scan_kernel<false>, dim3(scan_grid(n, b_ks, B, b_rs), gy), dim3(NT), 0, st, B, n, k,
+                               b_rs, b_ks, bp);
+        hipLaunchKernelGGL(crt_shift_kernel, dim3((unsigned)((p.Np + 63) / 64)), dim3(NT), 0, st, bp, n, p.Np,
                            nchunk, q, eb);
         DG_LAUNCH_CHECK();
     } else {
@@ -1314,8 +1424,15 @@ int dg_ozaki_dgemm(char opA, char opB, int64_t m, int64_t n, int64_t k, double a
 #undef DG_CRT_CASE
             default: break;
         }
-        hipLaunchKernelGGL(rsk, ga, dim3(256), 0, st, A, m, k, a_rs, a_ks, ea, q, sa, p.Kp, p.a_slice);
-        hipLaunchKernelGGL(rsk, gb, dim3(256), 0, st, B, n, k, b_rs, b_ks, eb, q, sb, p.Kp, p.b_slice);
+        ResidueConsts rcs{};
+        for (int t = 0; t < kMaxModuli; ++t) {
+            const int pm = crt_modulus(t);
+            rcs.c24[t] = (double)((1 << 24) % pm);
+            rcs.inv_p[t] = 1.0 / (double)pm;
+            rcs.pd[t] = (double)pm;
+        }
+        hipLaunchKernelGGL(rsk, ga, dim3(256), 0, st, A, m, k, a_rs, a_ks, ea, q, sa, p.Kp, p.a_slice, rcs);
+        hipLaunchKernelGGL(rsk, gb, dim3(256), 0, st, B, n, k, b_rs, b_ks, eb, q, sb, p.Kp, p.b_slice, rcs);
         DG_LAUNCH_CHECK();
         CrtArgs cg{};
         cg.a = sa; cg.b = sb; cg.res = reinterpret_cast<uint8_t*>(ws + p.off_w);

@@ -290,6 +290,7 @@ def grouped_dgemm_t(descs):
     check("dg_grouped_dgemm", lib().dg_grouped_dgemm(arr, len(descs), _stream()))
 
 
+
 class GroupedPlan:
     """dg_grouped_plan over a list of GemmDesc; destroyed with close()."""
 

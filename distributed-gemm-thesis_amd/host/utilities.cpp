@@ -3,10 +3,12 @@
 // TransposeEmblems,Validation,Logging}.cpp).
 #include <unistd.h>
 
+#include <chrono>
 #include <cmath>
 #include <cstdarg>
 #include <cstring>
 #include <ctime>
+#include <fstream>
 #include <iostream>
 #include <limits.h>
 #include <mutex>
@@ -88,13 +90,10 @@ void getGPUMemoryInfo(long long* freeMemory, long long* maxMemory, const int dev
     HIP_CHECK(hipSetDevice(prev));
 }
 
+// Seconds on the monotonic clock (callers take differences of two readings).
 double csecond(void) {
-    struct timespec tms;
-    if (clock_gettime(CLOCK_REALTIME, &tms)) return 0.0;
-    int64_t micros = (int64_t)tms.tv_sec * 1000000;
-    micros += tms.tv_nsec / 1000;
-    if (tms.tv_nsec % 1000 >= 500) ++micros;
-    return (double)micros / 1000000.0;
+    const auto t = std::chrono::steady_clock::now().time_since_epoch();
+    return std::chrono::duration<double>(t).count();
 }
 
 // ------------------------------------------------------------------ validation
@@ -202,25 +201,27 @@ const char* const kCannonCsvHeader =
     "Algo,M,N,K,TotalNodes,TotalGPUs,dRow,dCol,StackSize,DecompositionTime,ExecutionTime,GatherTime,GFlops,"
     "DataLocation,CommunicationTime,GemmCallTime";
 
+// Opens `filename` for appending and writes `header` unless the file's first
+// line already is it.  The two console messages are the reference's
+// (src/Utilities/Logging.cpp), kept as drop-in output.
 FILE* createLogCsv(std::string filename, std::string header) {
-    FILE* logfile = fopen(filename.c_str(), "a+");
-    if (logfile == NULL) dgHostError(EXIT_FAILURE, "cannot open log file %s", filename.c_str());
-    char* headerLine = NULL;
-    size_t len = 0;
-    rewind(logfile);
-    if (getline(&headerLine, &len, logfile) != -1) {
-        if (std::string(headerLine) == header + "\n") {
-            std::cout << "Header already placed, skip it" << std::endl;
-        } else {
-            fputs(header.c_str(), logfile);
-            fputs("\n", logfile);
+    bool header_present = false, file_has_line = false;
+    {
+        std::ifstream in(filename);
+        std::string first;
+        if (in && std::getline(in, first)) {
+            file_has_line = true;
+            header_present = first == header && !in.eof();  // the line ended with a newline
         }
-    } else {
-        std::cout << "Header not placed" << std::endl;
-        fputs(header.c_str(), logfile);
-        fputs("\n", logfile);
     }
-    free(headerLine);
+    FILE* logfile = fopen(filename.c_str(), "a");
+    if (logfile == NULL) dgHostError(EXIT_FAILURE, "cannot open log file %s", filename.c_str());
+    if (header_present) {
+        std::cout << "Header already placed, skip it" << std::endl;
+        return logfile;
+    }
+    if (!file_has_line) std::cout << "Header not placed" << std::endl;
+    fprintf(logfile, "%s\n", header.c_str());
     return logfile;
 }
 
