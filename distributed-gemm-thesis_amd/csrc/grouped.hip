@@ -392,11 +392,134 @@ int autotune(dg_grouped_plan* p, hipStream_t st) {
     return DG_OK;
 }
 
+// ------------------------------------------------------------ no allocation
+// The allocation-free form of the one-shot grouped GEMM keeps its descriptor
+// tables in this file-scope device array instead of the table cache.  Every
+// GPU holds its own copy until the process ends; its address on a device
+// comes from hipGetSymbolAddress, taken once per device with that device
+// current, and is the only handle host code uses for it.  Calls on one GPU
+// share the array: calls that overlap on one GPU race.
+constexpr size_t kScratchBytes = 256 * 1024;
+__device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+int scratch_for_device(int dev, void** out) {
+    if (dev < 0 || dev >= kMaxDevices) return fail(DG_ERR_NO_DEVICE, "scratch: device %d", dev);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* ptr = nullptr;
+        hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return fail(DG_ERR_HIP, "scratch: hipGetSymbolAddress: %s", hipGetErrorString(e));
+        g_scratch_ptr[dev] = ptr;
+    }
+    *out = g_scratch_ptr[dev];
+    return DG_OK;
+}
+
+// Tables reach the scratch array through kernel arguments (copied at launch,
+// and kept by value in a captured graph), in chunks of kStoreWords 16-byte
+// words: no host buffer has to outlive the call.
+constexpr int kStoreWords = 224;
+struct StoreChunk {
+    uint4 w[kStoreWords];
+    int32_t n;
+};
+
+__global__ void store_chunk_kernel(uint4* __restrict__ dst, StoreChunk c) {
+    const int t = threadIdx.x;
+    if (t < c.n) dst[t] = c.w[t];
+}
+
+int store_bytes(void* dst, const std::vector<unsigned char>& bytes, hipStream_t st) {
+    const size_t words = (bytes.size() + 15) / 16;
+    for (size_t w0 = 0; w0 < words; w0 += kStoreWords) {
+        StoreChunk c{};
+        c.n = (int32_t)std::min<size_t>(kStoreWords, words - w0);
+        const size_t b0 = w0 * 16, nb = std::min<size_t>((size_t)c.n * 16, bytes.size() - b0);
+        memcpy(c.w, bytes.data() + b0, nb);
+        hipLaunchKernelGGL(store_chunk_kernel, dim3(1), dim3(256), 0, st, static_cast<uint4*>(dst) + w0, c);
+        DG_LAUNCH_CHECK();
+    }
+    return DG_OK;
+}
+
 }  // namespace
 
 }  // namespace dg
 
 extern "C" {
+
+int dg_grouped_dgemm_noalloc(const dg_gemm_desc* descs, int count, void* stream) {
+    using namespace dg;
+    int rc = validate_descs(descs, count);
+    if (rc) return rc;
+    if (count == 0) return DG_OK;
+    rc = require_device();
+    if (rc) return rc;
+    int dev = 0;
+    DG_HIP(hipGetDevice(&dev));
+    void* base = nullptr;
+    rc = scratch_for_device(dev, &base);
+    if (rc) return rc;
+    hipStream_t st = as_stream(stream);
+    // Lay out every op group's tables (problems routed to each kernel, then
+    // segments) in one host image, 256-byte aligned pieces, and check the size
+    // before anything is launched.
+    struct Piece { char opA, opB; GemmBatch hb; std::vector<Seg> segs; size_t off_dma, off_gen, off_seg; };
+    std::vector<Piece> pieces;
+    size_t total = 0;
+    auto take = [&](size_t bytes) { const size_t o = total; total = (total + bytes + 255) / 256 * 256; return o; };
+    for (auto& kv : by_ops(descs, count)) {
+        std::vector<Prob> probs;
+        std::vector<Seg> segs;
+        build_group(descs, kv.second, probs, segs);
+        if (probs.empty()) continue;
+        Piece pc{kv.first.first, kv.first.second, GemmBatch{}, std::move(segs), 0, 0, 0};
+        rc = prepare_gemm(pc.opA, pc.opB, probs, pc.segs, pc.hb);
+        if (rc) return rc;
+        pc.off_dma = take(pc.hb.dma.size() * sizeof(Prob));
+        pc.off_gen = take(pc.hb.gen.size() * sizeof(Prob));
+        pc.off_seg = take(pc.segs.size() * sizeof(Seg));
+        pieces.push_back(std::move(pc));
+    }
+    if (total > kScratchBytes)
+        return fail(DG_ERR_WORKSPACE,
+                    "dg_grouped_dgemm_noalloc: the batch's descriptor tables need %zu bytes, more than the %zu-byte "
+                    "scratch array (about %zu problems of one segment each)",
+                    total, kScratchBytes, kScratchBytes / (2 * sizeof(Prob) + sizeof(Seg)));
+    rc = scale_only(descs, count, st);
+    if (rc) return rc;
+    if (pieces.empty()) return DG_OK;
+    std::vector<unsigned char> image(total, 0);
+    for (const Piece& pc : pieces) {
+        if (!pc.hb.dma.empty()) memcpy(image.data() + pc.off_dma, pc.hb.dma.data(), pc.hb.dma.size() * sizeof(Prob));
+        if (!pc.hb.gen.empty()) memcpy(image.data() + pc.off_gen, pc.hb.gen.data(), pc.hb.gen.size() * sizeof(Prob));
+        memcpy(image.data() + pc.off_seg, pc.segs.data(), pc.segs.size() * sizeof(Seg));
+    }
+    rc = store_bytes(base, image, st);
+    if (rc) return rc;
+    unsigned char* b = static_cast<unsigned char*>(base);
+    for (const Piece& pc : pieces) {
+        GemmBatchDev d;
+        d.dma = pc.hb.dma.empty() ? nullptr : reinterpret_cast<const Prob*>(b + pc.off_dma);
+        d.ndma = (int)pc.hb.dma.size();
+        d.tiles_dma = pc.hb.tiles_dma;
+        d.gen = pc.hb.gen.empty() ? nullptr : reinterpret_cast<const Prob*>(b + pc.off_gen);
+        d.ngen = (int)pc.hb.gen.size();
+        d.tiles_gen[0] = pc.hb.tiles_gen[0];
+        d.tiles_gen[1] = pc.hb.tiles_gen[1];
+        d.vec = pc.hb.vec;
+        d.dma_edge = pc.hb.dma_edge;
+        d.segs = reinterpret_cast<const Seg*>(b + pc.off_seg);
+        rc = launch_gemm_f64_dev(pc.opA, pc.opB, DG_GEMM_MODE_ROUND_ROBIN, default_tile_cfg(), d, nullptr, st);
+        if (rc) return rc;
+    }
+    return DG_OK;
+}
 
 int dg_grouped_dgemm(const dg_gemm_desc* descs, int count, void* stream) {
     using namespace dg;

@@ -87,6 +87,7 @@ SIGNATURES = {
     "dg_generate": (_int, [_u64, _int, _ptr, _i64, _ptr]),
     "dg_generate_f32": (_int, [_u64, _int, _ptr, _i64, _ptr]),
     "dg_grouped_dgemm": (_int, [_descp, _int, _ptr]),
+    "dg_grouped_dgemm_noalloc": (_int, [_descp, _int, _ptr]),
     "dg_grouped_plan_create": (_int, [_descp, _int, _int, ctypes.POINTER(ctypes.c_void_p)]),
     "dg_grouped_plan_create_form": (_int, [_descp, _int, _int, _int, _int, ctypes.POINTER(ctypes.c_void_p)]),
     "dg_grouped_plan_problem_form": (_int, [ctypes.c_void_p, _int, ctypes.POINTER(_int)]),
@@ -289,6 +290,10 @@ def grouped_dgemm_t(descs):
     arr = _desc_array(descs)
     check("dg_grouped_dgemm", lib().dg_grouped_dgemm(arr, len(descs), _stream()))
 
+
+def grouped_dgemm_noalloc_t(descs):
+    arr = _desc_array(descs)
+    check("dg_grouped_dgemm_noalloc", lib().dg_grouped_dgemm_noalloc(arr, len(descs), _stream()))
 
 
 class GroupedPlan:

@@ -232,6 +232,17 @@ typedef struct dg_gemm_desc {
  * calls of src/25DCannon.cpp:324-357. */
 int dg_grouped_dgemm(const dg_gemm_desc* descs, int count, void* stream);
 
+/* dg_grouped_dgemm that allocates nothing: the batch's descriptor tables go
+ * into the library's static device scratch array (256 KiB per GPU, held
+ * until the process ends) instead of the table cache, written by kernels
+ * whose arguments carry them, so the call also records cleanly into a HIP
+ * graph.  A batch whose tables exceed the array (about 1900 single-segment
+ * problems) fails with DG_ERR_WORKSPACE before any launch.  Calls that
+ * overlap on one GPU share the array and race.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (graph capture,
+ * tight serving loops). */
+int dg_grouped_dgemm_noalloc(const dg_gemm_desc* descs, int count, void* stream);
+
 /* A grouped batch prepared once and executed many times: descriptor tables
  * are uploaded at creation, the tile shape / scheduling variant is chosen by
  * the autotuner (timed on first use, cached per shape class), and
