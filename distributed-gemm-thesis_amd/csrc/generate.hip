@@ -40,9 +40,9 @@ __device__ __forceinline__ double generated_value(uint64_t seed, int dist, int64
     return value_of(seed + step, dist, (seed ^ kExpSeedXor) + step);
 }
 
-// One element per thread and pass: generation is bound by the 64-bit
-// multiplies of the output function (about 10 32-bit multiplies per
-// element), not by the 8-byte stores.
+// One element per thread and pass (two per thread measured slower); the
+// output function costs about 10 32-bit multiplies per element, which the
+// 8-byte stores still hide.
 __global__ void generate_kernel(uint64_t seed, int dist, double* __restrict__ out, int64_t n) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         out[i] = generated_value(seed, dist, i);
