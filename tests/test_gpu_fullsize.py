@@ -146,3 +146,50 @@ def test_block2d_full_round_trip_20000(dev):
     base = ro[6] * n + (n - ro[6]) * co[4]
     assert Pk[base].item() == X[ro[6] + co[4] * n].item()
     assert Pk[base + 2857 + 3999 * 2858].item() == X[n - 1 + (n - 1) * n].item()
+
+
+def test_config5_grouped_ozaki_form_full(dev):
+    """The configuration-5 batch (7x5 uneven tiles of M=N=K=20000 plus 256
+    log-uniform GEMMs) as one plan in the Ozaki-INT8 form: the autotuner runs
+    the large problems as INT8 CRT emulation with the FP64-level moduli count,
+    and every output agrees with the FP64 plan's to the FP64 level."""
+    n = 20000
+    drow, dcol = dgamd.virtual_grid(35)
+    ro, co = dgamd.block_splits(n, drow), dgamd.block_splits(n, dcol)
+    A, B = gen(n * n, 21, dev), gen(n * n, 22, dev)
+    C = torch.full((n * n,), float("nan"), dtype=torch.float64, device=dev)
+    descs, outs = [], [C]
+    for i in range(drow):
+        for j in range(dcol):
+            m_, n_ = ro[i + 1] - ro[i], co[j + 1] - co[j]
+            descs.append(dgamd.gemm_desc("N", "N", m_, n_, n, 1.0, A[ro[i]:], n, B[co[j] * n:], n, 0.0,
+                                         C[ro[i] + co[j] * n:], n))
+    rng = np.random.default_rng(5)
+    keep = []
+    for t in range(256):
+        m_, n_, k_ = (int(round(np.exp(rng.uniform(np.log(64), np.log(4096))))) for _ in range(3))
+        Ae, Be = gen(m_ * k_, 100 + 3 * t, dev), gen(k_ * n_, 101 + 3 * t, dev)
+        Ce = torch.full((m_ * n_,), float("nan"), dtype=torch.float64, device=dev)
+        keep += [Ae, Be]
+        outs.append(Ce)
+        descs.append(dgamd.gemm_desc("N", "N", m_, n_, k_, 1.0, Ae, m_, Be, k_, 0.0, Ce, m_))
+    plan = dgamd.GroupedPlan(descs, dgamd.DG_PLAN_AUTOTUNE)
+    plan.graph_launch()
+    torch.cuda.synchronize()
+    ref = [o.clone() for o in outs]
+    plan.close()
+    for o in outs:
+        o.fill_(float("nan"))
+    oz = dgamd.GroupedPlan(descs, dgamd.DG_PLAN_AUTOTUNE, form=dgamd.DG_FORM_OZAKI_INT8)
+    oz.graph_launch()
+    torch.cuda.synchronize()
+    forms = [oz.problem_form(i) for i in range(len(descs))]
+    n_oz = sum(1 for f, _ in forms if f == dgamd.DG_FORM_OZAKI_INT8)
+    assert n_oz >= 1, forms[:40]
+    # the FP64-level count for each problem's k: 16 moduli at k = 20000, 15 at the extras' smaller k
+    assert all(m in (15, 16) for f, m in forms if f == dgamd.DG_FORM_OZAKI_INT8)
+    worst = max((torch.linalg.vector_norm(o - r) / torch.linalg.vector_norm(r)).item() for o, r in zip(outs, ref))
+    print("Ozaki-form problems: %d of %d; worst relative error vs the FP64 plan %.3e" % (n_oz, len(descs), worst))
+    assert worst <= 1e-13, worst
+    oz.close()
+    del keep
