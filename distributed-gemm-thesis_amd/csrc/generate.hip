@@ -40,18 +40,35 @@ __device__ __forceinline__ double generated_value(uint64_t seed, int dist, int64
     return value_of(seed + step, dist, (seed ^ kExpSeedXor) + step);
 }
 
-// One element per thread and pass (two per thread measured slower); the
-// output function costs about 10 32-bit multiplies per element, which the
-// 8-byte stores still hide.
-__global__ void generate_kernel(uint64_t seed, int dist, double* __restrict__ out, int64_t n) {
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        out[i] = generated_value(seed, dist, i);
-}
-
-// FP32 form: the same FP64 values rounded to nearest (ties to even).
-__global__ void generate_f32_kernel(uint64_t seed, int dist, float* __restrict__ out, int64_t n) {
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        out[i] = (float)generated_value(seed, dist, i);
+// E = 16/sizeof(T) consecutive elements per thread, written as one 16-byte
+// vector, the grid covering the whole matrix (one store per thread keeps
+// the resident workgroups on a compact window of the output).  The first
+// `peel` elements (up to a 16-byte boundary) and the tail elements go
+// singly.  Without 16-byte vectors (VEC false) one element per thread.
+template <typename T, bool VEC>
+__global__ __launch_bounds__(256) void generate_kernel(uint64_t seed, int dist, T* __restrict__ out, int64_t n,
+                                                       int peel) {
+    typedef __attribute__((address_space(1))) T gT;
+    typedef T vT __attribute__((ext_vector_type(16 / sizeof(T))));
+    typedef __attribute__((address_space(1))) vT gV;
+    constexpr int E = VEC ? 16 / sizeof(T) : 1;
+    const int64_t gid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int64_t nv = (n - peel) / E;
+    for (int64_t v = gid; v < nv; v += stride) {
+        const int64_t i0 = peel + v * E;
+        if constexpr (VEC) {
+            vT w;
+#pragma unroll
+            for (int e = 0; e < E; ++e) w[e] = (T)generated_value(seed, dist, i0 + e);
+            *(gV*)(out + i0) = w;
+        } else {
+            ((gT*)out)[i0] = (T)generated_value(seed, dist, i0);
+        }
+    }
+    if (gid < peel) ((gT*)out)[gid] = (T)generated_value(seed, dist, gid);
+    const int64_t t0 = peel + nv * E;
+    if (gid < n - t0) ((gT*)out)[t0 + gid] = (T)generated_value(seed, dist, t0 + gid);
 }
 
 }  // namespace
@@ -68,13 +85,21 @@ static int dg_generate_common(uint64_t seed, int dist, T* ptr, int64_t n, void* 
     if (!ptr) return fail(DG_ERR_INVALID, "%s: null pointer", name);
     int rc = require_device();
     if (rc != DG_OK) return rc;
-    int64_t blocks = (n + 255) / 256;
-    if (blocks > 8192) blocks = 8192;
-    if constexpr (sizeof(T) == 8) {
-        hipLaunchKernelGGL(generate_kernel, dim3((unsigned)blocks), dim3(256), 0, as_stream(stream), seed, dist, ptr, n);
-    } else {
-        hipLaunchKernelGGL(generate_f32_kernel, dim3((unsigned)blocks), dim3(256), 0, as_stream(stream), seed, dist, ptr, n);
-    }
+    // The FP32 form rounds the FP64 value to nearest (ties to even).
+    const uintptr_t addr = reinterpret_cast<uintptr_t>(ptr);
+    const bool vec = (addr % sizeof(T)) == 0;
+    int peel = vec ? (int)(((16 - (addr & 15)) & 15) / sizeof(T)) : 0;
+    if (peel > n) peel = (int)n;
+    const int64_t per = vec ? 16 / sizeof(T) : 1;
+    int64_t blocks = ((n - peel) / per + 255) / 256;
+    if (blocks > (1 << 30)) blocks = 1 << 30;  // beyond that the kernel loops
+    if (blocks < 1) blocks = 1;
+    if (vec)
+        hipLaunchKernelGGL((generate_kernel<T, true>), dim3((unsigned)blocks), dim3(256), 0, as_stream(stream), seed,
+                           dist, ptr, n, peel);
+    else
+        hipLaunchKernelGGL((generate_kernel<T, false>), dim3((unsigned)blocks), dim3(256), 0, as_stream(stream), seed,
+                           dist, ptr, n, 0);
     DG_LAUNCH_CHECK();
     return DG_OK;
 }

@@ -24,6 +24,23 @@ def test_generate_bit_exact(dev, dist):
     assert np.array_equal(got.view(np.uint64), ref.view(np.uint64))
 
 
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+@pytest.mark.parametrize("n,off", [(1, 0), (2, 1), (3, 3), (5, 1), (100003, 1), (100003, 2), (100003, 3), (4097, 0)])
+def test_generate_offsets_and_tails(dev, dtype, n, off):
+    """Generation into buffers that start 4, 8 or 12 bytes past a 16-byte
+    boundary and whose lengths leave a tail: the single-element head and
+    tail and the 16-byte body agree with the oracle bit for bit, and the
+    elements around the buffer stay untouched."""
+    dt = torch.float64 if dtype == "f64" else torch.float32
+    buf = torch.full((n + off + 4,), 7.0, dtype=dt, device=dev)
+    dgamd.generate_t(4321, dgamd.DG_DIST_WIDE, buf[off:off + n])
+    got = buf.cpu().numpy()
+    ref = oracle.generate(4321, dgamd.DG_DIST_WIDE, n) if dtype == "f64" else oracle.generate_f32(4321, dgamd.DG_DIST_WIDE, n)
+    u = np.uint64 if dtype == "f64" else np.uint32
+    assert np.array_equal(got[off:off + n].view(u), ref.view(u))
+    assert np.all(got[:off] == 7.0) and np.all(got[off + n:] == 7.0)
+
+
 @pytest.mark.parametrize("rows,cols,ldg,pr,pc", [(2048, 2048, 2048, 2, 2), (23, 17, 29, 7, 5), (20000, 40, 20003, 7, 5),
                                                  (5, 3, 5, 1, 1), (9, 9, 9, 3, 3)])
 def test_block2d_pack_unpack(dev, rows, cols, ldg, pr, pc):
@@ -78,7 +95,8 @@ def test_block_sequential_tiles(dev, col_major, M, N, K, size):
 
 
 @pytest.mark.parametrize("rows,cols,mb,nb,P,Q", [(23, 17, 3, 2, 7, 5), (64, 64, 8, 8, 4, 4), (1000, 777, 64, 100, 3, 2),
-                                                 (5, 5, 8, 8, 2, 2), (512, 520, 512, 512, 4, 4)])
+                                                 (5, 5, 8, 8, 2, 2), (512, 520, 512, 512, 4, 4),
+                                                 (2100, 5, 1500, 2, 2, 2), (2, 1100000, 1, 3, 2, 3)])
 def test_block_cyclic_pack_unpack(dev, rows, cols, mb, nb, P, Q):
     rng = np.random.default_rng(rows + 7 * cols)
     ldg = rows + 3
@@ -125,6 +143,45 @@ def test_copy_block(dev):
     S, D = dev_copy(src, dev), dev_copy(dst0, dev)
     dgamd.copy_block_t(60, 50, S, 101, D, 77)
     assert np.array_equal(D.cpu().numpy(), ref)
+
+
+@pytest.mark.parametrize("rows,cols,lds,ldd,soff,doff", [(1, 1100000, 1, 2, 0, 0), (3, 400000, 5, 3, 1, 0),
+                                                         (1030, 70, 1031, 1030, 1, 1), (515, 9, 515, 517, 0, 1),
+                                                         (2048, 5, 2049, 2048, 1, 1), (17, 3, 17, 17, 1, 0)])
+def test_copy_block_offsets_and_wide(dev, rows, cols, lds, ldd, soff, doff):
+    """Strided FP64 copies whose buffers start 8 bytes past a 16-byte
+    boundary (same or different offsets on the two sides) and whose column
+    count exceeds one launch's column groups, against the oracle."""
+    rng = np.random.default_rng(rows * 7 + cols)
+    src = rng.standard_normal(lds * cols + soff)
+    dst0 = np.full(ldd * cols + doff, 9.0)
+    ref = dst0.copy()
+    oracle.lib().or_copy_block(rows, cols, oracle._p(src[soff:]), lds, oracle._p(ref[doff:]), ldd)
+    S, D = dev_copy(src, dev), dev_copy(dst0, dev)
+    dgamd.copy_block_t(rows, cols, S[soff:], lds, D[doff:], ldd)
+    assert np.array_equal(D.cpu().numpy(), ref)
+
+
+@pytest.mark.parametrize("rows,cols,ldg,pr,pc,off", [(2857, 6, 2858, 7, 2, 1), (1030, 4, 1031, 3, 2, 1),
+                                                     (5, 1100000, 5, 2, 3, 0)])
+def test_block2d_offsets_and_wide(dev, rows, cols, ldg, pr, pc, off):
+    """2D-block pack and unpack from a global matrix that starts 8 bytes past
+    a 16-byte boundary (bands of odd heights, so the two sides' offsets
+    differ from column to column) and with more columns than one launch's
+    column groups, against the oracle."""
+    rng = np.random.default_rng(rows + cols + off)
+    G = rng.standard_normal(ldg * cols + off)
+    ro, co = oracle.block_splits(rows, pr), oracle.block_splits(cols, pc)
+    ref = oracle.pack_block2d(G[off:], ldg, rows, cols, ro, co)
+    Gd = dev_copy(G, dev)
+    P = torch.full((rows * cols,), float("nan"), dtype=torch.float64, device=dev)
+    dgamd.pack_block2d_t(Gd[off:], ldg, rows, cols, ro, co, P)
+    assert np.array_equal(P.cpu().numpy(), ref)
+    G2 = torch.full((ldg * cols + off,), 3.5, dtype=torch.float64, device=dev)
+    dgamd.unpack_block2d_t(P, ro, co, G2[off:], ldg, rows, cols)
+    g2 = G2.cpu().numpy()[off:].reshape(cols, ldg)
+    assert np.array_equal(g2[:, :rows], G[off:].reshape(cols, ldg)[:, :rows])
+    assert np.all(g2[:, rows:] == 3.5) and np.all(G2.cpu().numpy()[:off] == 3.5)
 
 
 @pytest.mark.parametrize("rows,cols,lds,ldd", [(60, 50, 101, 77), (1, 1, 1, 1), (4099, 3, 4099, 4100), (257, 129, 300, 257)])
