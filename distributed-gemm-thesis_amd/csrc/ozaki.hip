@@ -62,7 +62,7 @@ constexpr size_t kGemmLds = (size_t)STG * SSZ;
 static_assert(LA * 16 * NW == BM && LB * 16 * NW == BN, "loader split");
 static_assert(PAD % BM == 0 && PAD % BN == 0 && STG >= 4 && STG <= 5, "tile configuration (waits reach 3 steps ahead)");
 constexpr int NT = 256;
-constexpr int GROUP_M = 8;
+constexpr int GROUP_M = 4;
 
 __device__ __forceinline__ int xcd_remap(int bid, int total) {
     constexpr int X = 8;
@@ -212,7 +212,11 @@ __device__ __forceinline__ void stage_mma(iv4 (&acci)[FM][FN], fv4 (&accf)[FM][F
 }
 
 // Output tile of workgroup slot `bid` among `tiles`: XCD-aware, GROUP_M tile
-// rows sweep the tile columns together.
+// rows sweep the tile columns together.  With 256x256 tiles and 32 resident
+// workgroups per XCD, GROUP_M = 4 gives each XCD a 4 x 8 block of tiles (12
+// distinct operand slices per k-step, as for 8 x 4) while the A panels all
+// XCDs revisit stay at half the size, which keeps them in the last-level
+// cache (measured on the 16384^3 CRT product: 4.8% faster than 8 x 4).
 __device__ __forceinline__ void tile_origin(int bid, int tiles, int tiles_m, int tiles_n, int64_t& m0, int64_t& n0) {
     const int tile = xcd_remap(bid, tiles);
     const int per_group = GROUP_M * tiles_n;
@@ -413,26 +417,64 @@ struct ResidueConsts {
     double pd[kMaxModuli];    // p
 };
 
+// Tile order of the residue pass (blockIdx.x -> row tile, k tile): lines
+// contiguous in k walk each tile row along k; strided lines go in groups of
+// RT_KG k tiles, row tiles within a group, so that the workgroups in flight
+// read long runs of each column of op(X) and still write 2 KB runs of each
+// residue row.
+constexpr uint32_t RT_KG = 16;
+
 template <int NMOD>
 __global__ __launch_bounds__(256) void crt_residue_kernel(const double* __restrict__ x, int64_t rows, int64_t K,
                                                           int64_t rs, int64_t ks, const int32_t* __restrict__ e, int q,
                                                           int8_t* __restrict__ out, int64_t Kp, int64_t slice_bytes,
-                                                          ResidueConsts rc) {
+                                                          ResidueConsts rc, uint32_t nkt, uint32_t nrt) {
     __shared__ double tile[RT_R][RT_K + 1];
-    const int64_t r0 = (int64_t)blockIdx.y * RT_R;
-    const int64_t k0 = (int64_t)blockIdx.x * RT_K;
-    const int t = threadIdx.x;
+    uint32_t kt, rt;
     if (ks == 1) {
-#pragma unroll 4
-        for (int rr = t >> 7; rr < RT_R; rr += 2) {
-            const int64_t r = r0 + rr, k = k0 + (t & 127);
-            tile[rr][t & 127] = (r < rows && k < K) ? x[r * rs + k] : 0.0;
+        rt = blockIdx.x / nkt;
+        kt = blockIdx.x - rt * nkt;
+    } else {
+        const uint32_t grp = blockIdx.x / (RT_KG * nrt);
+        const uint32_t first = grp * RT_KG;
+        const uint32_t gk = nkt - first < RT_KG ? nkt - first : RT_KG;
+        const uint32_t in = blockIdx.x - grp * RT_KG * nrt;
+        rt = in / gk;
+        kt = first + (in - rt * gk);
+    }
+    const int64_t r0 = (int64_t)rt * RT_R;
+    const int64_t k0 = (int64_t)kt * RT_K;
+    const int t = threadIdx.x;
+    // Eight loads per thread, all issued before the first wait: addresses
+    // outside op(X) are clamped onto its last row / k (in bounds) and the
+    // loaded value replaced by zero.
+    if (ks == 1) {
+        const int64_t k = k0 + (t & 127);
+        const int64_t kc = k < K ? k : K - 1;
+        double v[RT_R / 2];
+#pragma unroll
+        for (int i = 0; i < RT_R / 2; ++i) {
+            const int64_t r = r0 + (t >> 7) + 2 * i;
+            v[i] = x[(r < rows ? r : rows - 1) * rs + kc];
+        }
+#pragma unroll
+        for (int i = 0; i < RT_R / 2; ++i) {
+            const int rr = (t >> 7) + 2 * i;
+            tile[rr][t & 127] = (r0 + rr < rows && k < K) ? v[i] : 0.0;
         }
     } else {
-#pragma unroll 4
-        for (int kk = t / RT_R; kk < RT_K; kk += 256 / RT_R) {
-            const int64_t r = r0 + (t % RT_R), k = k0 + kk;
-            tile[t % RT_R][kk] = (r < rows && k < K) ? x[r * rs + k * ks] : 0.0;
+        const int64_t r = r0 + (t % RT_R);
+        const int64_t rc = r < rows ? r : rows - 1;
+        double v[RT_K * RT_R / 256];
+#pragma unroll
+        for (int i = 0; i < RT_K * RT_R / 256; ++i) {
+            const int64_t k = k0 + t / RT_R + (256 / RT_R) * i;
+            v[i] = x[rc * rs + (k < K ? k : K - 1) * ks];
+        }
+#pragma unroll
+        for (int i = 0; i < RT_K * RT_R / 256; ++i) {
+            const int kk = t / RT_R + (256 / RT_R) * i;
+            tile[t % RT_R][kk] = (r < rows && k0 + kk < K) ? v[i] : 0.0;
         }
     }
     __syncthreads();
@@ -504,14 +546,24 @@ static_assert(BM == BN && BM / NW == 32, "scheme II loader: 32 rows per wave and
 
 __device__ __forceinline__ int next_slot(int x) { return x + 1 == NSLOT ? 0 : x + 1; }
 
-// This wave's 32 rows of one operand for k-step s into `slot`; g points at
-// the lane's row and chunk, rows `ld` bytes apart.
-__device__ __forceinline__ void crt_issue(int8_t* lds, int slot, int wave, const int8_t* g, int64_t s, int64_t ld) {
+// This wave's 32 rows of one operand for k-step s into `slot`; g (uniform
+// across the wave) points at the wave's first row, loff is the lane's
+// offset to its row and chunk, rows `ld` bytes apart.  A uniform base and a
+// 32-bit lane offset let every load take the scalar-base address form.
+__device__ __forceinline__ const int8_t* wave_uniform(const int8_t* p) {
+    const uint64_t v = reinterpret_cast<uint64_t>(p);
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return reinterpret_cast<const int8_t*>(((uint64_t)hi << 32) | lo);
+}
+
+__device__ __forceinline__ void crt_issue(int8_t* lds, int slot, int wave, const int8_t* g, uint32_t loff, int64_t s,
+                                          int64_t ld) {
     int8_t* d = lds + slot * CSLOT + wave * 4096;
 #pragma unroll
     for (int i = 0; i < 4; ++i)
-        __builtin_amdgcn_global_load_lds((gvoid_t*)(g + s * CK2 + (int64_t)(8 * i) * ld), (lds_void_t*)(d + 1024 * i), 16, 0,
-                                         0);
+        __builtin_amdgcn_global_load_lds((gvoid_t*)(wave_uniform(g + s * CK2 + (int64_t)(8 * i) * ld) + loff),
+                                         (lds_void_t*)(d + 1024 * i), 16, 0, 0);
 }
 
 // One MFMA half-step on fa / cb, reading the fragments of the following
@@ -537,6 +589,46 @@ __device__ __forceinline__ void crt_half(iv4 (&acc)[FM][FN], iv4 (&fa)[FM], cons
     }
 }
 
+// crt_half that also issues this wave's loads of B step sb (into slot_b)
+// and A step sa (into slot_a), one global_load_lds after each group of FN
+// MFMAs:
+// issued back to back, the eight 1 KB loads of every wave queue at the
+// vector memory unit while the matrix cores wait.
+__device__ __forceinline__ void crt_half_ld(iv4 (&acc)[FM][FN], iv4 (&fa)[FM], const iv4 (&cb)[FN], iv4 (&nb)[FN],
+                                            const int8_t* nas, const int8_t* nbs, int8_t* lds, int slot_b,
+                                            int slot_a, int wave, const int8_t* gb, const int8_t* ga, uint32_t loff,
+                                            int64_t sb, int64_t sa, int64_t ld) {
+    static_assert(FM == 8, "one load per MFMA group: four of B, then four of A");
+    int8_t* db = lds + slot_b * CSLOT + wave * 4096;
+    int8_t* da = lds + slot_a * CSLOT + wave * 4096;
+#pragma unroll
+    for (int i = 0; i < FM; ++i) {
+#pragma unroll
+        for (int j = 0; j < FN; ++j) acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa[i], cb[j], acc[i][j], 0, 0, 0);
+        if (i == 0) {
+#pragma unroll
+            for (int j = 0; j < FN; ++j) nb[j] = *reinterpret_cast<const iv4*>(nbs + 2048 * j);
+        }
+        fa[i] = *reinterpret_cast<const iv4*>(nas + 2048 * i);
+        if (i < 4)
+            __builtin_amdgcn_global_load_lds((gvoid_t*)(wave_uniform(gb + sb * CK2 + (int64_t)(8 * i) * ld) + loff),
+                                             (lds_void_t*)(db + 1024 * i), 16, 0, 0);
+        else
+            __builtin_amdgcn_global_load_lds(
+                (gvoid_t*)(wave_uniform(ga + sa * CK2 + (int64_t)(8 * (i - 4)) * ld) + loff),
+                (lds_void_t*)(da + 1024 * (i - 4)), 16, 0, 0);
+    }
+    __builtin_amdgcn_sched_group_barrier(0x008, FN, 0);
+    __builtin_amdgcn_sched_group_barrier(0x100, FN + 1, 0);
+    __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);
+#pragma unroll
+    for (int g = 1; g < FM; ++g) {
+        __builtin_amdgcn_sched_group_barrier(0x008, FN, 0);
+        __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+        __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);
+    }
+}
+
 // One INT8 GEMM per modulus in one launch: workgroup b computes tile
 // b % tiles of modulus b / tiles and stores the CRT digit
 // x_t = ((A'_t B'_t) mod p_t) * ((P / p_t)^-1 mod p_t) mod p_t as a byte.
@@ -558,8 +650,9 @@ __global__ __launch_bounds__(CNT, OCC) void crt_gemm_kernel(CrtArgs g) {
     // holding global chunk (lane % 8) ^ (row % 8)
     const int lrow = lane >> 3;
     const int lchunk = (lane & 7) ^ lrow;
-    const int8_t* ga = g.a + tmod * g.a_slice + (m0 + 32 * wave + lrow) * g.Kp + 16 * lchunk;
-    const int8_t* gb = g.b + tmod * g.b_slice + (n0 + 32 * wave + lrow) * g.Kp + 16 * lchunk;
+    const int8_t* ga = g.a + tmod * g.a_slice + (m0 + 32 * wave) * g.Kp;
+    const int8_t* gb = g.b + tmod * g.b_slice + (n0 + 32 * wave) * g.Kp;
+    const uint32_t loff = (uint32_t)(lrow * g.Kp + 16 * lchunk);
     const int roff0 = lr * CK2 + 16 * (lg ^ (lr & 7));
     const int roff1 = lr * CK2 + 16 * ((4 + lg) ^ (lr & 7));
 
@@ -570,17 +663,19 @@ __global__ __launch_bounds__(CNT, OCC) void crt_gemm_kernel(CrtArgs g) {
         for (int j = 0; j < FN; ++j) acc[i][j] = iv4{0, 0, 0, 0};
 
     const int64_t nk = g.Kp / CK2;  // >= 1
+    // Every step issues the same loads, so that every wait is the same: a
+    // load of a step past the end reloads the last step into a slot that is
+    // free and never read again (its data unused).
+    const auto clamp = [nk](int64_t s) { return s < nk ? s : nk - 1; };
     // prologue: A_0 B_0 A_1 B_1 A_2 into slots 0..4
+    crt_issue(lds, 0, wave, ga, loff, 0, g.Kp);
+    crt_issue(lds, 1, wave, gb, loff, 0, g.Kp);
+    crt_issue(lds, 2, wave, ga, loff, clamp(1), g.Kp);
+    crt_issue(lds, 3, wave, gb, loff, clamp(1), g.Kp);
+    crt_issue(lds, 4, wave, ga, loff, clamp(2), g.Kp);
     int isl = 0;
-    crt_issue(lds, isl, wave, ga, 0, g.Kp); isl = next_slot(isl);
-    crt_issue(lds, isl, wave, gb, 0, g.Kp); isl = next_slot(isl);
-    if (nk > 1) {
-        crt_issue(lds, isl, wave, ga, 1, g.Kp); isl = next_slot(isl);
-        crt_issue(lds, isl, wave, gb, 1, g.Kp); isl = next_slot(isl);
-    }
-    if (nk > 2) { crt_issue(lds, isl, wave, ga, 2, g.Kp); isl = next_slot(isl); }
     // step 0 landed: at most A_1 B_1 A_2 (4 loads each) pending
-    if (nk > 2) wait_vmcnt<12>(); else if (nk > 1) wait_vmcnt<8>(); else wait_vmcnt<0>();
+    wait_vmcnt<12>();
     __builtin_amdgcn_s_barrier();
     int csl = 0;  // slot of A_s; B_s in the next one
     iv4 fa[FM], cb[FN], nb[FN];
@@ -592,25 +687,32 @@ __global__ __launch_bounds__(CNT, OCC) void crt_gemm_kernel(CrtArgs g) {
 #pragma unroll
         for (int i = 0; i < FM; ++i) fa[i] = *reinterpret_cast<const iv4*>(as + 2048 * i);
     }
-    for (int64_t s = 0; s < nk; ++s) {
+    for (int64_t s = 0; s + 1 < nk; ++s) {
         const int bsl = next_slot(csl), asl1 = next_slot(bsl), bsl1 = next_slot(asl1);
         // first half of step s; fragments of its second half read meanwhile
         crt_half(acc, fa, cb, nb, lds + csl * CSLOT + wm * CK2 + roff1, lds + bsl * CSLOT + wn * CK2 + roff1);
-        if (s + 1 < nk) {
-            // every fragment of step s is in registers once this wave's LDS
-            // reads are done; A_{s+1} and B_{s+1} landed with at most A_{s+2}
-            // pending
-            __builtin_amdgcn_s_waitcnt((0xF) | (0x3 << 14) | (0x7 << 4));  // lgkmcnt(0)
-            if (s + 2 < nk) wait_vmcnt<4>(); else wait_vmcnt<0>();
-            __builtin_amdgcn_s_barrier();
-            if (s + 2 < nk) { crt_issue(lds, isl, wave, gb, s + 2, g.Kp); isl = next_slot(isl); }
-            if (s + 3 < nk) { crt_issue(lds, isl, wave, ga, s + 3, g.Kp); isl = next_slot(isl); }
-        }
-        // second half of step s; fragments of the first half of step s+1
-        // read meanwhile (in-bounds and unused after the last step)
-        crt_half(acc, fa, nb, cb, lds + asl1 * CSLOT + wm * CK2 + roff0, lds + bsl1 * CSLOT + wn * CK2 + roff0);
+        // every fragment of step s is in registers once this wave's LDS reads
+        // are done; A_{s+1} and B_{s+1} landed with at most A_{s+2} pending
+        __builtin_amdgcn_s_waitcnt((0xF) | (0x3 << 14) | (0x7 << 4));  // lgkmcnt(0)
+        wait_vmcnt<4>();
+        __builtin_amdgcn_s_barrier();
+        // second half of step s, fragments of the first half of step s+1 read
+        // meanwhile, B_{s+2} and A_{s+3} issued among its MFMAs into the slots
+        // of step s
+        const int sb = isl, sa = next_slot(isl);
+        isl = next_slot(sa);
+        crt_half_ld(acc, fa, nb, cb, lds + asl1 * CSLOT + wm * CK2 + roff0, lds + bsl1 * CSLOT + wn * CK2 + roff0,
+                    lds, sb, sa, wave, gb, ga, loff, clamp(s + 2), clamp(s + 3), g.Kp);
         csl = asl1;
     }
+    {
+        // last step: no loads; the second half's reads of a following step
+        // are in bounds and unused
+        const int bsl = next_slot(csl), asl1 = next_slot(bsl), bsl1 = next_slot(asl1);
+        crt_half(acc, fa, cb, nb, lds + csl * CSLOT + wm * CK2 + roff1, lds + bsl * CSLOT + wn * CK2 + roff1);
+        crt_half(acc, fa, nb, cb, lds + asl1 * CSLOT + wm * CK2 + roff0, lds + bsl1 * CSLOT + wn * CK2 + roff0);
+    }
+    wait_vmcnt<0>();  // the reloads past the end land before the workgroup's LDS is released
     // acc[i][j][v] = V_t(m0 + wm + 16i + 4lg + v, n0 + wn + 16j + lr): four
     // consecutive rows of one column, stored as one 32-bit word of digits.
     // x = acc * cinv is exact (|x| < 2^39) and floor(fl(x fl(1/p) + 2^-10)) is
@@ -692,17 +794,21 @@ __host__ __device__ inline double crt_value(double L0, double L1, double L2, con
 // when C allows it.
 // NMOD (the moduli count) is a template argument so that every digit load
 // of a thread is issued before the first is used.
+// Thread x (x < ceil(m/4) n < 2^32) takes rows 4 (x % qm) .. + 3 of column
+// x / qm, qm = ceil(m/4), divided by multiply-shift.
 template <int NMOD>
-__global__ void crt_reconstruct_kernel(const uint8_t* __restrict__ res, CrtTables tb, int64_t m, int64_t n,
-                                       int64_t Mp, int64_t plane, const int32_t* __restrict__ ea,
-                                       const int32_t* __restrict__ eb, int q, double alpha, double beta,
-                                       double* __restrict__ C, int64_t ldc) {
-    const int64_t qm = (m + 3) / 4;
-    const int64_t total = qm * n;
+__global__ __launch_bounds__(256) void crt_reconstruct_kernel(const uint8_t* __restrict__ res, CrtTables tb, int64_t m,
+                                                              int64_t n, int64_t Mp, int64_t plane,
+                                                              const int32_t* __restrict__ ea,
+                                                              const int32_t* __restrict__ eb, int q, double alpha,
+                                                              double beta, double* __restrict__ C, int64_t ldc,
+                                                              FastDiv dqm) {
+    const uint32_t total = dqm.d * (uint32_t)n;
     {
-        const int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;  // one thread per 4 rows
+        const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;  // one thread per 4 rows
         if (x >= total) return;
-        const int64_t j = x / qm, i0 = (x - j * qm) * 4;
+        const uint32_t jq = fdiv(x, dqm);
+        const int64_t j = jq, i0 = (int64_t)(x - jq * dqm.d) * 4;
         const uint8_t* rp = res + j * Mp + i0;
         uint32_t w[NMOD];
 #pragma unroll
@@ -873,24 +979,31 @@ __device__ __forceinline__ int crt_chunk_exp(double amax) {
 // Lines contiguous in k: 16 lanes per line with 16-byte pair loads when every
 // line starts 16-byte aligned, else a wave per line.
 __device__ __forceinline__ void scan_contiguous(const double* __restrict__ x, int64_t rows, int64_t K, int64_t rs,
-                                                int64_t k0, uint64_t* __restrict__ part) {
+                                                int64_t tile, int64_t chunk, uint64_t* __restrict__ part) {
+    const int64_t k0 = chunk * CRT_KC;
     if (pair_rows(x, rs)) {
         // 16 lanes per line, two 16-byte pairs each (k0 + 2 l16 and 32 further)
         const int l16 = threadIdx.x & 15;
-        const int64_t r = blockIdx.x * (int64_t)(blockDim.x >> 4) + (threadIdx.x >> 4);
+        const int64_t r = tile * (int64_t)(blockDim.x >> 4) + (threadIdx.x >> 4);
         if (r >= rows) return;  // whole 16-lane groups leave together
         const double* xr = x + r * rs;
         double v[4];
+        if (k0 + CRT_KC <= K) {  // the whole chunk inside the line: both loads in flight together
+            const pdv2 w0 = *reinterpret_cast<const pdv2*>(xr + k0 + 2 * l16);
+            const pdv2 w1 = *reinterpret_cast<const pdv2*>(xr + k0 + 32 + 2 * l16);
+            v[0] = w0.x; v[1] = w0.y; v[2] = w1.x; v[3] = w1.y;
+        } else {
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            const int64_t k = k0 + 32 * h + 2 * l16;
-            if (k + 1 < K) {
-                const pdv2 w = *reinterpret_cast<const pdv2*>(xr + k);
-                v[2 * h] = w.x;
-                v[2 * h + 1] = w.y;
-            } else {
-                v[2 * h] = k < K ? xr[k] : 0.0;
-                v[2 * h + 1] = 0.0;
+            for (int h = 0; h < 2; ++h) {
+                const int64_t k = k0 + 32 * h + 2 * l16;
+                if (k + 1 < K) {
+                    const pdv2 w = *reinterpret_cast<const pdv2*>(xr + k);
+                    v[2 * h] = w.x;
+                    v[2 * h + 1] = w.y;
+                } else {
+                    v[2 * h] = k < K ? xr[k] : 0.0;
+                    v[2 * h + 1] = 0.0;
+                }
             }
         }
         double mx = fmax(fmax(fabs(v[0]), fabs(v[1])), fmax(fabs(v[2]), fabs(v[3])));
@@ -901,10 +1014,10 @@ __device__ __forceinline__ void scan_contiguous(const double* __restrict__ x, in
                      (crt_bound_term(v[2], ec) + crt_bound_term(v[3], ec));
 #pragma unroll
         for (int off = 8; off > 0; off >>= 1) sum += __shfl_xor(sum, off);
-        if (l16 == 0) part[blockIdx.y * rows + r] = crt_chunk_word(mx, (uint64_t)sum);
+        if (l16 == 0) part[chunk * rows + r] = crt_chunk_word(mx, (uint64_t)sum);
     } else {
         const int lane = threadIdx.x & 63;
-        const int64_t r = blockIdx.x * (int64_t)(blockDim.x >> 6) + (threadIdx.x >> 6);
+        const int64_t r = tile * (int64_t)(blockDim.x >> 6) + (threadIdx.x >> 6);
         if (r >= rows) return;
         const double* xr = x + r * rs;
         const double v = k0 + lane < K ? xr[k0 + lane] : 0.0;
@@ -915,103 +1028,63 @@ __device__ __forceinline__ void scan_contiguous(const double* __restrict__ x, in
         double sum = crt_bound_term(v, ec);
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) sum += __shfl_xor(sum, off);
-        if (lane == 0) part[blockIdx.y * rows + r] = crt_chunk_word(mx, (uint64_t)sum);
+        if (lane == 0) part[chunk * rows + r] = crt_chunk_word(mx, (uint64_t)sum);
     }
 }
 
-// Lines strided in memory with 16-byte aligned line pairs (rs == 1, even ks,
-// 16-byte aligned x): a 128-line x 64-k tile through LDS, loaded as line
-// pairs (one 16-byte load covers lines 2j, 2j + 1 at one k, so a wave's load
-// reads 1 KB of a column), then two lanes per line, 32 k each.
-__global__ __launch_bounds__(256) void crt_scan_pairs_kernel(const double* __restrict__ x, int64_t rows, int64_t K,
-                                                             int64_t ks, uint64_t* __restrict__ part) {
-    __shared__ double tile[128][CRT_KC + 1];
+// Lines strided in memory (ks != 1): a thread per line, consecutive lanes on
+// consecutive lines (a wave's load reads 512 contiguous bytes when rs == 1),
+// every load of the chunk in flight at once.  Each entry is kept as its
+// exponent ex and mantissa bound M = ceil(256 m) (|x| = m 2^ex, M in
+// [128, 256], 0 for a zero entry) packed as ((ex + 2048) << 9) | M, so the
+// largest packed word of the chunk carries the exponent e of its largest
+// entry, and the terms of crt_bound_term follow from (M, ex).  blockIdx.x =
+// line tile, blockIdx.y = chunk (the grid walks down the columns).
+__global__ __launch_bounds__(256) void crt_scan_lines_kernel(const double* __restrict__ x, int64_t rows, int64_t K,
+                                                             int64_t rs, int64_t ks, uint64_t* __restrict__ part) {
+    const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (r >= rows) return;
     const int64_t k0 = (int64_t)blockIdx.y * CRT_KC;
-    const int t = threadIdx.x;
-    const int64_t r0 = (int64_t)blockIdx.x * 128;
-    {
-        const int j = t & 63;
-        const int64_t r = r0 + 2 * j;
+    const int64_t kn = K - k0 < CRT_KC ? K - k0 : CRT_KC;
+    const double* xr = x + r * rs + k0 * ks;
+    // every load unconditional (a clamped, in-bounds address past the end
+    // of the line), so that all of them issue before the first wait
+    double raw[CRT_KC];
 #pragma unroll
-        for (int i = 0; i < CRT_KC / 4; ++i) {
-            const int kk = (t >> 6) + 4 * i;
-            double a = 0.0, b = 0.0;
-            if (k0 + kk < K) {
-                if (r + 1 < rows) {
-                    const pdv2 w = *reinterpret_cast<const pdv2*>(x + r + (k0 + kk) * ks);
-                    a = w.x;
-                    b = w.y;
-                } else if (r < rows) {
-                    a = x[r + (k0 + kk) * ks];
-                }
-            }
-            tile[2 * j][kk] = a;
-            tile[2 * j + 1][kk] = b;
-        }
+    for (int i = 0; i < CRT_KC; ++i) raw[i] = xr[(i < kn ? i : kn - 1) * ks];
+    uint32_t wd[CRT_KC];
+    uint32_t top = 0;
+#pragma unroll
+    for (int i = 0; i < CRT_KC; ++i) {
+        const double v = i < kn ? raw[i] : 0.0;
+        int ex;
+        const double m = frexp(fabs(v), &ex);
+        const uint32_t M = (uint32_t)ceil(m * 256.0);
+        wd[i] = M ? ((uint32_t)(ex + 2048) << 9) | M : 0u;
+        top = top > wd[i] ? top : wd[i];
     }
-    __syncthreads();
-    const int rr = t >> 1, h = t & 1;
-    const int64_t r = r0 + rr;
-    double v[CRT_KC / 2];
-    double mx = 0.0;
-#pragma unroll
-    for (int i = 0; i < CRT_KC / 2; ++i) {
-        v[i] = tile[rr][32 * h + i];
-        mx = fmax(mx, fabs(v[i]));
-    }
-    mx = fmax(mx, __shfl_xor(mx, 1));
-    const int ec = crt_chunk_exp(mx);
-    double sum = 0.0;
-#pragma unroll
-    for (int i = 0; i < CRT_KC / 2; ++i) sum += crt_bound_term(v[i], ec);
-    sum += __shfl_xor(sum, 1);
-    if (h == 0 && r < rows) part[blockIdx.y * rows + r] = crt_chunk_word(mx, (uint64_t)sum);
-}
-
-// STRIDED: lines strided in memory (ks != 1); the static LDS tile of that
-// path would otherwise limit the occupancy of the k-contiguous paths too.
-template <bool STRIDED>
-__global__ __launch_bounds__(256) void crt_scan_kernel(const double* __restrict__ x, int64_t rows, int64_t K,
-                                                       int64_t rs, int64_t ks, uint64_t* __restrict__ part) {
-    const int64_t k0 = (int64_t)blockIdx.y * CRT_KC;
-    if constexpr (STRIDED) {
-        // 64 lines x 64 k through LDS: loads along the lines (consecutive
-        // lanes on consecutive lines, 16 per thread in flight), then 4 lanes
-        // per line, 16 k each, reduce the chunk with two shuffles.
-        __shared__ double tile[64][CRT_KC + 1];
-        const int t = threadIdx.x;
-        const int64_t r0 = (int64_t)blockIdx.x * 64;
-        {
-            const int64_t r = r0 + (t & 63);
-            const bool rv = r < rows;
-#pragma unroll
-            for (int i = 0; i < CRT_KC / 4; ++i) {
-                const int kk = (t >> 6) + 4 * i;
-                tile[t & 63][kk] = (rv && k0 + kk < K) ? x[r * rs + (k0 + kk) * ks] : 0.0;
-            }
-        }
-        __syncthreads();
-        const int rr = t >> 2, q = t & 3;
-        const int64_t r = r0 + rr;
-        double v[CRT_KC / 4];
-        double mx = 0.0;
-#pragma unroll
-        for (int i = 0; i < CRT_KC / 4; ++i) {
-            v[i] = tile[rr][16 * q + i];
-            mx = fmax(mx, fabs(v[i]));
-        }
-        mx = fmax(mx, __shfl_xor(mx, 1));
-        mx = fmax(mx, __shfl_xor(mx, 2));
-        const int ec = crt_chunk_exp(mx);
+    uint64_t word = 0;
+    if (top) {
+        const int ec = (int)(top >> 9) - 2048;
         double sum = 0.0;
 #pragma unroll
-        for (int i = 0; i < CRT_KC / 4; ++i) sum += crt_bound_term(v[i], ec);
-        sum += __shfl_xor(sum, 1);
-        sum += __shfl_xor(sum, 2);
-        if (q == 0 && r < rows) part[blockIdx.y * rows + r] = crt_chunk_word(mx, (uint64_t)sum);
-    } else {
-        scan_contiguous(x, rows, K, rs, k0, part);
+        for (int i = 0; i < CRT_KC; ++i) {
+            const double M = (double)(wd[i] & 511u);
+            const int ex = (int)(wd[i] >> 9) - 2048;
+            sum += wd[i] ? ceil(ldexp(M * M, -min(2 * (ec - ex) - 16, 64))) : 0.0;
+        }
+        word = ((uint64_t)(ec + 2048) << 48) | (uint64_t)sum;
     }
+    part[(int64_t)blockIdx.y * rows + r] = word;
+}
+
+// Lines contiguous in k: blockIdx.x = line tile * nchunk + chunk, so the grid
+// walks along the lines (consecutive workgroups read consecutive chunks of
+// the same lines).
+__global__ __launch_bounds__(256) void crt_scan_rows_kernel(const double* __restrict__ x, int64_t rows, int64_t K,
+                                                            int64_t rs, uint32_t nchunk, uint64_t* __restrict__ part) {
+    const uint32_t tile = blockIdx.x / nchunk;
+    scan_contiguous(x, rows, K, rs, tile, blockIdx.x - tile * nchunk, part);
 }
 
 // Line shifts: e = the largest chunk exponent, S = sum over the nonzero
@@ -1358,34 +1431,28 @@ int dg_ozaki_dgemm(char opA, char opB, int64_t m, int64_t n, int64_t k, double a
     if (crt) {
         // Line exponents and Cauchy-Schwarz shifts: one scan per operand
         // (chunk exponents and chunk bounds), then the per-line combination.
-        auto scan_grid = [](int64_t rows, int64_t ks, const double* x, int64_t rs) {
-            const int64_t per = ks == 1 ? (pair_rows(x, rs) ? NT / 16 : NT / 64) : 64;
-            return (unsigned)((rows + per - 1) / per);
-        };
+        // strided lines: a thread per line; contiguous lines: 16 lanes per
+        // line when the lines pair up in 16-byte loads, else a wave per line
         uint64_t* bp = reinterpret_cast<uint64_t*>(part);
-        auto pairs = [](const double* x, int64_t rs, int64_t ks) {
-            return rs == 1 && !(ks & 1) && !(reinterpret_cast<uintptr_t>(x) & 15);
+        auto scan = [&](const double* x, int64_t rows, int64_t rs, int64_t ks) -> int {
+            if (ks != 1) {
+                hipLaunchKernelGGL(crt_scan_lines_kernel, dim3((unsigned)((rows + 255) / 256), gy), dim3(NT), 0, st, x,
+                                   rows, k, rs, ks, bp);
+            } else {
+                const int64_t per = pair_rows(x, rs) ? NT / 16 : NT / 64;
+                const int64_t blocks = (rows + per - 1) / per * nchunk;
+                if (blocks > INT32_MAX) return fail(DG_ERR_UNSUPPORTED, "dg_ozaki_dgemm: too many lines for one scan");
+                hipLaunchKernelGGL(crt_scan_rows_kernel, dim3((unsigned)blocks), dim3(NT), 0, st, x, rows, k, rs,
+                                   (uint32_t)nchunk, bp);
+            }
+            return DG_OK;
         };
-        if (a_ks != 1 && pairs(A, a_rs, a_ks))
-            hipLaunchKernelGGL(crt_scan_pairs_kernel, dim3((unsigned)((m + 127) / 128), gy), dim3(NT), 0, st, A, m, k,
-                               a_ks, bp);
-        else if (a_ks != 1)
-            hipLaunchKernelGGL(crt_scan_kernel<true>, dim3(scan_grid(m, a_ks, A, a_rs), gy), dim3(NT), 0, st, A, m, k,
-                               a_rs, a_ks, bp);
-        else
-            hipLaunchKernelGGL(crt_scan_kernel<false>, dim3(scan_grid(m, a_ks, A, a_rs), gy), dim3(NT), 0, st, A, m, k,
-                               a_rs, a_ks, bp);
+        int src = scan(A, m, a_rs, a_ks);
+        if (src) return src;
         hipLaunchKernelGGL(crt_shift_kernel, dim3((unsigned)((p.Mp + 63) / 64)), dim3(NT), 0, st, bp, m, p.Mp,
                            nchunk, q, ea);
-        if (b_ks != 1 && pairs(B, b_rs, b_ks))
-            hipLaunchKernelGGL(crt_scan_pairs_kernel, dim3((unsigned)((n + 127) / 128), gy), dim3(NT), 0, st, B, n, k,
-                               b_ks, bp);
-        else if (b_ks != 1)
-            hipLaunchKernelGGL(crt_scan_kernel<true>, dim3(scan_grid(n, b_ks, B, b_rs), gy), dim3(NT), 0, st, B, n, k,
-                               b_rs, b_ks, bp);
-        else
-            hipLaunchKernelGGL(crt_scan_kernel<false>, dim3(scan_grid(n, b_ks, B, b_rs), gy), dim3(NT), 0, st, B, n, k,
-                               b_rs, b_ks, bp);
+        src = scan(B, n, b_rs, b_ks);
+        if (src) return src;
         hipLaunchKernelGGL(crt_shift_kernel, dim3((unsigned)((p.Np + 63) / 64)), dim3(NT), 0, st, bp, n, p.Np,
                            nchunk, q, eb);
         DG_LAUNCH_CHECK();
@@ -1412,8 +1479,10 @@ int dg_ozaki_dgemm(char opA, char opB, int64_t m, int64_t n, int64_t k, double a
         DG_LAUNCH_CHECK();
     }
     if (crt) {
-        dim3 ga((unsigned)(p.Kp / RT_K), (unsigned)(p.Mp / RT_R));
-        dim3 gb((unsigned)(p.Kp / RT_K), (unsigned)(p.Np / RT_R));
+        const uint32_t nkt = (uint32_t)(p.Kp / RT_K), nrta = (uint32_t)(p.Mp / RT_R), nrtb = (uint32_t)(p.Np / RT_R);
+        if ((int64_t)nkt * nrta > INT32_MAX || (int64_t)nkt * nrtb > INT32_MAX)
+            return fail(DG_ERR_UNSUPPORTED, "dg_ozaki_dgemm: too many residue tiles");
+        dim3 ga(nkt * nrta), gb(nkt * nrtb);
         decltype(&crt_residue_kernel<16>) rsk = crt_residue_kernel<16>;
         decltype(&crt_reconstruct_kernel<16>) rk = crt_reconstruct_kernel<16>;
         switch (slices) {
@@ -1431,8 +1500,8 @@ int dg_ozaki_dgemm(char opA, char opB, int64_t m, int64_t n, int64_t k, double a
             rcs.inv_p[t] = 1.0 / (double)pm;
             rcs.pd[t] = (double)pm;
         }
-        hipLaunchKernelGGL(rsk, ga, dim3(256), 0, st, A, m, k, a_rs, a_ks, ea, q, sa, p.Kp, p.a_slice, rcs);
-        hipLaunchKernelGGL(rsk, gb, dim3(256), 0, st, B, n, k, b_rs, b_ks, eb, q, sb, p.Kp, p.b_slice, rcs);
+        hipLaunchKernelGGL(rsk, ga, dim3(256), 0, st, A, m, k, a_rs, a_ks, ea, q, sa, p.Kp, p.a_slice, rcs, nkt, nrta);
+        hipLaunchKernelGGL(rsk, gb, dim3(256), 0, st, B, n, k, b_rs, b_ks, eb, q, sb, p.Kp, p.b_slice, rcs, nkt, nrtb);
         DG_LAUNCH_CHECK();
         CrtArgs cg{};
         cg.a = sa; cg.b = sb; cg.res = reinterpret_cast<uint8_t*>(ws + p.off_w);
@@ -1444,10 +1513,11 @@ int dg_ozaki_dgemm(char opA, char opB, int64_t m, int64_t n, int64_t k, double a
         if (grid > INT32_MAX) return fail(DG_ERR_INVALID, "dg_ozaki_dgemm: too many tiles");
         hipLaunchKernelGGL(crt_gemm_kernel, dim3((unsigned)grid), dim3(CNT), kCrtLds, st, cg);
         DG_LAUNCH_CHECK();
-        const int64_t blocks = (((m + 3) / 4) * n + 255) / 256;
-        if (blocks > INT32_MAX) return fail(DG_ERR_INVALID, "dg_ozaki_dgemm: too many output elements");
-        hipLaunchKernelGGL(rk, dim3((unsigned)blocks), dim3(256), 0, st, cg.res, tables, m, n, p.Mp, p.res_plane, ea,
-                           eb, q, alpha, beta, C, ldc);
+        const int64_t quads = ((m + 3) / 4) * n;
+        if (quads > (int64_t)UINT32_MAX - 255)
+            return fail(DG_ERR_UNSUPPORTED, "dg_ozaki_dgemm: too many output elements for one reconstruction");
+        hipLaunchKernelGGL(rk, dim3((unsigned)((quads + 255) / 256)), dim3(256), 0, st, cg.res, tables, m, n, p.Mp,
+                           p.res_plane, ea, eb, q, alpha, beta, C, ldc, make_fastdiv((uint32_t)((m + 3) / 4)));
         DG_LAUNCH_CHECK();
         return DG_OK;
     }

@@ -125,31 +125,6 @@ __global__ __launch_bounds__(kThreads) void block2d_kernel(const double* __restr
     }
 }
 
-// Division of 32-bit unsigned integers by a launch constant d >= 1 with
-// one high multiply, a subtract, an add and two shifts (Granlund and
-// Montgomery, "Division by invariant integers using multiplication", PLDI
-// 1994, figure 4.1): l = ceil(log2 d), m = floor(2^32 (2^l - d) / d) + 1,
-// t = mulhi(m, n), n / d = (t + ((n - t) >> min(l, 1))) >> max(l - 1, 0).
-struct FastDiv {
-    uint32_t d, m, s1, s2;
-};
-
-FastDiv make_fastdiv(uint32_t d) {
-    FastDiv f{};
-    uint32_t l = 0;
-    while (l < 32 && ((uint64_t)1 << l) < d) ++l;
-    f.d = d;
-    f.m = (uint32_t)((((uint64_t)1 << 32) * (((uint64_t)1 << l) - d)) / d + 1);
-    f.s1 = l < 1 ? l : 1;
-    f.s2 = l > 1 ? l - 1 : 0;
-    return f;
-}
-
-__device__ __forceinline__ uint32_t fdiv(uint32_t n, const FastDiv& f) {
-    const uint32_t t = __umulhi(f.m, n);
-    return (t + ((n - t) >> f.s1)) >> f.s2;
-}
-
 // numroc(n, b, p, 0, procs) = base*b + (b if p < extra, rem if p == extra,
 // else 0) with full = n / b, rem = n % b, base = full / procs, extra = full
 // % procs, and its prefix over the processes before p.
