@@ -183,4 +183,18 @@ int launch_scale(int64_t m, int64_t n, double beta, double* C, int64_t ldc, hipS
 // inner dimension k (16 when none does).
 int crt_moduli_for_fp64(int64_t k);
 
+// Ozaki scheme II in two parts.  A prepared operand holds the residues and
+// line exponents of `rows` lines of length k (crt_operand_bytes bytes);
+// crt_prepare builds one from lines x[i*rs + k*ks] with a scan buffer of
+// crt_scan_bytes bytes; crt_multiply forms C = alpha op(A) op(B) + beta C
+// from prepared op(A) rows and op(B) columns with a digit buffer of
+// crt_digit_bytes bytes.  Buffers 256-byte aligned.
+int64_t crt_operand_bytes(int64_t rows, int64_t k, int nmod);
+int64_t crt_scan_bytes(int64_t rows, int64_t k);
+int64_t crt_digit_bytes(int64_t m, int64_t n, int nmod);
+int crt_prepare(const double* x, int64_t rows, int64_t k, int64_t rs, int64_t ks, int nmod, void* operand,
+                void* scan, hipStream_t st);
+int crt_multiply(int64_t m, int64_t n, int64_t k, int nmod, double alpha, const void* a_op, const void* b_op,
+                 double beta, double* C, int64_t ldc, void* digits, hipStream_t st);
+
 }  // namespace dg

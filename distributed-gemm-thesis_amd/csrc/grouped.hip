@@ -8,6 +8,7 @@
 #include <map>
 #include <mutex>
 #include <string>
+#include <tuple>
 
 #include "dg_internal.hpp"
 
@@ -37,6 +38,21 @@ struct dg_grouped_plan {
     std::vector<int> oz_idx;            // problems run in the Ozaki form, in order
     void* oz_ws = nullptr;
     int64_t oz_ws_bytes = 0;
+    // Operands shared by Ozaki problems (a tile row of A read by every tile of
+    // that row, ...): each distinct line set is prepared once per launch into
+    // oz_shared at `off`, then every product reads it.  Empty when no
+    // operand repeats or the buffers do not fit; each problem then runs
+    // dg_ozaki_dgemm in oz_ws.
+    struct OzOperand {
+        const double* x;
+        int64_t rows, k, rs, ks;
+        int nmod;
+        int64_t off;
+    };
+    std::vector<OzOperand> oz_ops;
+    std::vector<int> oz_a, oz_b;   // per entry of oz_idx: its op(A) rows and op(B) columns in oz_ops
+    void* oz_shared = nullptr;
+    int64_t oz_scan_off = 0, oz_digit_off = 0;
 };
 
 namespace dg {
@@ -154,11 +170,80 @@ int plan_launch(dg_grouped_plan* p, int variant, int cfg, hipStream_t st, bool w
         int rc = launch_gemm_f64_dev(g.opA, g.opB, variant, cfg, g.b, p->counter, st);
         if (rc) return rc;
     }
-    if (with_ozaki)
-        for (int i : p->oz_idx) {
-            int rc = launch_ozaki_problem(p, i, st);
+    if (!with_ozaki) return DG_OK;
+    if (p->oz_shared) {
+        char* base = static_cast<char*>(p->oz_shared);
+        for (const auto& o : p->oz_ops) {
+            int rc = crt_prepare(o.x, o.rows, o.k, o.rs, o.ks, o.nmod, base + o.off, base + p->oz_scan_off, st);
             if (rc) return rc;
         }
+        for (size_t t = 0; t < p->oz_idx.size(); ++t) {
+            const int i = p->oz_idx[t];
+            const dg_gemm_desc& d = p->descs[i];
+            int rc = crt_multiply(d.m, d.n, d.k, p->prob_moduli[i], d.alpha, base + p->oz_ops[p->oz_a[t]].off,
+                                  base + p->oz_ops[p->oz_b[t]].off, d.beta, d.C, d.ldc, base + p->oz_digit_off, st);
+            if (rc) return rc;
+        }
+        return DG_OK;
+    }
+    for (int i : p->oz_idx) {
+        int rc = launch_ozaki_problem(p, i, st);
+        if (rc) return rc;
+    }
+    return DG_OK;
+}
+
+// Finds the operands that several Ozaki problems read (same lines: pointer,
+// line count, k, strides, moduli) and, when any repeats and the prepared
+// copies fit in half of the free device memory, allocates one buffer for
+// all of them plus one scan and one digit buffer; otherwise leaves the
+// per-problem path in place.  The products are bit for bit those of
+// dg_ozaki_dgemm: a prepared operand depends only on its own lines.
+int share_operands(dg_grouped_plan* p) {
+    p->oz_ops.clear();
+    p->oz_a.clear();
+    p->oz_b.clear();
+    std::map<std::tuple<uintptr_t, int64_t, int64_t, int64_t, int64_t, int>, int> index;
+    int64_t off = 0, scan = 0, digits = 0;
+    auto add = [&](const double* x, int64_t rows, int64_t k, int64_t rs, int64_t ks, int nmod) {
+        const auto key = std::make_tuple(reinterpret_cast<uintptr_t>(x), rows, k, rs, ks, nmod);
+        auto it = index.find(key);
+        if (it != index.end()) return it->second;
+        const int id = (int)p->oz_ops.size();
+        p->oz_ops.push_back({x, rows, k, rs, ks, nmod, off});
+        off += (crt_operand_bytes(rows, k, nmod) + 255) / 256 * 256;
+        scan = std::max(scan, crt_scan_bytes(rows, k));
+        index[key] = id;
+        return id;
+    };
+    for (int i : p->oz_idx) {
+        const dg_gemm_desc& d = p->descs[i];
+        const char oa = norm_op(d.opA), ob = norm_op(d.opB);
+        const int nm = p->prob_moduli[i];
+        // op(A)(i, k) at A[i*rs + k*ks]; op(B) as the rows of its transpose
+        p->oz_a.push_back(add(d.A, d.m, d.k, oa == 'N' ? 1 : d.lda, oa == 'N' ? d.lda : 1, nm));
+        p->oz_b.push_back(add(d.B, d.n, d.k, ob == 'N' ? d.ldb : 1, ob == 'N' ? 1 : d.ldb, nm));
+        digits = std::max(digits, crt_digit_bytes(d.m, d.n, nm));
+    }
+    size_t free_b = 0, total_b = 0;
+    const int64_t bytes = off + (scan + 255) / 256 * 256 + digits;
+    const bool repeats = p->oz_ops.size() < 2 * p->oz_idx.size();
+    if (!repeats || hipMemGetInfo(&free_b, &total_b) != hipSuccess || bytes > (int64_t)(free_b / 2) ||
+        hipMalloc(&p->oz_shared, (size_t)bytes) != hipSuccess) {
+        (void)hipGetLastError();  // a refused allocation is not an error here
+        p->oz_shared = nullptr;
+        p->oz_ops.clear();
+        p->oz_a.clear();
+        p->oz_b.clear();
+        return DG_OK;
+    }
+    p->oz_scan_off = off;
+    p->oz_digit_off = off + (scan + 255) / 256 * 256;
+    if (p->oz_ws) {  // the per-problem workspace served the form trials only
+        (void)hipFree(p->oz_ws);
+        p->oz_ws = nullptr;
+        p->oz_ws_bytes = 0;
+    }
     return DG_OK;
 }
 
@@ -341,6 +426,10 @@ int choose_forms(dg_grouped_plan* p, hipStream_t st) {
         (void)hipFree(p->oz_ws);
         p->oz_ws = nullptr;
         p->oz_ws_bytes = 0;
+    }
+    if (!p->oz_idx.empty()) {
+        int rc = share_operands(p);
+        if (rc) return rc;
     }
     int rc = build_groups(p, p->descs.data(), count, &use);
     if (rc) return rc;
@@ -656,6 +745,7 @@ int dg_grouped_plan_problem_form(const dg_grouped_plan* p, int index, int* modul
 int dg_grouped_plan_destroy(dg_grouped_plan* p) {
     if (!p) return DG_OK;
     if (p->oz_ws) (void)hipFree(p->oz_ws);
+    if (p->oz_shared) (void)hipFree(p->oz_shared);
     if (p->graph) (void)hipGraphExecDestroy(p->graph);
     if (p->capture_stream) (void)hipStreamDestroy(p->capture_stream);
     for (auto& g : p->groups) {

@@ -1189,6 +1189,28 @@ constexpr int64_t kMaxChunk = 256;  // k-chunk of the max reduction
 
 int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
 
+}  // namespace
+
+// Prepared operand of the CRT form (one side of a product): the balanced
+// residues of its scaled lines, [nmod][rows_p][Kp] int8 with rows_p = rows
+// rounded up to the tile height and Kp = k rounded up to the k-step (zeros
+// in the padding), then the lines' exponents e' = Q - sigma (rows_p int32).
+int64_t crt_operand_bytes(int64_t rows, int64_t k, int nmod) {
+    const int64_t rp = round_up(rows > 0 ? rows : 1, PAD), kp = round_up(k > 0 ? k : 1, KSB);
+    return round_up((int64_t)nmod * rp * kp, 256) + rp * 4;
+}
+
+int64_t crt_scan_bytes(int64_t rows, int64_t k) {
+    const int64_t nchunk = k > 0 ? (k + CRT_KC - 1) / CRT_KC : 1;
+    return nchunk * (rows > 0 ? rows : 1) * 8;
+}
+
+int64_t crt_digit_bytes(int64_t m, int64_t n, int nmod) {
+    return (int64_t)nmod * round_up(m > 0 ? m : 1, PAD) * round_up(n > 0 ? n : 1, PAD);
+}
+
+namespace {
+
 struct Plan {
     int64_t Mp, Np, Kp, esz, a_slice, b_slice, res_plane, nchunk_a, nchunk_b, kchunk_max;
     size_t off_a, off_b, off_ea, off_eb, off_part, off_w, total;
@@ -1248,6 +1270,8 @@ int crt_moduli_for_fp64(int64_t k) {
 
 namespace {
 
+// Workspace of one CRT call: the prepared A and B operands (crt_operand_bytes
+// each), the line-scan words, then the CRT digits of V.
 Plan make_crt_plan(int64_t m, int64_t n, int64_t k, int nmod) {
     Plan p{};
     p.esz = 1;
@@ -1263,12 +1287,10 @@ Plan make_crt_plan(int64_t m, int64_t n, int64_t k, int nmod) {
     p.nchunk_b = p.nchunk_a;
     size_t off = 0;
     auto take = [&](size_t bytes) { size_t o = off; off = (off + bytes + 255) / 256 * 256; return o; };
-    p.off_a = take((size_t)nmod * p.a_slice);
-    p.off_b = take((size_t)nmod * p.b_slice);
-    p.off_ea = take((size_t)p.Mp * 4);
-    p.off_eb = take((size_t)p.Np * 4);
-    p.off_part = take((size_t)p.nchunk_a * (size_t)(m > n ? m : n) * 8);
-    p.off_w = take((size_t)nmod * (size_t)p.res_plane);  // CRT digits of V
+    p.off_a = take((size_t)crt_operand_bytes(m, k, nmod));
+    p.off_b = take((size_t)crt_operand_bytes(n, k, nmod));
+    p.off_part = take((size_t)crt_scan_bytes(m > n ? m : n, k));
+    p.off_w = take((size_t)crt_digit_bytes(m, n, nmod));
     p.total = off;
     return p;
 }
@@ -1345,6 +1367,104 @@ std::vector<OzSeg> make_schedule(int s, int bits, int64_t Kp, int64_t K, bool bf
 }
 
 }  // namespace
+
+// Scheme II, one operand: line scan, shifts and residues into `operand`
+// (crt_operand_bytes(rows, k, nmod) bytes, 256-byte aligned) for lines
+// x[i*rs + k*ks], using `scan` (crt_scan_bytes(rows, k) bytes).  The result
+// depends only on the operand, k and nmod, so a prepared operand serves
+// every product that reads it.
+int crt_prepare(const double* x, int64_t rows, int64_t k, int64_t rs, int64_t ks, int nmod, void* operand,
+                void* scan, hipStream_t st) {
+    CrtTables tables{};
+    int q = 0;
+    int rc = crt_setup(k, nmod, &tables, &q);
+    if (rc) return rc;
+    const int64_t rp = round_up(rows, PAD), kp = round_up(k, KSB);
+    const int nchunk = (int)((k + CRT_KC - 1) / CRT_KC);
+    int8_t* slices = static_cast<int8_t*>(operand);
+    int32_t* e = reinterpret_cast<int32_t*>(static_cast<char*>(operand) + round_up((int64_t)nmod * rp * kp, 256));
+    uint64_t* bp = static_cast<uint64_t*>(scan);
+    // Line exponents and Cauchy-Schwarz shifts: one scan (chunk exponents
+    // and chunk bounds), then the per-line combination.  Strided lines: a
+    // thread per line; contiguous lines: 16 lanes per line when the lines
+    // pair up in 16-byte loads, else a wave per line.
+    if (ks != 1) {
+        hipLaunchKernelGGL(crt_scan_lines_kernel, dim3((unsigned)((rows + 255) / 256), (unsigned)nchunk), dim3(NT), 0,
+                           st, x, rows, k, rs, ks, bp);
+    } else {
+        const int64_t per = pair_rows(x, rs) ? NT / 16 : NT / 64;
+        const int64_t blocks = (rows + per - 1) / per * nchunk;
+        if (blocks > INT32_MAX) return fail(DG_ERR_UNSUPPORTED, "dg_ozaki_dgemm: too many lines for one scan");
+        hipLaunchKernelGGL(crt_scan_rows_kernel, dim3((unsigned)blocks), dim3(NT), 0, st, x, rows, k, rs,
+                           (uint32_t)nchunk, bp);
+    }
+    hipLaunchKernelGGL(crt_shift_kernel, dim3((unsigned)((rp + 63) / 64)), dim3(NT), 0, st, bp, rows, rp, nchunk, q, e);
+    const uint32_t nkt = (uint32_t)(kp / RT_K), nrt = (uint32_t)(rp / RT_R);
+    if ((int64_t)nkt * nrt > INT32_MAX) return fail(DG_ERR_UNSUPPORTED, "dg_ozaki_dgemm: too many residue tiles");
+    decltype(&crt_residue_kernel<16>) rsk = crt_residue_kernel<16>;
+    switch (nmod) {
+#define DG_CRT_CASE(N) case N: rsk = crt_residue_kernel<N>; break;
+        DG_CRT_CASE(2) DG_CRT_CASE(3) DG_CRT_CASE(4) DG_CRT_CASE(5) DG_CRT_CASE(6) DG_CRT_CASE(7)
+        DG_CRT_CASE(8) DG_CRT_CASE(9) DG_CRT_CASE(10) DG_CRT_CASE(11) DG_CRT_CASE(12) DG_CRT_CASE(13)
+        DG_CRT_CASE(14) DG_CRT_CASE(15)
+#undef DG_CRT_CASE
+        default: break;
+    }
+    ResidueConsts rcs{};
+    for (int t = 0; t < kMaxModuli; ++t) {
+        const int pm = crt_modulus(t);
+        rcs.c24[t] = (double)((1 << 24) % pm);
+        rcs.inv_p[t] = 1.0 / (double)pm;
+        rcs.pd[t] = (double)pm;
+    }
+    hipLaunchKernelGGL(rsk, dim3(nkt * nrt), dim3(256), 0, st, x, rows, k, rs, ks, e, q, slices, kp, rp * kp, rcs, nkt,
+                       nrt);
+    DG_LAUNCH_CHECK();
+    return DG_OK;
+}
+
+// Scheme II, the product of two prepared operands (m lines of op(A), n lines
+// of op(B)^T, both over k with nmod moduli): the nmod INT8 GEMMs into the CRT
+// digits (`digits`, crt_digit_bytes(m, n, nmod) bytes), then
+// C = alpha * V * 2^(e'_i + f'_j - 2Q) + beta * C.
+int crt_multiply(int64_t m, int64_t n, int64_t k, int nmod, double alpha, const void* a_op, const void* b_op,
+                 double beta, double* C, int64_t ldc, void* digits, hipStream_t st) {
+    CrtTables tables{};
+    int q = 0;
+    int rc = crt_setup(k, nmod, &tables, &q);
+    if (rc) return rc;
+    const int64_t mp = round_up(m, PAD), np = round_up(n, PAD), kp = round_up(k, KSB);
+    CrtArgs cg{};
+    cg.a = static_cast<const int8_t*>(a_op);
+    cg.b = static_cast<const int8_t*>(b_op);
+    cg.res = static_cast<uint8_t*>(digits);
+    cg.Mp = mp; cg.Np = np; cg.Kp = kp; cg.a_slice = mp * kp; cg.b_slice = np * kp;
+    cg.res_plane = mp * np;
+    cg.tiles_m = (int32_t)(mp / BM); cg.tiles_n = (int32_t)(np / BN); cg.nmod = nmod;
+    for (int t = 0; t < kMaxModuli; ++t) cg.cinv[t] = tables.cinv[t];
+    const int32_t* ea = reinterpret_cast<const int32_t*>(static_cast<const char*>(a_op) + round_up((int64_t)nmod * mp * kp, 256));
+    const int32_t* eb = reinterpret_cast<const int32_t*>(static_cast<const char*>(b_op) + round_up((int64_t)nmod * np * kp, 256));
+    const int64_t grid = (int64_t)cg.tiles_m * cg.tiles_n * nmod;
+    if (grid > INT32_MAX) return fail(DG_ERR_INVALID, "dg_ozaki_dgemm: too many tiles");
+    const int64_t quads = ((m + 3) / 4) * n;
+    if (quads > (int64_t)UINT32_MAX - 255)
+        return fail(DG_ERR_UNSUPPORTED, "dg_ozaki_dgemm: too many output elements for one reconstruction");
+    hipLaunchKernelGGL(crt_gemm_kernel, dim3((unsigned)grid), dim3(CNT), kCrtLds, st, cg);
+    DG_LAUNCH_CHECK();
+    decltype(&crt_reconstruct_kernel<16>) rk = crt_reconstruct_kernel<16>;
+    switch (nmod) {
+#define DG_CRT_CASE(N) case N: rk = crt_reconstruct_kernel<N>; break;
+        DG_CRT_CASE(2) DG_CRT_CASE(3) DG_CRT_CASE(4) DG_CRT_CASE(5) DG_CRT_CASE(6) DG_CRT_CASE(7)
+        DG_CRT_CASE(8) DG_CRT_CASE(9) DG_CRT_CASE(10) DG_CRT_CASE(11) DG_CRT_CASE(12) DG_CRT_CASE(13)
+        DG_CRT_CASE(14) DG_CRT_CASE(15)
+#undef DG_CRT_CASE
+        default: break;
+    }
+    hipLaunchKernelGGL(rk, dim3((unsigned)((quads + 255) / 256)), dim3(256), 0, st, cg.res, tables, m, n, mp,
+                       cg.res_plane, ea, eb, q, alpha, beta, C, ldc, make_fastdiv((uint32_t)((m + 3) / 4)));
+    DG_LAUNCH_CHECK();
+    return DG_OK;
+}
 
 }  // namespace dg
 
@@ -1429,34 +1549,13 @@ int dg_ozaki_dgemm(char opA, char opB, int64_t m, int64_t n, int64_t k, double a
     const unsigned gy = (unsigned)nchunk;
 
     if (crt) {
-        // Line exponents and Cauchy-Schwarz shifts: one scan per operand
-        // (chunk exponents and chunk bounds), then the per-line combination.
-        // strided lines: a thread per line; contiguous lines: 16 lanes per
-        // line when the lines pair up in 16-byte loads, else a wave per line
-        uint64_t* bp = reinterpret_cast<uint64_t*>(part);
-        auto scan = [&](const double* x, int64_t rows, int64_t rs, int64_t ks) -> int {
-            if (ks != 1) {
-                hipLaunchKernelGGL(crt_scan_lines_kernel, dim3((unsigned)((rows + 255) / 256), gy), dim3(NT), 0, st, x,
-                                   rows, k, rs, ks, bp);
-            } else {
-                const int64_t per = pair_rows(x, rs) ? NT / 16 : NT / 64;
-                const int64_t blocks = (rows + per - 1) / per * nchunk;
-                if (blocks > INT32_MAX) return fail(DG_ERR_UNSUPPORTED, "dg_ozaki_dgemm: too many lines for one scan");
-                hipLaunchKernelGGL(crt_scan_rows_kernel, dim3((unsigned)blocks), dim3(NT), 0, st, x, rows, k, rs,
-                                   (uint32_t)nchunk, bp);
-            }
-            return DG_OK;
-        };
-        int src = scan(A, m, a_rs, a_ks);
-        if (src) return src;
-        hipLaunchKernelGGL(crt_shift_kernel, dim3((unsigned)((p.Mp + 63) / 64)), dim3(NT), 0, st, bp, m, p.Mp,
-                           nchunk, q, ea);
-        src = scan(B, n, b_rs, b_ks);
-        if (src) return src;
-        hipLaunchKernelGGL(crt_shift_kernel, dim3((unsigned)((p.Np + 63) / 64)), dim3(NT), 0, st, bp, n, p.Np,
-                           nchunk, q, eb);
-        DG_LAUNCH_CHECK();
-    } else {
+        // op(A)'s rows and op(B)'s columns prepared in the workspace, then their product
+        rc = crt_prepare(A, m, k, a_rs, a_ks, slices, ws + p.off_a, ws + p.off_part, st);
+        if (!rc) rc = crt_prepare(B, n, k, b_rs, b_ks, slices, ws + p.off_b, ws + p.off_part, st);
+        if (!rc) rc = crt_multiply(m, n, k, slices, alpha, ws + p.off_a, ws + p.off_b, beta, C, ldc, ws + p.off_w, st);
+        return rc;
+    }
+    {
         // Row exponents of op(A), column exponents of op(B).
         // ks == 1: 16 lanes per row when the rows pair up (NT/16 rows per block), else
         // a wave per row (NT/64); otherwise a thread per row
@@ -1477,49 +1576,6 @@ int dg_ozaki_dgemm(char opA, char opB, int64_t m, int64_t n, int64_t k, double a
         hipLaunchKernelGGL(oz_exponent_kernel, dim3((unsigned)((p.Np + NT - 1) / NT)), dim3(NT), 0, st, part, n, p.Np,
                            nchunk, eb);
         DG_LAUNCH_CHECK();
-    }
-    if (crt) {
-        const uint32_t nkt = (uint32_t)(p.Kp / RT_K), nrta = (uint32_t)(p.Mp / RT_R), nrtb = (uint32_t)(p.Np / RT_R);
-        if ((int64_t)nkt * nrta > INT32_MAX || (int64_t)nkt * nrtb > INT32_MAX)
-            return fail(DG_ERR_UNSUPPORTED, "dg_ozaki_dgemm: too many residue tiles");
-        dim3 ga(nkt * nrta), gb(nkt * nrtb);
-        decltype(&crt_residue_kernel<16>) rsk = crt_residue_kernel<16>;
-        decltype(&crt_reconstruct_kernel<16>) rk = crt_reconstruct_kernel<16>;
-        switch (slices) {
-#define DG_CRT_CASE(N) case N: rsk = crt_residue_kernel<N>; rk = crt_reconstruct_kernel<N>; break;
-            DG_CRT_CASE(2) DG_CRT_CASE(3) DG_CRT_CASE(4) DG_CRT_CASE(5) DG_CRT_CASE(6) DG_CRT_CASE(7)
-            DG_CRT_CASE(8) DG_CRT_CASE(9) DG_CRT_CASE(10) DG_CRT_CASE(11) DG_CRT_CASE(12) DG_CRT_CASE(13)
-            DG_CRT_CASE(14) DG_CRT_CASE(15)
-#undef DG_CRT_CASE
-            default: break;
-        }
-        ResidueConsts rcs{};
-        for (int t = 0; t < kMaxModuli; ++t) {
-            const int pm = crt_modulus(t);
-            rcs.c24[t] = (double)((1 << 24) % pm);
-            rcs.inv_p[t] = 1.0 / (double)pm;
-            rcs.pd[t] = (double)pm;
-        }
-        hipLaunchKernelGGL(rsk, ga, dim3(256), 0, st, A, m, k, a_rs, a_ks, ea, q, sa, p.Kp, p.a_slice, rcs, nkt, nrta);
-        hipLaunchKernelGGL(rsk, gb, dim3(256), 0, st, B, n, k, b_rs, b_ks, eb, q, sb, p.Kp, p.b_slice, rcs, nkt, nrtb);
-        DG_LAUNCH_CHECK();
-        CrtArgs cg{};
-        cg.a = sa; cg.b = sb; cg.res = reinterpret_cast<uint8_t*>(ws + p.off_w);
-        cg.Mp = p.Mp; cg.Np = p.Np; cg.Kp = p.Kp; cg.a_slice = p.a_slice; cg.b_slice = p.b_slice;
-        cg.res_plane = p.res_plane;
-        cg.tiles_m = (int32_t)(p.Mp / BM); cg.tiles_n = (int32_t)(p.Np / BN); cg.nmod = slices;
-        for (int t = 0; t < kMaxModuli; ++t) cg.cinv[t] = tables.cinv[t];
-        const int64_t grid = (int64_t)cg.tiles_m * cg.tiles_n * slices;
-        if (grid > INT32_MAX) return fail(DG_ERR_INVALID, "dg_ozaki_dgemm: too many tiles");
-        hipLaunchKernelGGL(crt_gemm_kernel, dim3((unsigned)grid), dim3(CNT), kCrtLds, st, cg);
-        DG_LAUNCH_CHECK();
-        const int64_t quads = ((m + 3) / 4) * n;
-        if (quads > (int64_t)UINT32_MAX - 255)
-            return fail(DG_ERR_UNSUPPORTED, "dg_ozaki_dgemm: too many output elements for one reconstruction");
-        hipLaunchKernelGGL(rk, dim3((unsigned)((quads + 255) / 256)), dim3(256), 0, st, cg.res, tables, m, n, p.Mp,
-                           p.res_plane, ea, eb, q, alpha, beta, C, ldc, make_fastdiv((uint32_t)((m + 3) / 4)));
-        DG_LAUNCH_CHECK();
-        return DG_OK;
     }
     // Slices (the padding is written as zeros by the same kernels).
     {

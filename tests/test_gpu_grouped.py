@@ -193,3 +193,55 @@ def test_ozaki_form_fixed_moduli(dev):
     else:
         assert (f, mod) == (dgamd.DG_FORM_FP64, 0) and err <= TOL
     plan.close()
+
+
+def test_ozaki_form_shared_operands(dev):
+    """Ozaki-form problems that read the same lines (the 2x2 tiles of one
+    8192^3 product share tile rows of A and tile columns of B; a fifth
+    problem reads a block of A transposed) are prepared once per operand and
+    still give, bit for bit, what dg_ozaki_dgemm gives each problem alone;
+    a graph replay reproduces them."""
+    n, h = 8192, 4096
+    A = torch.empty(n * n, dtype=torch.float64, device=dev)
+    B = torch.empty(n * n, dtype=torch.float64, device=dev)
+    dgamd.generate_t(701, dgamd.DG_DIST_WIDE, A)
+    dgamd.generate_t(702, dgamd.DG_DIST_WIDE, B)
+    outs, descs, args = [], [], []
+    for i in range(2):
+        for j in range(2):
+            C = torch.empty(h * h, dtype=torch.float64, device=dev)
+            dgamd.generate_t(710 + 2 * i + j, dgamd.DG_DIST_UNIFORM_PM1, C)
+            a = ("N", "N", h, h, n, 1.0, A[h * i:], n, B[h * j * n:], n, 0.5, C, h)
+            outs.append(C)
+            args.append(a)
+    # op(A) = A(0:4096, 0:4096)^T, op(B) = the first tile column of B: not the same lines as any tile's
+    C5 = torch.empty(h * h, dtype=torch.float64, device=dev)
+    dgamd.generate_t(720, dgamd.DG_DIST_UNIFORM_PM1, C5)
+    args.append(("T", "N", h, h, h, -2.0, A, n, B, n, 0.25, C5, h))
+    outs.append(C5)
+    descs = [dgamd.gemm_desc(*a) for a in args]
+    starts = [o.clone() for o in outs]
+    plan = dgamd.GroupedPlan(descs, dgamd.DG_PLAN_AUTOTUNE, form=dgamd.DG_FORM_OZAKI_INT8)
+    plan.execute()
+    torch.cuda.synchronize()
+    forms = [plan.problem_form(i) for i in range(len(descs))]
+    assert all(f == dgamd.DG_FORM_OZAKI_INT8 for f, _ in forms[:4]), forms
+    got = [o.clone() for o in outs]
+    for t, (a, (f, mod)) in enumerate(zip(args, forms)):
+        ref = starts[t].clone()
+        opA, opB, m_, n_, k_, alpha, Aa, lda, Bb, ldb, beta, _, ldc = a
+        if f == dgamd.DG_FORM_OZAKI_INT8:
+            dgamd.ozaki_dgemm_t(opA, opB, m_, n_, k_, alpha, Aa, lda, Bb, ldb, beta, ref, ldc, mod,
+                                kind=dgamd.DG_OZAKI_INT8_CRT)
+        else:
+            dgamd.dgemm_t(opA, opB, m_, n_, k_, alpha, Aa, lda, Bb, ldb, beta, ref, ldc)
+        torch.cuda.synchronize()
+        assert torch.equal(got[t], ref), (t, f, mod)
+    for o, s0 in zip(outs, starts):
+        o.copy_(s0)
+    torch.cuda.synchronize()
+    plan.graph_launch()
+    torch.cuda.synchronize()
+    for o, g in zip(outs, got):
+        assert torch.equal(o, g)
+    plan.close()
