@@ -17,7 +17,9 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "liboracle.so")
+# ORACLE_LIB selects another build of the same C source (the sanitizer build,
+# `make -C oracle sanitize`); the default is the plain liboracle.so.
+LIB_PATH = os.environ.get("ORACLE_LIB") or os.path.join(HERE, "liboracle.so")
 REF_PATH = os.path.join(HERE, "_ref", "libref_validation.so")
 
 _i64 = ctypes.c_int64
