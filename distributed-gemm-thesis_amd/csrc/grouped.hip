@@ -194,10 +194,10 @@ int plan_launch(dg_grouped_plan* p, int variant, int cfg, hipStream_t st, bool w
 }
 
 // Finds the operands that several Ozaki problems read (same lines: pointer,
-// line count, k, strides, moduli) and, when any repeats and the prepared
-// copies fit in half of the free device memory, allocates one buffer for
-// all of them plus one scan and one digit buffer; otherwise leaves the
-// per-problem path in place.  The products are bit for bit those of
+// line count, k, strides, moduli) and, when any repeats, no emulated
+// problem's C overlaps an operand and the prepared copies fit in half of the
+// free device memory, allocates one buffer for all of them plus one scan and
+// one digit buffer; otherwise leaves the per-problem path in place.  The products are bit for bit those of
 // dg_ozaki_dgemm: a prepared operand depends only on its own lines.
 int share_operands(dg_grouped_plan* p) {
     p->oz_ops.clear();
@@ -225,10 +225,24 @@ int share_operands(dg_grouped_plan* p) {
         p->oz_b.push_back(add(d.B, d.n, d.k, ob == 'N' ? d.ldb : 1, ob == 'N' ? 1 : d.ldb, nm));
         digits = std::max(digits, crt_digit_bytes(d.m, d.n, nm));
     }
+    // Preparing every operand before the first product is only the order of
+    // the per-problem path when no emulated problem writes memory that an
+    // operand reads (a batch whose outputs feed later problems keeps that path).
+    bool disjoint = true;
+    for (int i : p->oz_idx) {
+        const dg_gemm_desc& d = p->descs[i];
+        const uintptr_t c0 = reinterpret_cast<uintptr_t>(d.C);
+        const uintptr_t c1 = c0 + 8 * (uintptr_t)((d.n - 1) * d.ldc + d.m);
+        for (const auto& o : p->oz_ops) {
+            const uintptr_t x0 = reinterpret_cast<uintptr_t>(o.x);
+            const uintptr_t x1 = x0 + 8 * (uintptr_t)((o.rows - 1) * o.rs + (o.k - 1) * o.ks + 1);
+            if (c0 < x1 && x0 < c1) disjoint = false;
+        }
+    }
     size_t free_b = 0, total_b = 0;
     const int64_t bytes = off + (scan + 255) / 256 * 256 + digits;
     const bool repeats = p->oz_ops.size() < 2 * p->oz_idx.size();
-    if (!repeats || hipMemGetInfo(&free_b, &total_b) != hipSuccess || bytes > (int64_t)(free_b / 2) ||
+    if (!repeats || !disjoint || hipMemGetInfo(&free_b, &total_b) != hipSuccess || bytes > (int64_t)(free_b / 2) ||
         hipMalloc(&p->oz_shared, (size_t)bytes) != hipSuccess) {
         (void)hipGetLastError();  // a refused allocation is not an error here
         p->oz_shared = nullptr;

@@ -149,19 +149,32 @@ __device__ __forceinline__ void wait_step(int64_t ahead) {
 typedef __attribute__((address_space(3))) void lds_void_t;
 typedef __attribute__((address_space(1))) const void gvoid_t;
 
+// A pointer the caller knows to be equal across the wave, in scalar
+// registers (a uniform base plus a 32-bit lane offset lets a load take the
+// scalar-base address form instead of a 64-bit address per lane).
+__device__ __forceinline__ const int8_t* wave_uniform(const int8_t* p) {
+    const uint64_t v = reinterpret_cast<uint64_t>(p);
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return reinterpret_cast<const int8_t*>(((uint64_t)hi << 32) | lo);
+}
+
 // Issues one k-step of both operands into stage `stage`: this wave's A rows
 // 16*LA*w .. +16*LA and B rows 16*LB*w .. +16*LB, 16 rows per load; pa/pb
-// point at this lane's first row and chunk of the step, rows `rb` bytes apart.
+// (wave-uniform) point at the wave's first row of the step, loff is the
+// lane's offset to its row and chunk, rows `rb` bytes apart.
 __device__ __forceinline__ void issue_step(int8_t* lds, int stage, int wave, const int8_t* pa, const int8_t* pb,
-                                           int64_t rb) {
+                                           int64_t rb, uint32_t loff) {
     int8_t* sa = lds + stage * SSZ + wave * (1024 * LA);
     int8_t* sb = lds + stage * SSZ + BM * CK + wave * (1024 * LB);
 #pragma unroll
     for (int i = 0; i < LA; ++i)
-        __builtin_amdgcn_global_load_lds((gvoid_t*)(pa + 16 * i * rb), (lds_void_t*)(sa + 1024 * i), 16, 0, 0);
+        __builtin_amdgcn_global_load_lds((gvoid_t*)(wave_uniform(pa + 16 * i * rb) + loff),
+                                         (lds_void_t*)(sa + 1024 * i), 16, 0, 0);
 #pragma unroll
     for (int i = 0; i < LB; ++i)
-        __builtin_amdgcn_global_load_lds((gvoid_t*)(pb + 16 * i * rb), (lds_void_t*)(sb + 1024 * i), 16, 0, 0);
+        __builtin_amdgcn_global_load_lds((gvoid_t*)(wave_uniform(pb + 16 * i * rb) + loff),
+                                         (lds_void_t*)(sb + 1024 * i), 16, 0, 0);
 }
 
 // Fragments of one step: fa (A) and fb (B) from this lane's place in the
@@ -209,6 +222,41 @@ __device__ __forceinline__ void stage_mma(iv4 (&acci)[FM][FN], fv4 (&accf)[FM][F
     for (int i = 0; i < FM; ++i)
 #pragma unroll
         for (int j = 0; j < FN; ++j) mma<BF16>(acci[i][j], accf[i][j], fb[j], fa[i]);
+}
+
+// stage_mma that also issues this wave's loads of one later k-step into
+// stage `stage`: A rows from pa, B rows from pb (wave-uniform, lane offset
+// loff, rows rb bytes apart), one global_load_lds after every two groups of
+// FN MFMAs instead of four back to back before them.
+template <bool BF16>
+__device__ __forceinline__ void stage_mma_ld(iv4 (&acci)[FM][FN], fv4 (&accf)[FM][FN], const int8_t* as,
+                                             const int8_t* bs, int8_t* lds, int stage, int wave, const int8_t* pa,
+                                             const int8_t* pb, uint32_t loff, int64_t rb) {
+    static_assert(LA + LB == FM / 2, "one load after every two MFMA groups");
+    iv4 fa[FM], fb[FN];
+    read_frags(fa, fb, as, bs);
+    int8_t* sa = lds + stage * SSZ + wave * (1024 * LA);
+    int8_t* sb = lds + stage * SSZ + BM * CK + wave * (1024 * LB);
+#pragma unroll
+    for (int i = 0; i < FM; ++i) {
+#pragma unroll
+        for (int j = 0; j < FN; ++j) mma<BF16>(acci[i][j], accf[i][j], fb[j], fa[i]);
+        if (i & 1) {
+            const int l = i >> 1;
+            if (l < LA)
+                __builtin_amdgcn_global_load_lds((gvoid_t*)(wave_uniform(pa + 16 * l * rb) + loff),
+                                                 (lds_void_t*)(sa + 1024 * l), 16, 0, 0);
+            else
+                __builtin_amdgcn_global_load_lds((gvoid_t*)(wave_uniform(pb + 16 * (l - LA) * rb) + loff),
+                                                 (lds_void_t*)(sb + 1024 * (l - LA)), 16, 0, 0);
+        }
+    }
+    __builtin_amdgcn_sched_group_barrier(0x100, FM + FN, 0);  // every fragment read ahead of the MFMAs
+#pragma unroll
+    for (int g = 0; g < FM / 2; ++g) {
+        __builtin_amdgcn_sched_group_barrier(0x008, 2 * FN, 0);
+        __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);
+    }
 }
 
 // Output tile of workgroup slot `bid` among `tiles`: XCD-aware, GROUP_M tile
@@ -291,23 +339,26 @@ __global__ __launch_bounds__(CNT, OCC) void oz_gemm_kernel(OzArgs g) {
     const int64_t rb = g.Kp * esz;  // bytes per slice row
     const int lrow = lane >> 2;
     const int lchunk = (lane & 3) ^ swz(lrow);
-    const int8_t* ga = g.a + (m0 + 16 * LA * wave + lrow) * rb + 16 * lchunk;
-    const int8_t* gb = g.b + (n0 + 16 * LB * wave + lrow) * rb + 16 * lchunk;
+    const int8_t* ga = g.a + (m0 + 16 * LA * wave) * rb;  // wave-uniform; lanes add loff
+    const int8_t* gb = g.b + (n0 + 16 * LB * wave) * rb;
+    const uint32_t loff = (uint32_t)(lrow * rb + 16 * lchunk);
     const int roff = lr * CK + 16 * (lg ^ swz(lr));
 
-    // issue cursor: product iseg, byte offset ikb within it
+    // issue cursor: product iseg, byte offset ikb within it.  Past the last
+    // product it starts the last product over: those reloads land in stages
+    // that are free and never read again, so that every step issues one
+    // step of loads and every wait is the same.
     int iseg = 0;
     int64_t ikb = 0;
     OzSeg is = g.segs[0];
-    int ist = 0;  // stage of the next issue
-    auto issue_next = [&]() {
+    auto next_issue = [&](const int8_t*& pa, const int8_t*& pb) {
         const int64_t koff = (int64_t)is.k0 * esz + ikb;
-        issue_step(lds, ist, wave, ga + is.t * g.a_slice + koff, gb + is.u * g.b_slice + koff, rb);
-        ist = ist + 1 == STG ? 0 : ist + 1;
+        pa = ga + is.t * g.a_slice + koff;
+        pb = gb + is.u * g.b_slice + koff;
         ikb += CK;
         if (ikb >= (int64_t)is.klen * esz) {
             ikb = 0;
-            if (++iseg < g.nseg) is = g.segs[iseg];
+            if (iseg + 1 < g.nseg) is = g.segs[++iseg];
         }
     };
 
@@ -322,10 +373,14 @@ __global__ __launch_bounds__(CNT, OCC) void oz_gemm_kernel(OzArgs g) {
         }
 
     // Scheme I keeps the barrier at the start of each step: wait for this
-    // wave's loads of step k, publish it, refill the stage of step k-1 with
-    // step k+STG-1, read every fragment, then the MFMAs.
-    const int64_t nsteps = g.nsteps;
-    for (int64_t s = 0; s < STG - 1 && s < nsteps; ++s) issue_next();
+    // wave's loads of step k, publish it, read every fragment, then the
+    // MFMAs, among which the stage of step k-1 is refilled with step k+STG-1.
+    for (int st0 = 0; st0 < STG - 1; ++st0) {
+        const int8_t *pa, *pb;
+        next_issue(pa, pb);
+        issue_step(lds, st0, wave, pa, pb, rb, loff);
+    }
+    int ist = STG - 1;  // stage of the next issue
     const int64_t mrow = m0 + wm + lr;
     const int64_t ncol = n0 + wn + 4 * lg;
     const bool full = (m0 + BM <= g.m) && (n0 + BN <= g.n);
@@ -335,12 +390,14 @@ __global__ __launch_bounds__(CNT, OCC) void oz_gemm_kernel(OzArgs g) {
         const OzSeg cs = g.segs[cseg];
         const int64_t kend = kt + (int64_t)cs.klen * esz / CK;
         for (; kt < kend; ++kt) {
-            const int64_t rem = nsteps - 1 - kt;
-            wait_step(rem < STG - 2 ? rem : STG - 2);
+            wait_vmcnt<(STG - 2) * LPS>();  // step kt landed; the STG-2 later steps may be in flight
             __builtin_amdgcn_s_barrier();
-            if (kt + STG - 1 < nsteps) issue_next();
+            const int8_t *pa, *pb;
+            next_issue(pa, pb);
             const int8_t* st = lds + cst * SSZ;
-            stage_mma<BF16>(acci, accf, st + wm * CK + roff, st + BM * CK + wn * CK + roff);
+            stage_mma_ld<BF16>(acci, accf, st + wm * CK + roff, st + BM * CK + wn * CK + roff, lds, ist, wave, pa,
+                               pb, loff, rb);
+            ist = ist + 1 == STG ? 0 : ist + 1;
             cst = cst + 1 == STG ? 0 : cst + 1;
         }
         if (cs.flush) {
@@ -354,6 +411,7 @@ __global__ __launch_bounds__(CNT, OCC) void oz_gemm_kernel(OzArgs g) {
                 }
         }
     }
+    wait_vmcnt<0>();  // the reloads past the end land before the workgroup's LDS is released
 }
 
 // ------------------------------------------------------------ Ozaki scheme II
@@ -550,13 +608,6 @@ __device__ __forceinline__ int next_slot(int x) { return x + 1 == NSLOT ? 0 : x 
 // across the wave) points at the wave's first row, loff is the lane's
 // offset to its row and chunk, rows `ld` bytes apart.  A uniform base and a
 // 32-bit lane offset let every load take the scalar-base address form.
-__device__ __forceinline__ const int8_t* wave_uniform(const int8_t* p) {
-    const uint64_t v = reinterpret_cast<uint64_t>(p);
-    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
-    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
-    return reinterpret_cast<const int8_t*>(((uint64_t)hi << 32) | lo);
-}
-
 __device__ __forceinline__ void crt_issue(int8_t* lds, int slot, int wave, const int8_t* g, uint32_t loff, int64_t s,
                                           int64_t ld) {
     int8_t* d = lds + slot * CSLOT + wave * 4096;

@@ -225,7 +225,8 @@ typedef struct dg_gemm_desc {
 
 /* Every GEMM of `descs` in one launch per distinct (opA, opB) pair: the
  * tiles of all problems are dealt from one tile space, largest K first.
- * Output ranges must not overlap.
+ * Output ranges must not overlap, and no problem's output may be another
+ * problem's input: the problems run in no specified order.
  * Replaces the per-rank local GEMMs the reference runs one rank at a time
  * on BlockSequentialDecomposer tiles (PARALiADgemm per rank,
  * src/distributedParalia.cpp:57-64, 159-167) and the per-step cublasDgemm
