@@ -73,6 +73,8 @@ SIGNATURES = {
     "dg_unpack_block2d": (_int, [_ptr, _i64p, _int, _i64p, _int, _ptr, _i64, _i64, _i64, _ptr]),
     "dg_copy_block": (_int, [_i64, _i64, _ptr, _i64, _ptr, _i64, _ptr]),
     "dg_copy_block_f32": (_int, [_i64, _i64, _ptr, _i64, _ptr, _i64, _ptr]),
+    "dg_copy2d_h2d": (_int, [_ptr, _i64, _ptr, _i64, _i64, _i64, _ptr]),
+    "dg_copy2d_d2h": (_int, [_ptr, _i64, _ptr, _i64, _i64, _i64, _ptr]),
     "dg_numroc": (_i64, [_i64, _i64, _int, _int, _int]),
     "dg_block_cyclic_size": (_i64, [_i64, _i64, _i64, _i64, _int, _int]),
     "dg_pack_block_cyclic": (_int, [_ptr, _i64, _i64, _i64, _i64, _i64, _int, _int, _ptr, _ptr]),
@@ -227,6 +229,26 @@ def copy_block_t(rows, cols, src, lds, dst, ldd):
         check("dg_copy_block_f32", lib().dg_copy_block_f32(rows, cols, _dp(src), lds, _dp(dst), ldd, _stream()))
     else:
         check("dg_copy_block", lib().dg_copy_block(rows, cols, _dp(src), lds, _dp(dst), ldd, _stream()))
+
+
+def _host_ptr(a):
+    """Address of a host buffer: a numpy array or a CPU torch tensor (pinned or not)."""
+    import numpy as np
+    if isinstance(a, np.ndarray):
+        return a.ctypes.data
+    return a.data_ptr()
+
+
+def copy2d_h2d(dev, dev_pitch, host, host_pitch, width, height):
+    """Host-to-device copy of `height` columns of `width` bytes (pitches in bytes)."""
+    check("dg_copy2d_h2d", lib().dg_copy2d_h2d(_dp(dev), dev_pitch, _host_ptr(host), host_pitch, width, height,
+                                                _stream()))
+
+
+def copy2d_d2h(host, host_pitch, dev, dev_pitch, width, height):
+    """Device-to-host copy of `height` columns of `width` bytes (pitches in bytes)."""
+    check("dg_copy2d_d2h", lib().dg_copy2d_d2h(_host_ptr(host), host_pitch, _dp(dev), dev_pitch, width, height,
+                                                _stream()))
 
 
 def pack_block_cyclic_t(G, ldg, rows, cols, mb, nb, P, Q, packed):

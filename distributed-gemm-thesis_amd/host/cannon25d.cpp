@@ -114,7 +114,8 @@ void Summa25Decomposer::decompose2D(int senderRank, double* A, double* B, double
         const bool host = !isDevicePointer(G);
         if (host) {
             HIP_CHECK(hipMalloc(&d, sizeof(double) * (size_t)(rows * cols > 0 ? rows * cols : 1)));
-            HIP_CHECK(hipMemcpy(d, G, sizeof(double) * rows * cols, hipMemcpyHostToDevice));
+            const int64_t w = (int64_t)sizeof(double) * rows;
+            DG_CHECK(dg_copy2d_h2d(d, w, G, w, w, cols, stream));
         }
         DG_CHECK(dg_pack_block2d(d, rows, rows, cols, ro.data(), p, co.data(), p, out, stream));
         HIP_CHECK(hipStreamSynchronize(stream));
@@ -140,7 +141,8 @@ void Summa25Decomposer::reorderMatrix(int gatherRank, double* C, double* cTile) 
     DG_CHECK(dg_unpack_block2d(cTile, ro.data(), p, co.data(), p, d, M, M, N, stream));
     HIP_CHECK(hipStreamSynchronize(stream));
     if (host) {
-        HIP_CHECK(hipMemcpy(C, d, sizeof(double) * M * N, hipMemcpyDeviceToHost));
+        const int64_t w = (int64_t)sizeof(double) * M;
+        DG_CHECK(dg_copy2d_d2h(C, w, d, w, w, N, stream));
         HIP_CHECK(hipFree(d));
     }
 }

@@ -21,12 +21,12 @@ void with_device_matrix(double* p, long long ld, long long cols, bool copy_in, b
         return;
     }
     double* d = nullptr;
-    const size_t bytes = sizeof(double) * (size_t)ld * (size_t)(cols > 0 ? cols : 1);
-    HIP_CHECK(hipMalloc(&d, bytes));
-    if (copy_in) HIP_CHECK(hipMemcpy(d, p, bytes, hipMemcpyHostToDevice));
+    const int64_t w = (int64_t)sizeof(double) * ld;
+    HIP_CHECK(hipMalloc(&d, (size_t)w * (size_t)(cols > 0 ? cols : 1)));
+    if (copy_in) DG_CHECK(dg_copy2d_h2d(d, w, p, w, w, cols, nullptr));
     fn(d);
     HIP_CHECK(hipDeviceSynchronize());
-    if (copy_out) HIP_CHECK(hipMemcpy(p, d, bytes, hipMemcpyDeviceToHost));
+    if (copy_out) DG_CHECK(dg_copy2d_d2h(p, w, d, w, w, cols, nullptr));
     HIP_CHECK(hipFree(d));
 }
 

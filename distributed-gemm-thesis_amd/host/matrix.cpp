@@ -141,7 +141,10 @@ hipMemcpyKind findMemcpyKind(const void* src, const void* dst) {
 template <typename T>
 void copy(size_t size, const T* src, T* dst, hipMemcpyKind dir) {
     static_assert(std::is_trivially_copyable<T>::value, "elements must be trivially copyable");
-    HIP_CHECK(hipMemcpy(dst, src, sizeof(T) * size, dir));
+    const int64_t bytes = (int64_t)(sizeof(T) * size);
+    if (dir == hipMemcpyHostToDevice) DG_CHECK(dg_copy2d_h2d(dst, bytes, src, bytes, bytes, 1, nullptr));
+    else if (dir == hipMemcpyDeviceToHost) DG_CHECK(dg_copy2d_d2h(dst, bytes, src, bytes, bytes, 1, nullptr));
+    else HIP_CHECK(hipMemcpy(dst, src, sizeof(T) * size, dir));
 }
 
 void memcpy2D(void* src, long long lds, void* dst, long long ldd, long long rows, long long columns,
@@ -152,8 +155,11 @@ void memcpy2D(void* src, long long lds, void* dst, long long ldd, long long rows
     }
     if (!columnMajor) std::swap(rows, columns);
     if (rows <= 0 || columns <= 0) return;
-    HIP_CHECK(hipMemcpy2D(dst, ldd * elementSize, src, lds * elementSize, rows * elementSize, columns,
-                          findMemcpyKind(src, dst)));
+    const int64_t w = rows * (int64_t)elementSize, sp = lds * (int64_t)elementSize, dp = ldd * (int64_t)elementSize;
+    const hipMemcpyKind dir = findMemcpyKind(src, dst);
+    if (dir == hipMemcpyHostToDevice) DG_CHECK(dg_copy2d_h2d(dst, dp, src, sp, w, columns, nullptr));
+    else if (dir == hipMemcpyDeviceToHost) DG_CHECK(dg_copy2d_d2h(dst, dp, src, sp, w, columns, nullptr));
+    else HIP_CHECK(hipMemcpy2D(dst, dp, src, sp, w, columns, dir));
 }
 
 template <typename T>
@@ -232,7 +238,8 @@ void generateMatrixGPU(T* array, const long long size, const int deviceID) {
         DG_CHECK(dg_generate_f32(seed, dist, target, size, nullptr));
     }
     if (target != array) {
-        HIP_CHECK(hipMemcpy(array, target, sizeof(T) * size, hipMemcpyDeviceToHost));
+        const int64_t bytes = (int64_t)(sizeof(T) * size);
+        DG_CHECK(dg_copy2d_d2h(array, bytes, target, bytes, bytes, 1, nullptr));
         HIP_CHECK(hipFree(target));
     }
     HIP_CHECK(hipDeviceSynchronize());

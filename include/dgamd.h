@@ -127,6 +127,27 @@ int dg_copy_block(int64_t rows, int64_t cols, const double* src, int64_t lds,
 int dg_copy_block_f32(int64_t rows, int64_t cols, const float* src, int64_t lds,
                       float* dst, int64_t ldd, void* stream);
 
+/* Host to device: `height` columns of `width` bytes, column j at
+ * host + j*host_pitch, to dev + j*dev_pitch (pitches in bytes, >= width).
+ * Pinned host memory is copied by DMA directly.  Pageable host memory goes
+ * through a process-wide ring of three 32 MiB pinned buffers (allocated at
+ * the first such copy, kept until the process ends): up to eight host
+ * threads fill one buffer while the DMA engine moves the previous one, and
+ * such copies run one at a time per process.  The copy follows earlier work
+ * on `stream` and is complete when the call returns; it cannot be captured
+ * into a graph.
+ * Replaces: the host-to-device branches of copy, copyBlock and memcpy2D,
+ *           src/Transfers/sharedTransfers.cpp:22-81, as decompose2D uses
+ *           them on host matrices, src/25DCannon.cpp:87-149. */
+int dg_copy2d_h2d(void* dev, int64_t dev_pitch, const void* host, int64_t host_pitch,
+                  int64_t width, int64_t height, void* stream);
+
+/* Device to host, the same way; returns once `host` holds the data.
+ * Replaces: the device-to-host branches of the same functions, as
+ *           reorderMatrix gathers C into host memory, src/25DCannon.cpp:231-284. */
+int dg_copy2d_d2h(void* host, int64_t host_pitch, const void* dev, int64_t dev_pitch,
+                  int64_t width, int64_t height, void* stream);
+
 /* ------------------------------------------------- block-cyclic layout -- */
 
 /* ScaLAPACK NUMROC: rows (or columns) of an n-long dimension split in nb

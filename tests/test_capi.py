@@ -110,6 +110,17 @@ def test_argument_errors_need_no_device():
     assert st == dgamd.DG_ERR_INVALID and "moduli" in dgamd.last_error()
     mod = ctypes.c_int(5)
     assert L.dg_grouped_plan_problem_form(None, 0, ctypes.byref(mod)) == -1 and mod.value == 0
+    # host <-> device copies: sizes, pitches and pointers are checked first
+    buf = (ctypes.c_char * 64)()
+    hb = ctypes.cast(buf, ctypes.c_void_p)
+    for fn in (L.dg_copy2d_h2d, L.dg_copy2d_d2h):
+        assert fn(hb, 8, hb, 8, -1, 2, None) == dgamd.DG_ERR_INVALID and "negative" in dgamd.last_error()
+        assert fn(hb, 8, hb, 16, 16, 2, None) == dgamd.DG_ERR_INVALID and "pitch" in dgamd.last_error()
+        assert fn(hb, 16, hb, 8, 16, 2, None) == dgamd.DG_ERR_INVALID and "pitch" in dgamd.last_error()
+        assert fn(None, 8, hb, 8, 8, 2, None) == dgamd.DG_ERR_INVALID and "null" in dgamd.last_error()
+        assert fn(hb, 1 << 62, hb, 1 << 62, 8, 3, None) == dgamd.DG_ERR_INVALID and "64 bits" in dgamd.last_error()
+        assert fn(None, 0, None, 0, 0, 5, None) == dgamd.DG_OK   # empty copies are a no-op
+        assert fn(None, 8, None, 8, 8, 0, None) == dgamd.DG_OK
 
 
 def test_device_calls_fail_loudly_without_gpu():
@@ -122,3 +133,6 @@ def test_device_calls_fail_loudly_without_gpu():
     buf = (ctypes.c_double * 4)()
     st = L.dg_generate(1, 0, ctypes.cast(buf, ctypes.c_void_p), 4, None)
     assert st == dgamd.DG_ERR_NO_DEVICE
+    hb = ctypes.cast(buf, ctypes.c_void_p)
+    assert L.dg_copy2d_h2d(hb, 8, hb, 8, 8, 4, None) == dgamd.DG_ERR_NO_DEVICE
+    assert L.dg_copy2d_d2h(hb, 8, hb, 8, 8, 4, None) == dgamd.DG_ERR_NO_DEVICE
