@@ -42,6 +42,7 @@ typedef int iv4 __attribute__((ext_vector_type(4)));
 typedef float fv4 __attribute__((ext_vector_type(4)));
 typedef double dv4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
 
 constexpr int PAD = 256;     // row padding of the slices and residues (a multiple of BM and BN)
 constexpr int KSB = 128;     // K padding granule of the slices and residues, bytes
@@ -505,7 +506,8 @@ __global__ __launch_bounds__(256) void crt_residue_kernel(const double* __restri
     const int t = threadIdx.x;
     // Eight loads per thread, all issued before the first wait: addresses
     // outside op(X) are clamped onto its last row / k (in bounds) and the
-    // loaded value replaced by zero.
+    // loaded value replaced by zero.  op(X) is read once here and the
+    // residues written once, so both go nontemporal.
     if (ks == 1) {
         const int64_t k = k0 + (t & 127);
         const int64_t kc = k < K ? k : K - 1;
@@ -513,7 +515,7 @@ __global__ __launch_bounds__(256) void crt_residue_kernel(const double* __restri
 #pragma unroll
         for (int i = 0; i < RT_R / 2; ++i) {
             const int64_t r = r0 + (t >> 7) + 2 * i;
-            v[i] = x[(r < rows ? r : rows - 1) * rs + kc];
+            v[i] = __builtin_nontemporal_load(x + (r < rows ? r : rows - 1) * rs + kc);
         }
 #pragma unroll
         for (int i = 0; i < RT_R / 2; ++i) {
@@ -527,7 +529,7 @@ __global__ __launch_bounds__(256) void crt_residue_kernel(const double* __restri
 #pragma unroll
         for (int i = 0; i < RT_K * RT_R / 256; ++i) {
             const int64_t k = k0 + t / RT_R + (256 / RT_R) * i;
-            v[i] = x[rc * rs + (k < K ? k : K - 1) * ks];
+            v[i] = __builtin_nontemporal_load(x + rc * rs + (k < K ? k : K - 1) * ks);
         }
 #pragma unroll
         for (int i = 0; i < RT_K * RT_R / 256; ++i) {
@@ -568,7 +570,7 @@ __global__ __launch_bounds__(256) void crt_residue_kernel(const double* __restri
                                                   __builtin_amdgcn_perm(v[1], v[0], 0x0c0c0400u), 0x05040100u);
         const uint32_t w1 = __builtin_amdgcn_perm(__builtin_amdgcn_perm(v[7], v[6], 0x0c0c0400u),
                                                   __builtin_amdgcn_perm(v[5], v[4], 0x0c0c0400u), 0x05040100u);
-        *reinterpret_cast<uint2*>(dst + m * slice_bytes) = uint2{w0, w1};
+        __builtin_nontemporal_store(u32x2{w0, w1}, reinterpret_cast<u32x2*>(dst + m * slice_bytes));
     }
 }
 
@@ -863,7 +865,8 @@ __global__ __launch_bounds__(256) void crt_reconstruct_kernel(const uint8_t* __r
         const uint8_t* rp = res + j * Mp + i0;
         uint32_t w[NMOD];
 #pragma unroll
-        for (int t = 0; t < NMOD; ++t) w[t] = *reinterpret_cast<const uint32_t*>(rp + (int64_t)t * plane);
+        for (int t = 0; t < NMOD; ++t)  // digits and C are touched once: nontemporal accesses
+            w[t] = __builtin_nontemporal_load(reinterpret_cast<const uint32_t*>(rp + (int64_t)t * plane));
         double L[4][3];
 #pragma unroll
         for (int e = 0; e < 4; ++e)
@@ -889,11 +892,11 @@ __global__ __launch_bounds__(256) void crt_reconstruct_kernel(const uint8_t* __r
         if (i0 + 3 < m && (reinterpret_cast<uintptr_t>(cp) & 31) == 0) {
             dv4* c4 = reinterpret_cast<dv4*>(cp);
             if (beta != 0.0) {
-                const dv4 old = *c4;
+                const dv4 old = __builtin_nontemporal_load(c4);
 #pragma unroll
                 for (int e = 0; e < 4; ++e) val[e] = __dadd_rn(val[e], __dmul_rn(beta, old[e]));
             }
-            *c4 = dv4{val[0], val[1], val[2], val[3]};
+            __builtin_nontemporal_store(dv4{val[0], val[1], val[2], val[3]}, c4);
         } else {
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
@@ -1040,8 +1043,8 @@ __device__ __forceinline__ void scan_contiguous(const double* __restrict__ x, in
         const double* xr = x + r * rs;
         double v[4];
         if (k0 + CRT_KC <= K) {  // the whole chunk inside the line: both loads in flight together
-            const pdv2 w0 = *reinterpret_cast<const pdv2*>(xr + k0 + 2 * l16);
-            const pdv2 w1 = *reinterpret_cast<const pdv2*>(xr + k0 + 32 + 2 * l16);
+            const pdv2 w0 = __builtin_nontemporal_load(reinterpret_cast<const pdv2*>(xr + k0 + 2 * l16));
+            const pdv2 w1 = __builtin_nontemporal_load(reinterpret_cast<const pdv2*>(xr + k0 + 32 + 2 * l16));
             v[0] = w0.x; v[1] = w0.y; v[2] = w1.x; v[3] = w1.y;
         } else {
 #pragma unroll
@@ -1102,7 +1105,7 @@ __global__ __launch_bounds__(256) void crt_scan_lines_kernel(const double* __res
     // of the line), so that all of them issue before the first wait
     double raw[CRT_KC];
 #pragma unroll
-    for (int i = 0; i < CRT_KC; ++i) raw[i] = xr[(i < kn ? i : kn - 1) * ks];
+    for (int i = 0; i < CRT_KC; ++i) raw[i] = __builtin_nontemporal_load(xr + (i < kn ? i : kn - 1) * ks);
     uint32_t wd[CRT_KC];
     uint32_t top = 0;
 #pragma unroll
