@@ -20,14 +20,16 @@
 //      so that the CPU oracle reproduces every bit.
 //
 // Slices are stored k-contiguous, [t][row][Kp] with rows padded to 256 and K
-// padded to 128 bytes with zeros, so the product kernel has no edge cases
-// until the final store.  The scheme I product kernel uses 256x256 output
-// tiles, 8 waves of 128x64 (128 accumulator registers per lane) and a 5-stage
-// LDS ring of 64-byte k-steps filled by direct global-to-LDS loads (scheme II
-// moves 128-byte k-steps, described with its kernel).  The FP64 partial sums
-// W live in HBM (workspace): a flush is one coalesced read-modify-write of
-// the wave's 128x64 block, a handful per tile, so the registers hold only the
-// 32-bit accumulators.
+// padded to 128 bytes with zeros, so the product kernels have no edge cases
+// until the final store.  INT8 slices run each run of products between two
+// flushes (a group) as one long-K product through the scheme II pipeline
+// (oz_plane_kernel: 128-byte k-steps, described with crt_gemm_kernel) into a
+// 32-bit plane per group, and oz_combine_kernel adds the planes into the FP64
+// partial sums W (workspace) in schedule order.  BF16 slices run
+// oz_gemm_kernel: 256x256 output tiles, 8 waves of 128x64 (128 accumulator
+// registers per lane) and a 5-stage LDS ring of 64-byte k-steps filled by
+// direct global-to-LDS loads, flushing into W inside the product loop (one
+// coalesced read-modify-write of the wave's 128x64 block per flush).
 #include "dg_internal.hpp"
 
 // Every FP64 operation of the recombination is rounded on its own (no fused
@@ -619,6 +621,11 @@ __device__ __forceinline__ void crt_issue(int8_t* lds, int slot, int wave, const
                                          (lds_void_t*)(d + 1024 * i), 16, 0, 0);
 }
 
+// One 64-byte MFMA of the scheme II pipeline: 16x16 INT8 products summed in INT32.
+__device__ __forceinline__ void mfma64(iv4& acc, const iv4& a, const iv4& b) {
+    acc = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b, acc, 0, 0, 0);
+}
+
 // One MFMA half-step on fa / cb, reading the fragments of the following
 // half-step meanwhile: fa[i] right after its last use, B into nb.
 __device__ __forceinline__ void crt_half(iv4 (&acc)[FM][FN], iv4 (&fa)[FM], const iv4 (&cb)[FN], iv4 (&nb)[FN],
@@ -626,7 +633,7 @@ __device__ __forceinline__ void crt_half(iv4 (&acc)[FM][FN], iv4 (&fa)[FM], cons
 #pragma unroll
     for (int i = 0; i < FM; ++i) {
 #pragma unroll
-        for (int j = 0; j < FN; ++j) acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa[i], cb[j], acc[i][j], 0, 0, 0);
+        for (int j = 0; j < FN; ++j) mfma64(acc[i][j], fa[i], cb[j]);
         if (i == 0) {
 #pragma unroll
             for (int j = 0; j < FN; ++j) nb[j] = *reinterpret_cast<const iv4*>(nbs + 2048 * j);
@@ -657,7 +664,7 @@ __device__ __forceinline__ void crt_half_ld(iv4 (&acc)[FM][FN], iv4 (&fa)[FM], c
 #pragma unroll
     for (int i = 0; i < FM; ++i) {
 #pragma unroll
-        for (int j = 0; j < FN; ++j) acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa[i], cb[j], acc[i][j], 0, 0, 0);
+        for (int j = 0; j < FN; ++j) mfma64(acc[i][j], fa[i], cb[j]);
         if (i == 0) {
 #pragma unroll
             for (int j = 0; j < FN; ++j) nb[j] = *reinterpret_cast<const iv4*>(nbs + 2048 * j);
@@ -789,6 +796,182 @@ __global__ __launch_bounds__(CNT, OCC) void crt_gemm_kernel(CrtArgs g) {
             }
             *reinterpret_cast<uint32_t*>(out + n * g.Mp + m) = w;
         }
+    }
+}
+
+// ------------------------------------- scheme I (INT8) on the scheme II pipeline
+// The products of scheme I between two flushes (a group: slices t0 + p of A
+// against slices u0 - p of B for p < count, one level, one k range) add up
+// to one long-K product, so a group's k-steps run back to back through the
+// scheme II pipeline above.  Workgroup b computes tile b % tiles of the
+// launch's group b / tiles and stores its INT32 sums as a plane, [Np][Mp]
+// column-major; oz_combine_kernel then adds the planes into FP64 in group
+// order.  BF16 slices stay on oz_gemm_kernel: built on this pipeline, the
+// BF16 products came out wrong in three of every four accumulator rows
+// while the INT8 ones, from the same code, were exact.  In that build about
+// a fifth of the BF16 MFMAs had a destination overlapping their A/B source
+// registers and a third renamed their accumulator, neither of which
+// oz_gemm_kernel's BF16 MFMAs do; the cause was not confirmed.
+struct OzGroup {
+    int32_t t0, u0;   // 0-based slices of the first product
+    int32_t count;    // products
+    int32_t level;    // t + u + 2 (t, u 1-based) of every product
+    int32_t kb;       // byte offset of the k range within a slice row
+    FastDiv nks;      // 128-byte k-steps per product (nks.d), as a divisor
+};
+
+struct OzPlaneArgs {
+    const int8_t* a;             // A slices, [s][Mp] rows of rb bytes
+    const int8_t* b;             // B slices, [s][Np] rows of rb bytes
+    int32_t* planes;             // one [Np][Mp] plane per group of the launch
+    const OzGroup* groups;
+    int32_t g0;                  // first group of the launch
+    int32_t tiles_m, tiles_n;
+    int64_t Mp, rb;
+    int64_t a_slice, b_slice;    // bytes per slice
+    int64_t plane;               // elements per plane
+};
+
+// This wave's rows of k-step s of a group (clamped to its last step, for
+// the pipeline's reloads past the end): product p = s / nks, step s % nks
+// within it, the product's slice `stride` bytes from the previous one's.
+// Branch-free scalar arithmetic, so the loop body stays one basic block.
+__device__ __forceinline__ const int8_t* group_step(const int8_t* first, int64_t stride, uint32_t s, uint32_t last,
+                                                    const FastDiv& nks) {
+    const uint32_t sc = s < last ? s : last;
+    const uint32_t p = fdiv(sc, nks);
+    return first + (int64_t)p * stride + (int64_t)(sc - p * nks.d) * CK2;
+}
+
+__global__ __launch_bounds__(CNT, OCC) void oz_plane_kernel(OzPlaneArgs g) {
+    extern __shared__ __attribute__((aligned(1024))) int8_t lds[];
+    const int tiles = g.tiles_m * g.tiles_n;
+    const int gi = blockIdx.x / tiles;
+    int64_t m0, n0;
+    tile_origin(blockIdx.x - gi * tiles, tiles, g.tiles_m, g.tiles_n, m0, n0);
+    const OzGroup gr = g.groups[g.g0 + gi];
+
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int wm = (wave / (BN / WN)) * WM;
+    const int wn = (wave % (BN / WN)) * WN;
+    const int lr = lane & 15;
+    const int lg = lane >> 4;
+    const int lrow = lane >> 3;
+    const int lchunk = (lane & 7) ^ lrow;
+    const uint32_t loff = (uint32_t)(lrow * g.rb + 16 * lchunk);
+    const int roff0 = lr * CK2 + 16 * (lg ^ (lr & 7));
+    const int roff1 = lr * CK2 + 16 * ((4 + lg) ^ (lr & 7));
+
+    const uint32_t last = (uint32_t)gr.count * gr.nks.d - 1;  // the group's last k-step
+    const int8_t* ga = g.a + gr.t0 * g.a_slice + (m0 + 32 * wave) * g.rb + gr.kb;
+    const int8_t* gb = g.b + gr.u0 * g.b_slice + (n0 + 32 * wave) * g.rb + gr.kb;
+    const auto a_at = [&](uint32_t s) { return group_step(ga, g.a_slice, s, last, gr.nks); };
+    const auto b_at = [&](uint32_t s) { return group_step(gb, -g.b_slice, s, last, gr.nks); };
+
+    iv4 acc[FM][FN];
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j) acc[i][j] = iv4{0, 0, 0, 0};
+
+    // prologue: A_0 B_0 A_1 B_1 A_2 into slots 0..4, as in crt_gemm_kernel
+    crt_issue(lds, 0, wave, a_at(0), loff, 0, g.rb);
+    crt_issue(lds, 1, wave, b_at(0), loff, 0, g.rb);
+    crt_issue(lds, 2, wave, a_at(1), loff, 0, g.rb);
+    crt_issue(lds, 3, wave, b_at(1), loff, 0, g.rb);
+    crt_issue(lds, 4, wave, a_at(2), loff, 0, g.rb);
+    int isl = 0;
+    wait_vmcnt<12>();
+    __builtin_amdgcn_s_barrier();
+    int csl = 0;
+    iv4 fa[FM], fb[FN], nb[FN];
+    {
+        const int8_t* as = lds + wm * CK2 + roff0;
+        const int8_t* bs = lds + CSLOT + wn * CK2 + roff0;
+#pragma unroll
+        for (int j = 0; j < FN; ++j) fb[j] = *reinterpret_cast<const iv4*>(bs + 2048 * j);
+#pragma unroll
+        for (int i = 0; i < FM; ++i) fa[i] = *reinterpret_cast<const iv4*>(as + 2048 * i);
+    }
+    for (uint32_t s = 0; s < last; ++s) {
+        const int bsl = next_slot(csl), asl1 = next_slot(bsl), bsl1 = next_slot(asl1);
+        crt_half(acc, fa, fb, nb, lds + csl * CSLOT + wm * CK2 + roff1, lds + bsl * CSLOT + wn * CK2 + roff1);
+        __builtin_amdgcn_s_waitcnt((0xF) | (0x3 << 14) | (0x7 << 4));  // lgkmcnt(0)
+        wait_vmcnt<4>();
+        __builtin_amdgcn_s_barrier();
+        const int sb = isl, sa = next_slot(isl);
+        isl = next_slot(sa);
+        crt_half_ld(acc, fa, nb, fb, lds + asl1 * CSLOT + wm * CK2 + roff0, lds + bsl1 * CSLOT + wn * CK2 + roff0,
+                    lds, sb, sa, wave, b_at(s + 2), a_at(s + 3), loff, 0, 0, g.rb);
+        csl = asl1;
+    }
+    {
+        const int bsl = next_slot(csl), asl1 = next_slot(bsl), bsl1 = next_slot(asl1);
+        crt_half(acc, fa, fb, nb, lds + csl * CSLOT + wm * CK2 + roff1, lds + bsl * CSLOT + wn * CK2 + roff1);
+        crt_half(acc, fa, nb, fb, lds + asl1 * CSLOT + wm * CK2 + roff0, lds + bsl1 * CSLOT + wn * CK2 + roff0);
+    }
+    wait_vmcnt<0>();
+    // acc[i][j][v] = sum at (m0 + wm + 16i + 4lg + v, n0 + wn + 16j + lr): one 16-byte store
+    int32_t* out = g.planes + (int64_t)gi * g.plane;
+#pragma unroll
+    for (int j = 0; j < FN; ++j) {
+        const int64_t n = n0 + wn + 16 * j + lr;
+#pragma unroll
+        for (int i = 0; i < FM; ++i) {
+            const int64_t m = m0 + wm + 16 * i + 4 * lg;
+            __builtin_nontemporal_store(acc[i][j], reinterpret_cast<iv4*>(out + n * g.Mp + m));
+        }
+    }
+}
+
+// The FP64 recombination of scheme I from the planes of groups [gb, ge) of
+// ng, for one thread's 4 rows of one column of the padded Mp x Np extent:
+// each group's x = sum * 2^-(bits*level) (exact) is added in group order,
+// W = x for the first group of all and W + x after (W carried in HBM from
+// one launch to the next); after the last group C = alpha*ldexp(W, e_i + f_j)
+// + beta*C.  Every FP64 operation is rounded on its own, in this order.
+__global__ __launch_bounds__(256) void oz_combine_kernel(const int32_t* __restrict__ planes, int64_t plane,
+                                                         const OzGroup* __restrict__ groups, int gb, int ge, int ng,
+                                                         int bits, double* __restrict__ W, int64_t Mp, int64_t m,
+                                                         int64_t n, const int32_t* __restrict__ ea,
+                                                         const int32_t* __restrict__ eb, double alpha, double beta,
+                                                         double* __restrict__ C, int64_t ldc, FastDiv dqm,
+                                                         uint32_t total) {
+    const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;  // one thread per 4 rows
+    if (x >= total) return;
+    const uint32_t j = fdiv(x, dqm);
+    const int64_t i0 = (int64_t)(x - j * dqm.d) * 4;
+    const int64_t e0 = (int64_t)j * Mp + i0;
+    double cur[4] = {0.0, 0.0, 0.0, 0.0};
+    if (gb > 0) {
+        const dv4 w = __builtin_nontemporal_load(reinterpret_cast<const dv4*>(W + e0));
+#pragma unroll
+        for (int e = 0; e < 4; ++e) cur[e] = w[e];
+    }
+    for (int gg = gb; gg < ge; ++gg) {
+        const double wgt = ldexp(1.0, -bits * groups[gg].level);
+        const iv4 v = __builtin_nontemporal_load(reinterpret_cast<const iv4*>(planes + (int64_t)(gg - gb) * plane + e0));
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const double xv = (double)v[e] * wgt;
+            cur[e] = gg == 0 ? xv : __dadd_rn(cur[e], xv);
+        }
+    }
+    if (ge < ng) {
+        __builtin_nontemporal_store(dv4{cur[0], cur[1], cur[2], cur[3]}, reinterpret_cast<dv4*>(W + e0));
+        return;
+    }
+    if ((int64_t)j >= n) return;
+    const int jf = eb[j];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const int64_t i = i0 + e;
+        if (i >= m) break;
+        double* cp = C + i + (int64_t)j * ldc;
+        double val = __dmul_rn(alpha, ldexp(cur[e], ea[i] + jf));
+        if (beta != 0.0) val = __dadd_rn(val, __dmul_rn(beta, *cp));
+        *cp = val;
     }
 }
 
@@ -1265,9 +1448,14 @@ int64_t crt_digit_bytes(int64_t m, int64_t n, int nmod) {
 
 namespace {
 
+// Scheme I groups whose planes one launch holds (the combine reads them all
+// per element): 8 planes of 4 bytes per output element.
+constexpr int kPlaneBatch = 8;
+
 struct Plan {
     int64_t Mp, Np, Kp, esz, a_slice, b_slice, res_plane, nchunk_a, nchunk_b, kchunk_max;
-    size_t off_a, off_b, off_ea, off_eb, off_part, off_w, total;
+    int64_t plane_batch;  // scheme I: groups per product launch
+    size_t off_a, off_b, off_ea, off_eb, off_part, off_w, off_planes, total;
 };
 
 // Host side of the CRT form: q and the reconstruction tables for n moduli.
@@ -1349,6 +1537,8 @@ Plan make_crt_plan(int64_t m, int64_t n, int64_t k, int nmod) {
     return p;
 }
 
+std::vector<OzSeg> make_schedule(int s, int bits, int64_t Kp, int64_t K, bool bf16, int* err);
+
 Plan make_plan(int64_t m, int64_t n, int64_t k, int s, bool bf16) {
     Plan p{};
     p.esz = bf16 ? 2 : 1;
@@ -1370,6 +1560,13 @@ Plan make_plan(int64_t m, int64_t n, int64_t k, int s, bool bf16) {
     p.off_eb = take((size_t)p.Np * 4);
     p.off_part = take((size_t)p.nchunk_a * (size_t)(m > n ? m : n) * 8);
     p.off_w = take((size_t)p.Mp * (size_t)p.Np * 8);
+    if (!bf16) {  // INT8 group planes
+        int err = 0;
+        int64_t groups = 0;
+        for (const OzSeg& g : make_schedule(s, 7, p.Kp, k, false, &err)) groups += g.flush != 0;
+        p.plane_batch = groups < kPlaneBatch ? (groups > 0 ? groups : 1) : kPlaneBatch;
+        p.off_planes = take((size_t)p.plane_batch * (size_t)p.Mp * (size_t)p.Np * 4);
+    }
     p.total = off;
     return p;
 }
@@ -1418,6 +1615,31 @@ std::vector<OzSeg> make_schedule(int s, int bits, int64_t Kp, int64_t K, bool bf
     for (auto it = segs.rbegin(); it != segs.rend(); ++it)
         if (it->flush) { it->flush = it->flush == 2 ? 4 : 3; break; }
     return segs;
+}
+
+// The schedule's products cut at its flushes: each group is consecutive
+// products of one level and one k range with t rising and u falling by one.
+// Returns false if the schedule does not have that form.
+bool make_groups(const std::vector<OzSeg>& sched, int64_t esz, std::vector<OzGroup>& out) {
+    out.clear();
+    for (size_t i = 0; i < sched.size();) {
+        size_t j = i;
+        while (j < sched.size() && !sched[j].flush) ++j;
+        if (j == sched.size()) return false;
+        const OzSeg& f = sched[i];
+        for (size_t q = i; q <= j; ++q) {
+            const OzSeg& x = sched[q];
+            if (x.level != f.level || x.k0 != f.k0 || x.klen != f.klen || x.t != f.t + (int)(q - i) ||
+                x.u != f.u - (int)(q - i))
+                return false;
+        }
+        if ((f.klen * esz) % CK2 != 0 || (f.k0 * esz) % CK2 != 0 || f.klen <= 0) return false;
+        out.push_back(OzGroup{f.t, f.u, (int32_t)(j - i + 1), f.level, (int32_t)(f.k0 * esz),
+                              make_fastdiv((uint32_t)(f.klen * esz / CK2))});
+        if ((int64_t)out.back().count * out.back().nks.d > INT32_MAX) return false;
+        i = j + 1;
+    }
+    return !out.empty();
 }
 
 }  // namespace
@@ -1648,23 +1870,55 @@ int dg_ozaki_dgemm(char opA, char opB, int64_t m, int64_t n, int64_t k, double a
         }
         DG_LAUNCH_CHECK();
     }
-    TableUseScope uses{st};  // after the launch below returns
-    const OzSeg* dsched = static_cast<const OzSeg*>(cached_table(sched.data(), sched.size() * sizeof(OzSeg), st));
-    if (!dsched) return DG_ERR_ALLOC;
-    OzArgs g{};
-    g.a = sa; g.b = sb; g.w = reinterpret_cast<double*>(ws + p.off_w);
-    g.ea = ea; g.eb = eb; g.segs = dsched; g.nseg = (int32_t)sched.size(); g.bits = bits;
-    g.m = m; g.n = n; g.Mp = p.Mp; g.Np = p.Np; g.Kp = p.Kp; g.a_slice = p.a_slice; g.b_slice = p.b_slice;
-    g.alpha = alpha; g.beta = beta; g.c = C; g.ldc = ldc;
-    g.tiles_m = (int32_t)(p.Mp / BM);
-    g.tiles_n = (int32_t)(p.Np / BN);
-    g.nsteps = 0;
-    for (const OzSeg& sg : sched) g.nsteps += (int64_t)sg.klen * p.esz / CK;  // klen*esz is a multiple of KSB
-    const int64_t tiles = (int64_t)g.tiles_m * g.tiles_n;
-    if (tiles > INT32_MAX) return fail(DG_ERR_INVALID, "dg_ozaki_dgemm: too many tiles");
-    if (bf16) hipLaunchKernelGGL(oz_gemm_kernel<true>, dim3((unsigned)tiles), dim3(CNT), kGemmLds, st, g);
-    else      hipLaunchKernelGGL(oz_gemm_kernel<false>, dim3((unsigned)tiles), dim3(CNT), kGemmLds, st, g);
-    DG_LAUNCH_CHECK();
+    if (bf16) {
+        // BF16 slices keep the in-loop flush kernel
+        TableUseScope uses{st};  // after the launch below returns
+        const OzSeg* dsched = static_cast<const OzSeg*>(cached_table(sched.data(), sched.size() * sizeof(OzSeg), st));
+        if (!dsched) return DG_ERR_ALLOC;
+        OzArgs g{};
+        g.a = sa; g.b = sb; g.w = reinterpret_cast<double*>(ws + p.off_w);
+        g.ea = ea; g.eb = eb; g.segs = dsched; g.nseg = (int32_t)sched.size(); g.bits = bits;
+        g.m = m; g.n = n; g.Mp = p.Mp; g.Np = p.Np; g.Kp = p.Kp; g.a_slice = p.a_slice; g.b_slice = p.b_slice;
+        g.alpha = alpha; g.beta = beta; g.c = C; g.ldc = ldc;
+        g.tiles_m = (int32_t)(p.Mp / BM);
+        g.tiles_n = (int32_t)(p.Np / BN);
+        g.nsteps = 0;
+        for (const OzSeg& sg : sched) g.nsteps += (int64_t)sg.klen * p.esz / CK;  // klen*esz is a multiple of KSB
+        const int64_t tiles = (int64_t)g.tiles_m * g.tiles_n;
+        if (tiles > INT32_MAX) return fail(DG_ERR_INVALID, "dg_ozaki_dgemm: too many tiles");
+        hipLaunchKernelGGL(oz_gemm_kernel<true>, dim3((unsigned)tiles), dim3(CNT), kGemmLds, st, g);
+        DG_LAUNCH_CHECK();
+        return DG_OK;
+    }
+    // INT8 products: each launch computes up to plane_batch groups into
+    // planes, then the combine adds them into W (or, after the last group, C).
+    std::vector<OzGroup> groups;
+    if (!make_groups(sched, p.esz, groups))
+        return fail(DG_ERR_INVALID, "dg_ozaki_dgemm: internal error: product schedule without group form");
+    TableUseScope uses{st};  // after the launches below return
+    const OzGroup* dgroups = static_cast<const OzGroup*>(cached_table(groups.data(), groups.size() * sizeof(OzGroup), st));
+    if (!dgroups) return DG_ERR_ALLOC;
+    const int ng = (int)groups.size();
+    const int64_t tiles = (p.Mp / BM) * (p.Np / BN);
+    if (tiles * p.plane_batch > INT32_MAX || (p.Mp / 4) * p.Np > UINT32_MAX)
+        return fail(DG_ERR_INVALID, "dg_ozaki_dgemm: too many tiles");
+    OzPlaneArgs g{};
+    g.a = sa; g.b = sb; g.planes = reinterpret_cast<int32_t*>(ws + p.off_planes); g.groups = dgroups;
+    g.tiles_m = (int32_t)(p.Mp / BM); g.tiles_n = (int32_t)(p.Np / BN);
+    g.Mp = p.Mp; g.rb = p.Kp * p.esz; g.a_slice = p.a_slice; g.b_slice = p.b_slice; g.plane = p.Mp * p.Np;
+    double* W = reinterpret_cast<double*>(ws + p.off_w);
+    const uint32_t total = (uint32_t)((p.Mp / 4) * p.Np);
+    const FastDiv dqm = make_fastdiv((uint32_t)(p.Mp / 4));
+    for (int g0 = 0; g0 < ng; g0 += (int)p.plane_batch) {
+        const int cnt = ng - g0 < (int)p.plane_batch ? ng - g0 : (int)p.plane_batch;
+        g.g0 = g0;
+        const dim3 grid((unsigned)(tiles * cnt));
+        const dim3 cgrid((total + 255) / 256);
+        hipLaunchKernelGGL(oz_plane_kernel, grid, dim3(CNT), kCrtLds, st, g);
+        hipLaunchKernelGGL(oz_combine_kernel, cgrid, dim3(256), 0, st, g.planes, g.plane, dgroups, g0, g0 + cnt, ng,
+                           bits, W, p.Mp, m, n, ea, eb, alpha, beta, C, ldc, dqm, total);
+        DG_LAUNCH_CHECK();
+    }
     return DG_OK;
 }
 
