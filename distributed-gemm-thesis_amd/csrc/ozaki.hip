@@ -807,11 +807,12 @@ __global__ __launch_bounds__(CNT, OCC) void crt_gemm_kernel(CrtArgs g) {
 // launch's group b / tiles and stores its INT32 sums as a plane, [Np][Mp]
 // column-major; oz_combine_kernel then adds the planes into FP64 in group
 // order.  BF16 slices stay on oz_gemm_kernel: built on this pipeline, the
-// BF16 products came out wrong in three of every four accumulator rows
-// while the INT8 ones, from the same code, were exact.  In that build about
-// a fifth of the BF16 MFMAs had a destination overlapping their A/B source
-// registers and a third renamed their accumulator, neither of which
-// oz_gemm_kernel's BF16 MFMAs do; the cause was not confirmed.
+// BF16 sums came out wrong in registers 1-3 of every accumulator while the
+// INT8 ones, from the same code, were exact.  Single BF16 MFMAs with the
+// register assignments of that build (renamed accumulators, a destination
+// over a fragment register) are exact, so the failure depends on the
+// instruction sequence (most likely a hazard with the interleaved LDS
+// reads); the cause was not confirmed.
 struct OzGroup {
     int32_t t0, u0;   // 0-based slices of the first product
     int32_t count;    // products
