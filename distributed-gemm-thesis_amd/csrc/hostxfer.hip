@@ -17,6 +17,8 @@
 #include <mutex>
 #include <thread>
 
+#include <unistd.h>
+
 #include "dg_internal.hpp"
 
 namespace dg {
@@ -31,9 +33,11 @@ constexpr int64_t kMinParallel = 1 << 20;  // smaller pieces stay on the calling
 // Worker threads kept for the life of the process.  run(parts, f) calls
 // f(0), ..., f(parts - 1), part 0 on the calling thread, and returns when
 // every part has returned.  One run at a time (callers hold the staging lock).
+// A child created by fork() has none of the workers, so there every part
+// runs on the calling thread.
 class CopyPool {
    public:
-    explicit CopyPool(int threads) : n_(threads) {
+    explicit CopyPool(int threads) : n_(threads), owner_(getpid()) {
         for (int i = 1; i < n_; ++i) th_.emplace_back([this, i] { loop(i); });
     }
     ~CopyPool() {
@@ -47,6 +51,10 @@ class CopyPool {
     int size() const { return n_; }
     void run(int parts, const std::function<void(int)>& f) {
         parts = std::min(parts, n_);
+        if (getpid() != owner_) {
+            for (int i = 0; i < parts; ++i) f(i);
+            return;
+        }
         if (parts <= 1) {
             f(0);
             return;
@@ -84,6 +92,7 @@ class CopyPool {
         }
     }
     int n_;
+    pid_t owner_;
     std::vector<std::thread> th_;
     std::mutex mu_;
     std::condition_variable go_, done_;
