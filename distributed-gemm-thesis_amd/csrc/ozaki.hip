@@ -809,10 +809,9 @@ __global__ __launch_bounds__(CNT, OCC) void crt_gemm_kernel(CrtArgs g) {
 // order.  BF16 slices stay on oz_gemm_kernel: built on this pipeline, the
 // BF16 sums came out wrong in registers 1-3 of every accumulator while the
 // INT8 ones, from the same code, were exact.  Single BF16 MFMAs with the
-// register assignments of that build (renamed accumulators, a destination
-// over a fragment register) are exact, so the failure depends on the
-// instruction sequence (most likely a hazard with the interleaved LDS
-// reads); the cause was not confirmed.
+// register assignments of that build are exact, and the same wrong values
+// came back with the LDS reads moved after the MFMAs and wait states
+// added, so it was no timing hazard; the cause was not found.
 struct OzGroup {
     int32_t t0, u0;   // 0-based slices of the first product
     int32_t count;    // products
