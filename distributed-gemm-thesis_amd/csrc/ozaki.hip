@@ -21,16 +21,16 @@
 //
 // Slices are stored k-contiguous, [t][row][Kp] with rows padded to 256 and K
 // padded to 128 bytes with zeros, so the product kernels have no edge cases
-// until the final store.  INT8 slices run each run of products between two
-// flushes (a group) as one long-K product through the scheme II pipeline
+// until the final store.  Each run of products between two flushes (a
+// group) runs as one long-K product through the scheme II pipeline
 // (oz_plane_kernel: 128-byte k-steps, described with crt_gemm_kernel) into a
-// 32-bit plane per group, and oz_combine_kernel adds the planes into the FP64
-// partial sums W (workspace) in schedule order.  BF16 slices run
-// oz_gemm_kernel: 256x256 output tiles, 8 waves of 128x64 (128 accumulator
-// registers per lane) and a 5-stage LDS ring of 64-byte k-steps filled by
-// direct global-to-LDS loads, flushing into W inside the product loop (one
-// coalesced read-modify-write of the wave's 128x64 block per flush).
+// 32-bit plane per group (INT32 sums, or the FP32 sums of BF16 slices), and
+// oz_combine_kernel adds the planes into the FP64 partial sums W (workspace)
+// in schedule order, with the operations a flush inside the product loop
+// would make.
 #include "dg_internal.hpp"
+
+#include <type_traits>
 
 // Every FP64 operation of the recombination is rounded on its own (no fused
 // multiply-add), matching the CPU oracle bit for bit.
@@ -48,22 +48,15 @@ typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
 
 constexpr int PAD = 256;     // row padding of the slices and residues (a multiple of BM and BN)
 constexpr int KSB = 128;     // K padding granule of the slices and residues, bytes
-constexpr int CK = 64;       // k-step in bytes
-// Product kernels: BM x BN output tiles, waves of WM x WN, STG-stage LDS ring,
-// OCC workgroups per CU.
+// Product kernels: BM x BN output tiles of 8 waves of WM x WN (FM x FN
+// MFMA tiles of 16x16), one workgroup per CU.
 constexpr int BM = 256, BN = 256;
 constexpr int WM = 128, WN = 64;
-constexpr int STG = 5;
 constexpr int OCC = 1;
 constexpr int NW = (BM / WM) * (BN / WN);  // waves per workgroup
 constexpr int CNT = NW * 64;               // threads per workgroup
 constexpr int FM = WM / 16, FN = WN / 16;  // 16x16 MFMA tiles per wave
-constexpr int LA = BM / 16 / NW, LB = BN / 16 / NW;  // 16-row loads per wave and step
-constexpr int LPS = LA + LB;
-constexpr int SSZ = (BM + BN) * CK;        // bytes per stage (A rows, then B rows)
-constexpr size_t kGemmLds = (size_t)STG * SSZ;
-static_assert(LA * 16 * NW == BM && LB * 16 * NW == BN, "loader split");
-static_assert(PAD % BM == 0 && PAD % BN == 0 && STG >= 4 && STG <= 5, "tile configuration (waits reach 3 steps ahead)");
+static_assert(PAD % BM == 0 && PAD % BN == 0, "slice rows are padded to whole tiles");
 constexpr int NT = 256;
 constexpr int GROUP_M = 4;
 
@@ -85,68 +78,11 @@ struct OzSeg {
                         // 3: last flush (C = alpha*ldexp(W + x, e+f) + beta*C); 4: first and last
 };
 
-struct OzArgs {
-    const int8_t* a;      // A slices, [s][Mp][Kp] elements
-    const int8_t* b;      // B slices, [s][Np][Kp]
-    double* w;            // FP64 partial sums, Mp x Np column-major
-    const int32_t* ea;    // row exponents (Mp)
-    const int32_t* eb;    // column exponents (Np)
-    const OzSeg* segs;
-    int32_t nseg;
-    int32_t bits;
-    int64_t m, n, Mp, Np, Kp;   // Kp in elements
-    int64_t a_slice, b_slice;   // bytes per slice
-    double alpha, beta;
-    double* c;
-    int64_t ldc;
-    int32_t tiles_m, tiles_n;
-    int64_t nsteps;             // k-steps over all products
-};
-
-template <bool BF16>
-__device__ __forceinline__ void mma(iv4& acc_i, fv4& acc_f, const iv4& x, const iv4& y) {
-    if constexpr (BF16) {
-        acc_f = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf8, x), __builtin_bit_cast(bf8, y), acc_f,
-                                                        0, 0, 0);
-    } else {
-        acc_i = __builtin_amdgcn_mfma_i32_16x16x64_i8(x, y, acc_i, 0, 0, 0);
-    }
-}
-
-// The product kernels' pipeline.  BM x BN output tiles, waves of WM x WN
-// (FM x FN MFMA tiles of 16x16), 64-byte k-steps in an STG-stage LDS ring
-// filled by global_load_lds_dwordx4, which writes LDS without a register
-// round trip.  One such load of a wave covers 16 rows x 64 bytes; within a
-// row, 16-byte chunk c sits at position c ^ ((row >> 1) & 3).  With that
-// placement a wave's 128-bit fragment reads (lane l: row l % 16, chunk l / 16)
-// run at the LDS's full 256 bytes per clock on gfx950; without it, or with
-// the chunk flipped for rows 8..15, they run at half that rate (measured with
-// a read-only microbenchmark of the five placements).
-// In the scheme II kernel the barrier runs one step ahead: at the end of
-// step k each wave waits for its own loads of step k+2, and one barrier
-// publishes step k+2 and frees the stage of step k, which then receives step
-// k+STG.  The fragments of step k+1 (published a step earlier) are read while
-// the MFMAs of step k run: each A fragment right after its last use, the B
-// fragments into a second register set, so LDS latency hides behind the
-// matrix cores.  The scheme I kernel, whose product boundaries and flushes
-// break that rhythm, publishes step k at its start and reads every fragment
-// before the MFMAs.
-__host__ __device__ constexpr int swz(int r) { return (r >> 1) & 3; }
-
 template <int N>
 __device__ __forceinline__ void wait_vmcnt() {
     static_assert(N >= 0 && N < 64, "vmcnt field");
     // gfx9 s_waitcnt: vmcnt in bits 3:0 and 15:14; expcnt and lgkmcnt left at their maxima
     __builtin_amdgcn_s_waitcnt((N & 0xF) | ((N >> 4) << 14) | (0x7 << 4) | (0xF << 8));
-}
-
-// Waits until at most `ahead` steps (LPS loads each, ahead <= 3) of this
-// wave's loads are still pending.
-__device__ __forceinline__ void wait_step(int64_t ahead) {
-    if (ahead >= 3) wait_vmcnt<3 * LPS>();
-    else if (ahead == 2) wait_vmcnt<2 * LPS>();
-    else if (ahead == 1) wait_vmcnt<LPS>();
-    else wait_vmcnt<0>();
 }
 
 typedef __attribute__((address_space(3))) void lds_void_t;
@@ -160,106 +96,6 @@ __device__ __forceinline__ const int8_t* wave_uniform(const int8_t* p) {
     const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
     const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
     return reinterpret_cast<const int8_t*>(((uint64_t)hi << 32) | lo);
-}
-
-// Issues one k-step of both operands into stage `stage`: this wave's A rows
-// 16*LA*w .. +16*LA and B rows 16*LB*w .. +16*LB, 16 rows per load; pa/pb
-// (wave-uniform) point at the wave's first row of the step, loff is the
-// lane's offset to its row and chunk, rows `rb` bytes apart.
-__device__ __forceinline__ void issue_step(int8_t* lds, int stage, int wave, const int8_t* pa, const int8_t* pb,
-                                           int64_t rb, uint32_t loff) {
-    int8_t* sa = lds + stage * SSZ + wave * (1024 * LA);
-    int8_t* sb = lds + stage * SSZ + BM * CK + wave * (1024 * LB);
-#pragma unroll
-    for (int i = 0; i < LA; ++i)
-        __builtin_amdgcn_global_load_lds((gvoid_t*)(wave_uniform(pa + 16 * i * rb) + loff),
-                                         (lds_void_t*)(sa + 1024 * i), 16, 0, 0);
-#pragma unroll
-    for (int i = 0; i < LB; ++i)
-        __builtin_amdgcn_global_load_lds((gvoid_t*)(wave_uniform(pb + 16 * i * rb) + loff),
-                                         (lds_void_t*)(sb + 1024 * i), 16, 0, 0);
-}
-
-// Fragments of one step: fa (A) and fb (B) from this lane's place in the
-// wave's first 16-row block of A (as) and B (bs).
-__device__ __forceinline__ void read_frags(iv4 (&fa)[FM], iv4 (&fb)[FN], const int8_t* as, const int8_t* bs) {
-#pragma unroll
-    for (int j = 0; j < FN; ++j) fb[j] = *reinterpret_cast<const iv4*>(bs + 1024 * j);
-#pragma unroll
-    for (int i = 0; i < FM; ++i) fa[i] = *reinterpret_cast<const iv4*>(as + 1024 * i);
-}
-
-// The FM x FN MFMAs of one step on fa / cb, with the next step's fragments
-// read meanwhile: fa[i] right after its last use, the B set into nb.
-template <bool BF16>
-__device__ __forceinline__ void step_mma(iv4 (&acci)[FM][FN], fv4 (&accf)[FM][FN], iv4 (&fa)[FM], const iv4 (&cb)[FN],
-                                         iv4 (&nb)[FN], const int8_t* nas, const int8_t* nbs) {
-#pragma unroll
-    for (int i = 0; i < FM; ++i) {
-#pragma unroll
-        for (int j = 0; j < FN; ++j) mma<BF16>(acci[i][j], accf[i][j], cb[j], fa[i]);
-        if (i == 0) {
-#pragma unroll
-            for (int j = 0; j < FN; ++j) nb[j] = *reinterpret_cast<const iv4*>(nbs + 1024 * j);
-        }
-        fa[i] = *reinterpret_cast<const iv4*>(nas + 1024 * i);
-    }
-    // interleave: FN MFMAs, then the reads they free
-    __builtin_amdgcn_sched_group_barrier(0x008, FN, 0);
-    __builtin_amdgcn_sched_group_barrier(0x100, FN + 1, 0);
-#pragma unroll
-    for (int g = 1; g < FM; ++g) {
-        __builtin_amdgcn_sched_group_barrier(0x008, FN, 0);
-        __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
-    }
-}
-
-// The FM x FN MFMAs of one step with every fragment read first (the scheme I
-// kernel, whose product boundaries and flushes break the step-ahead rhythm).
-template <bool BF16>
-__device__ __forceinline__ void stage_mma(iv4 (&acci)[FM][FN], fv4 (&accf)[FM][FN], const int8_t* as, const int8_t* bs) {
-    iv4 fa[FM], fb[FN];
-    read_frags(fa, fb, as, bs);
-    __builtin_amdgcn_sched_barrier(0);  // every fragment read ahead of the MFMAs
-#pragma unroll
-    for (int i = 0; i < FM; ++i)
-#pragma unroll
-        for (int j = 0; j < FN; ++j) mma<BF16>(acci[i][j], accf[i][j], fb[j], fa[i]);
-}
-
-// stage_mma that also issues this wave's loads of one later k-step into
-// stage `stage`: A rows from pa, B rows from pb (wave-uniform, lane offset
-// loff, rows rb bytes apart), one global_load_lds after every two groups of
-// FN MFMAs instead of four back to back before them.
-template <bool BF16>
-__device__ __forceinline__ void stage_mma_ld(iv4 (&acci)[FM][FN], fv4 (&accf)[FM][FN], const int8_t* as,
-                                             const int8_t* bs, int8_t* lds, int stage, int wave, const int8_t* pa,
-                                             const int8_t* pb, uint32_t loff, int64_t rb) {
-    static_assert(LA + LB == FM / 2, "one load after every two MFMA groups");
-    iv4 fa[FM], fb[FN];
-    read_frags(fa, fb, as, bs);
-    int8_t* sa = lds + stage * SSZ + wave * (1024 * LA);
-    int8_t* sb = lds + stage * SSZ + BM * CK + wave * (1024 * LB);
-#pragma unroll
-    for (int i = 0; i < FM; ++i) {
-#pragma unroll
-        for (int j = 0; j < FN; ++j) mma<BF16>(acci[i][j], accf[i][j], fb[j], fa[i]);
-        if (i & 1) {
-            const int l = i >> 1;
-            if (l < LA)
-                __builtin_amdgcn_global_load_lds((gvoid_t*)(wave_uniform(pa + 16 * l * rb) + loff),
-                                                 (lds_void_t*)(sa + 1024 * l), 16, 0, 0);
-            else
-                __builtin_amdgcn_global_load_lds((gvoid_t*)(wave_uniform(pb + 16 * (l - LA) * rb) + loff),
-                                                 (lds_void_t*)(sb + 1024 * (l - LA)), 16, 0, 0);
-        }
-    }
-    __builtin_amdgcn_sched_group_barrier(0x100, FM + FN, 0);  // every fragment read ahead of the MFMAs
-#pragma unroll
-    for (int g = 0; g < FM / 2; ++g) {
-        __builtin_amdgcn_sched_group_barrier(0x008, 2 * FN, 0);
-        __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);
-    }
 }
 
 // Output tile of workgroup slot `bid` among `tiles`: XCD-aware, GROUP_M tile
@@ -277,144 +113,6 @@ __device__ __forceinline__ void tile_origin(int bid, int tiles, int tiles_m, int
     const int in_group = tile - group * per_group;
     m0 = (int64_t)(first_m + in_group % gsize) * BM;
     n0 = (int64_t)(in_group / gsize) * BN;
-}
-
-// Flushes: the first one writes W = x, later ones W += x, the last one
-// writes C = alpha * ldexp(W + x, e_i + f_j) + beta * C (or C from x alone
-// when the first flush is also the last).  x = acc * 2^-(bits*level) is exact.
-
-// Register v of lane group lg holds C(mrow + 16i, ncol + 16j + v) with
-// mrow = m0 + wm + l%16, ncol = n0 + wn + 4*(l/16): the 16x16 i32/f32 result
-// layout D(4*(l/16) + v, l%16) of D = B^T A^T.
-// kind: 2 first flush (W = x), 1 later flush (W += x), 3 last flush
-// (C from W + x), 4 first and last (C from x alone); guard: the tile crosses
-// the edge of C.  Every branch is uniform across the wave.
-template <bool BF16>
-__device__ __forceinline__ void oz_flush(const iv4 (&acci)[FM][FN], const fv4 (&accf)[FM][FN], const OzArgs& g,
-                                         double w, int64_t mrow, int64_t ncol, int kind, bool guard) {
-    const bool to_c = kind >= 3;
-    const bool read_w = kind == 1 || kind == 3;
-#pragma unroll
-    for (int j = 0; j < FN; ++j) {
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            const int64_t n = ncol + 16 * j + v;
-            double* wcol = g.w + n * g.Mp + mrow;
-            double* ccol = g.c + n * g.ldc + mrow;
-            const int ef = to_c && n < g.n ? g.eb[n] : 0;
-#pragma unroll
-            for (int i = 0; i < FM; ++i) {
-                double x;
-                if constexpr (BF16) x = (double)accf[i][j][v] * w;
-                else x = (double)acci[i][j][v] * w;
-                const double acc = read_w ? __dadd_rn(wcol[16 * i], x) : x;
-                if (!to_c) {
-                    wcol[16 * i] = acc;
-                } else if (!guard || (mrow + 16 * i < g.m && n < g.n)) {
-                    double val = __dmul_rn(g.alpha, ldexp(acc, g.ea[mrow + 16 * i] + ef));
-                    if (g.beta != 0.0) val = __dadd_rn(val, __dmul_rn(g.beta, ccol[16 * i]));
-                    ccol[16 * i] = val;
-                }
-            }
-            // one column group's read-modify-write at a time
-            __builtin_amdgcn_sched_barrier(0);
-        }
-    }
-}
-
-// Slice products of one 256x256 output tile: the k-steps of every product of
-// the schedule run back to back through the LDS ring; at the end of a product
-// marked for flushing the 32-bit sums go into W (or, at the last flush, C).
-template <bool BF16>
-__global__ __launch_bounds__(CNT, OCC) void oz_gemm_kernel(OzArgs g) {
-    extern __shared__ __attribute__((aligned(1024))) int8_t lds[];
-    int64_t m0, n0;
-    tile_origin(blockIdx.x, g.tiles_m * g.tiles_n, g.tiles_m, g.tiles_n, m0, n0);
-
-    const int lane = threadIdx.x & 63;
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int wm = (wave / (BN / WN)) * WM;
-    const int wn = (wave % (BN / WN)) * WN;
-    const int lr = lane & 15;
-    const int lg = lane >> 4;
-
-    const int esz = BF16 ? 2 : 1;
-    const int64_t rb = g.Kp * esz;  // bytes per slice row
-    const int lrow = lane >> 2;
-    const int lchunk = (lane & 3) ^ swz(lrow);
-    const int8_t* ga = g.a + (m0 + 16 * LA * wave) * rb;  // wave-uniform; lanes add loff
-    const int8_t* gb = g.b + (n0 + 16 * LB * wave) * rb;
-    const uint32_t loff = (uint32_t)(lrow * rb + 16 * lchunk);
-    const int roff = lr * CK + 16 * (lg ^ swz(lr));
-
-    // issue cursor: product iseg, byte offset ikb within it.  Past the last
-    // product it starts the last product over: those reloads land in stages
-    // that are free and never read again, so that every step issues one
-    // step of loads and every wait is the same.
-    int iseg = 0;
-    int64_t ikb = 0;
-    OzSeg is = g.segs[0];
-    auto next_issue = [&](const int8_t*& pa, const int8_t*& pb) {
-        const int64_t koff = (int64_t)is.k0 * esz + ikb;
-        pa = ga + is.t * g.a_slice + koff;
-        pb = gb + is.u * g.b_slice + koff;
-        ikb += CK;
-        if (ikb >= (int64_t)is.klen * esz) {
-            ikb = 0;
-            if (iseg + 1 < g.nseg) is = g.segs[++iseg];
-        }
-    };
-
-    iv4 acci[FM][FN];
-    fv4 accf[FM][FN];
-#pragma unroll
-    for (int i = 0; i < FM; ++i)
-#pragma unroll
-        for (int j = 0; j < FN; ++j) {
-            if constexpr (BF16) accf[i][j] = fv4{0.f, 0.f, 0.f, 0.f};
-            else acci[i][j] = iv4{0, 0, 0, 0};
-        }
-
-    // Scheme I keeps the barrier at the start of each step: wait for this
-    // wave's loads of step k, publish it, read every fragment, then the
-    // MFMAs, among which the stage of step k-1 is refilled with step k+STG-1.
-    for (int st0 = 0; st0 < STG - 1; ++st0) {
-        const int8_t *pa, *pb;
-        next_issue(pa, pb);
-        issue_step(lds, st0, wave, pa, pb, rb, loff);
-    }
-    int ist = STG - 1;  // stage of the next issue
-    const int64_t mrow = m0 + wm + lr;
-    const int64_t ncol = n0 + wn + 4 * lg;
-    const bool full = (m0 + BM <= g.m) && (n0 + BN <= g.n);
-    int64_t kt = 0;
-    int cst = 0;  // stage of step kt
-    for (int cseg = 0; cseg < g.nseg; ++cseg) {
-        const OzSeg cs = g.segs[cseg];
-        const int64_t kend = kt + (int64_t)cs.klen * esz / CK;
-        for (; kt < kend; ++kt) {
-            wait_vmcnt<(STG - 2) * LPS>();  // step kt landed; the STG-2 later steps may be in flight
-            __builtin_amdgcn_s_barrier();
-            const int8_t *pa, *pb;
-            next_issue(pa, pb);
-            const int8_t* st = lds + cst * SSZ;
-            stage_mma_ld<BF16>(acci, accf, st + wm * CK + roff, st + BM * CK + wn * CK + roff, lds, ist, wave, pa,
-                               pb, loff, rb);
-            ist = ist + 1 == STG ? 0 : ist + 1;
-            cst = cst + 1 == STG ? 0 : cst + 1;
-        }
-        if (cs.flush) {
-            oz_flush<BF16>(acci, accf, g, ldexp(1.0, -g.bits * cs.level), mrow, ncol, cs.flush, !full);
-#pragma unroll
-            for (int i = 0; i < FM; ++i)
-#pragma unroll
-                for (int j = 0; j < FN; ++j) {
-                    if constexpr (BF16) accf[i][j] = fv4{0.f, 0.f, 0.f, 0.f};
-                    else acci[i][j] = iv4{0, 0, 0, 0};
-                }
-        }
-    }
-    wait_vmcnt<0>();  // the reloads past the end land before the workgroup's LDS is released
 }
 
 // ------------------------------------------------------------ Ozaki scheme II
@@ -621,14 +319,21 @@ __device__ __forceinline__ void crt_issue(int8_t* lds, int slot, int wave, const
                                          (lds_void_t*)(d + 1024 * i), 16, 0, 0);
 }
 
-// One 64-byte MFMA of the scheme II pipeline: 16x16 INT8 products summed in INT32.
+// One 64-byte MFMA of the pipeline: 16x16 INT8 products summed in INT32
+// (iv4 accumulators) or BF16 products summed in FP32 (fv4 accumulators, for
+// the BF16 slices of scheme I).  Either way a lane's A and B operands are 16
+// bytes of one row, so the LDS layout is the same.
 __device__ __forceinline__ void mfma64(iv4& acc, const iv4& a, const iv4& b) {
     acc = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b, acc, 0, 0, 0);
+}
+__device__ __forceinline__ void mfma64(fv4& acc, const iv4& a, const iv4& b) {
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf8, a), __builtin_bit_cast(bf8, b), acc, 0, 0, 0);
 }
 
 // One MFMA half-step on fa / cb, reading the fragments of the following
 // half-step meanwhile: fa[i] right after its last use, B into nb.
-__device__ __forceinline__ void crt_half(iv4 (&acc)[FM][FN], iv4 (&fa)[FM], const iv4 (&cb)[FN], iv4 (&nb)[FN],
+template <class ACC>
+__device__ __forceinline__ void crt_half(ACC (&acc)[FM][FN], iv4 (&fa)[FM], const iv4 (&cb)[FN], iv4 (&nb)[FN],
                                          const int8_t* nas, const int8_t* nbs) {
 #pragma unroll
     for (int i = 0; i < FM; ++i) {
@@ -654,7 +359,8 @@ __device__ __forceinline__ void crt_half(iv4 (&acc)[FM][FN], iv4 (&fa)[FM], cons
 // MFMAs:
 // issued back to back, the eight 1 KB loads of every wave queue at the
 // vector memory unit while the matrix cores wait.
-__device__ __forceinline__ void crt_half_ld(iv4 (&acc)[FM][FN], iv4 (&fa)[FM], const iv4 (&cb)[FN], iv4 (&nb)[FN],
+template <class ACC>
+__device__ __forceinline__ void crt_half_ld(ACC (&acc)[FM][FN], iv4 (&fa)[FM], const iv4 (&cb)[FN], iv4 (&nb)[FN],
                                             const int8_t* nas, const int8_t* nbs, int8_t* lds, int slot_b,
                                             int slot_a, int wave, const int8_t* gb, const int8_t* ga, uint32_t loff,
                                             int64_t sb, int64_t sa, int64_t ld) {
@@ -799,19 +505,14 @@ __global__ __launch_bounds__(CNT, OCC) void crt_gemm_kernel(CrtArgs g) {
     }
 }
 
-// ------------------------------------- scheme I (INT8) on the scheme II pipeline
+// ---------------------------------------------- scheme I on the scheme II pipeline
 // The products of scheme I between two flushes (a group: slices t0 + p of A
 // against slices u0 - p of B for p < count, one level, one k range) add up
 // to one long-K product, so a group's k-steps run back to back through the
 // scheme II pipeline above.  Workgroup b computes tile b % tiles of the
-// launch's group b / tiles and stores its INT32 sums as a plane, [Np][Mp]
-// column-major; oz_combine_kernel then adds the planes into FP64 in group
-// order.  BF16 slices stay on oz_gemm_kernel: built on this pipeline, the
-// BF16 sums came out wrong in registers 1-3 of every accumulator while the
-// INT8 ones, from the same code, were exact.  Single BF16 MFMAs with the
-// register assignments of that build are exact, and the same wrong values
-// came back with the LDS reads moved after the MFMAs and wait states
-// added, so it was no timing hazard; the cause was not found.
+// launch's group b / tiles and stores its 32-bit sums (INT32 for INT8
+// slices, FP32 for BF16 slices) as a plane, [Np][Mp] column-major;
+// oz_combine_kernel then adds the planes into FP64 in group order.
 struct OzGroup {
     int32_t t0, u0;   // 0-based slices of the first product
     int32_t count;    // products
@@ -843,7 +544,9 @@ __device__ __forceinline__ const int8_t* group_step(const int8_t* first, int64_t
     return first + (int64_t)p * stride + (int64_t)(sc - p * nks.d) * CK2;
 }
 
+template <bool BF16>
 __global__ __launch_bounds__(CNT, OCC) void oz_plane_kernel(OzPlaneArgs g) {
+    typedef typename std::conditional<BF16, fv4, iv4>::type acc_t;
     extern __shared__ __attribute__((aligned(1024))) int8_t lds[];
     const int tiles = g.tiles_m * g.tiles_n;
     const int gi = blockIdx.x / tiles;
@@ -869,11 +572,11 @@ __global__ __launch_bounds__(CNT, OCC) void oz_plane_kernel(OzPlaneArgs g) {
     const auto a_at = [&](uint32_t s) { return group_step(ga, g.a_slice, s, last, gr.nks); };
     const auto b_at = [&](uint32_t s) { return group_step(gb, -g.b_slice, s, last, gr.nks); };
 
-    iv4 acc[FM][FN];
+    acc_t acc[FM][FN];
 #pragma unroll
     for (int i = 0; i < FM; ++i)
 #pragma unroll
-        for (int j = 0; j < FN; ++j) acc[i][j] = iv4{0, 0, 0, 0};
+        for (int j = 0; j < FN; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
 
     // prologue: A_0 B_0 A_1 B_1 A_2 into slots 0..4, as in crt_gemm_kernel
     crt_issue(lds, 0, wave, a_at(0), loff, 0, g.rb);
@@ -920,7 +623,7 @@ __global__ __launch_bounds__(CNT, OCC) void oz_plane_kernel(OzPlaneArgs g) {
 #pragma unroll
         for (int i = 0; i < FM; ++i) {
             const int64_t m = m0 + wm + 16 * i + 4 * lg;
-            __builtin_nontemporal_store(acc[i][j], reinterpret_cast<iv4*>(out + n * g.Mp + m));
+            __builtin_nontemporal_store(__builtin_bit_cast(iv4, acc[i][j]), reinterpret_cast<iv4*>(out + n * g.Mp + m));
         }
     }
 }
@@ -931,6 +634,7 @@ __global__ __launch_bounds__(CNT, OCC) void oz_plane_kernel(OzPlaneArgs g) {
 // W = x for the first group of all and W + x after (W carried in HBM from
 // one launch to the next); after the last group C = alpha*ldexp(W, e_i + f_j)
 // + beta*C.  Every FP64 operation is rounded on its own, in this order.
+template <bool BF16>
 __global__ __launch_bounds__(256) void oz_combine_kernel(const int32_t* __restrict__ planes, int64_t plane,
                                                          const OzGroup* __restrict__ groups, int gb, int ge, int ng,
                                                          int bits, double* __restrict__ W, int64_t Mp, int64_t m,
@@ -952,9 +656,12 @@ __global__ __launch_bounds__(256) void oz_combine_kernel(const int32_t* __restri
     for (int gg = gb; gg < ge; ++gg) {
         const double wgt = ldexp(1.0, -bits * groups[gg].level);
         const iv4 v = __builtin_nontemporal_load(reinterpret_cast<const iv4*>(planes + (int64_t)(gg - gb) * plane + e0));
+        // the FP32 sums of BF16 slices: the whole vector is bit-cast (a bit cast
+        // of one vector element, v[e], read the wrong element with this compiler)
+        const fv4 vf = __builtin_bit_cast(fv4, v);
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
-            const double xv = (double)v[e] * wgt;
+            const double xv = (BF16 ? (double)vf[e] : (double)v[e]) * wgt;
             cur[e] = gg == 0 ? xv : __dadd_rn(cur[e], xv);
         }
     }
@@ -1560,10 +1267,10 @@ Plan make_plan(int64_t m, int64_t n, int64_t k, int s, bool bf16) {
     p.off_eb = take((size_t)p.Np * 4);
     p.off_part = take((size_t)p.nchunk_a * (size_t)(m > n ? m : n) * 8);
     p.off_w = take((size_t)p.Mp * (size_t)p.Np * 8);
-    if (!bf16) {  // INT8 group planes
+    {  // group planes
         int err = 0;
         int64_t groups = 0;
-        for (const OzSeg& g : make_schedule(s, 7, p.Kp, k, false, &err)) groups += g.flush != 0;
+        for (const OzSeg& g : make_schedule(s, bf16 ? 5 : 7, p.Kp, k, bf16, &err)) groups += g.flush != 0;
         p.plane_batch = groups < kPlaneBatch ? (groups > 0 ? groups : 1) : kPlaneBatch;
         p.off_planes = take((size_t)p.plane_batch * (size_t)p.Mp * (size_t)p.Np * 4);
     }
@@ -1870,26 +1577,6 @@ int dg_ozaki_dgemm(char opA, char opB, int64_t m, int64_t n, int64_t k, double a
         }
         DG_LAUNCH_CHECK();
     }
-    if (bf16) {
-        // BF16 slices keep the in-loop flush kernel
-        TableUseScope uses{st};  // after the launch below returns
-        const OzSeg* dsched = static_cast<const OzSeg*>(cached_table(sched.data(), sched.size() * sizeof(OzSeg), st));
-        if (!dsched) return DG_ERR_ALLOC;
-        OzArgs g{};
-        g.a = sa; g.b = sb; g.w = reinterpret_cast<double*>(ws + p.off_w);
-        g.ea = ea; g.eb = eb; g.segs = dsched; g.nseg = (int32_t)sched.size(); g.bits = bits;
-        g.m = m; g.n = n; g.Mp = p.Mp; g.Np = p.Np; g.Kp = p.Kp; g.a_slice = p.a_slice; g.b_slice = p.b_slice;
-        g.alpha = alpha; g.beta = beta; g.c = C; g.ldc = ldc;
-        g.tiles_m = (int32_t)(p.Mp / BM);
-        g.tiles_n = (int32_t)(p.Np / BN);
-        g.nsteps = 0;
-        for (const OzSeg& sg : sched) g.nsteps += (int64_t)sg.klen * p.esz / CK;  // klen*esz is a multiple of KSB
-        const int64_t tiles = (int64_t)g.tiles_m * g.tiles_n;
-        if (tiles > INT32_MAX) return fail(DG_ERR_INVALID, "dg_ozaki_dgemm: too many tiles");
-        hipLaunchKernelGGL(oz_gemm_kernel<true>, dim3((unsigned)tiles), dim3(CNT), kGemmLds, st, g);
-        DG_LAUNCH_CHECK();
-        return DG_OK;
-    }
     // INT8 products: each launch computes up to plane_batch groups into
     // planes, then the combine adds them into W (or, after the last group, C).
     std::vector<OzGroup> groups;
@@ -1914,9 +1601,15 @@ int dg_ozaki_dgemm(char opA, char opB, int64_t m, int64_t n, int64_t k, double a
         g.g0 = g0;
         const dim3 grid((unsigned)(tiles * cnt));
         const dim3 cgrid((total + 255) / 256);
-        hipLaunchKernelGGL(oz_plane_kernel, grid, dim3(CNT), kCrtLds, st, g);
-        hipLaunchKernelGGL(oz_combine_kernel, cgrid, dim3(256), 0, st, g.planes, g.plane, dgroups, g0, g0 + cnt, ng,
-                           bits, W, p.Mp, m, n, ea, eb, alpha, beta, C, ldc, dqm, total);
+        if (bf16) {
+            hipLaunchKernelGGL(oz_plane_kernel<true>, grid, dim3(CNT), kCrtLds, st, g);
+            hipLaunchKernelGGL(oz_combine_kernel<true>, cgrid, dim3(256), 0, st, g.planes, g.plane, dgroups, g0,
+                               g0 + cnt, ng, bits, W, p.Mp, m, n, ea, eb, alpha, beta, C, ldc, dqm, total);
+        } else {
+            hipLaunchKernelGGL(oz_plane_kernel<false>, grid, dim3(CNT), kCrtLds, st, g);
+            hipLaunchKernelGGL(oz_combine_kernel<false>, cgrid, dim3(256), 0, st, g.planes, g.plane, dgroups, g0,
+                               g0 + cnt, ng, bits, W, p.Mp, m, n, ea, eb, alpha, beta, C, ldc, dqm, total);
+        }
         DG_LAUNCH_CHECK();
     }
     return DG_OK;
