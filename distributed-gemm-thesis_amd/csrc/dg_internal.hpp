@@ -148,29 +148,42 @@ int launch_gemm_f64(char opA, char opB, const std::vector<Prob>& probs,
 // B with opB = T) has 8-byte aligned bases with the same offset within 16
 // bytes in every segment; the register-staged kernel takes the others.
 // DG_GEMM_NO_DMA routes everything to the register-staged kernel.
+// Direct-to-LDS problems whose tiles are all interior (no row shift, m and n
+// multiples of 128, whole k-steps, an even step count) go to `dma`, which
+// the A-in-registers kernel runs; the others go to `dmae` (the edge-handling
+// kernel).  With split_edges, a direct-to-LDS problem with edges and at
+// least 8 interior tiles is cut into sub-problems: an interior block on
+// `dma`, and strips on `dmae` around it, placed so that the tile count does
+// not grow (a row-shifted problem's interior starts at its row 127).  The
+// sub-problems' segments are appended to `segs` (which starts as a copy of
+// the input segments); every element still sums in the same order.
 struct GemmBatch {
-    std::vector<Prob> dma, gen;   // tile ranges assigned per list
-    int tiles_dma = 0;            // 128x128 tiles of `dma`
-    int tiles_gen[2] = {0, 0};    // tiles of `gen` per tile shape
-    bool vec = true;              // every `gen` segment allows 128-bit loads
-    bool dma_edge = false;        // some `dma` problem has edge tiles, a row shift or partial k-steps
+    std::vector<Prob> dma, dmae, gen;  // tile ranges assigned per list
+    int tiles_dma = 0;                 // 128x128 tiles of `dma`
+    int tiles_dmae = 0;                // 128x128 tiles of `dmae`
+    int tiles_gen[2] = {0, 0};         // tiles of `gen` per tile shape
+    bool vec = true;                   // every `gen` segment allows 128-bit loads
+    std::vector<Seg> segs;             // the segment table all three lists index
 };
 int prepare_gemm(char opA, char opB, const std::vector<Prob>& probs_in, const std::vector<Seg>& segs,
-                 GemmBatch& out);
+                 GemmBatch& out, bool split_edges = true);
 
 // The same batch with its tables resident on the device.
 struct GemmBatchDev {
     const Prob* dma = nullptr;
     int ndma = 0, tiles_dma = 0;
+    const Prob* dmae = nullptr;
+    int ndmae = 0, tiles_dmae = 0;
     const Prob* gen = nullptr;
     int ngen = 0, tiles_gen[2] = {0, 0};
-    bool vec = true, dma_edge = false;
+    bool vec = true;
     const Seg* segs = nullptr;
 };
 // Device half.  The persistent mode needs a device int `counter`, which it
-// zeroes with a kernel before each launch (graph safe).  The direct-to-LDS
-// list runs that kernel in every mode and tile setting, the other list the
-// register-staged kernel of tile shape cfg.
+// zeroes with a kernel before each launch (graph safe).  `dma` runs the
+// A-in-registers kernel (the lean direct-to-LDS kernel under
+// DG_GEMM_NO_AREG), `dmae` the edge-handling direct-to-LDS kernel, `gen` the
+// register-staged kernel of tile shape cfg; one launch per non-empty list.
 int launch_gemm_f64_dev(char opA, char opB, int mode, int cfg, const GemmBatchDev& b, int* counter,
                         hipStream_t stream);
 // Tile shape used by the one-shot entry points (DG_GEMM_TILE env override).

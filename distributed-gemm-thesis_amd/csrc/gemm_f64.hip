@@ -955,12 +955,44 @@ int launch_scale(int64_t m, int64_t n, double beta, double* C, int64_t ldc, hipS
 }
 
 int prepare_gemm(char opA, char opB, const std::vector<Prob>& probs_in, const std::vector<Seg>& segs,
-                 GemmBatch& out) {
+                 GemmBatch& out, bool split_edges) {
     out = GemmBatch{};
+    out.segs = segs;
     const bool ak = opA == 'T', bkc = opB != 'T';  // k-contiguous operands
     static const bool no_dma = getenv("DG_GEMM_NO_DMA") != nullptr;
-    int64_t tdma = 0, tgen[2] = {0, 0};
+    int64_t tdma = 0, tdmae = 0, tgen[2] = {0, 0};
     const int bm[2] = {Cfg128::BM, Cfg256::BM}, bn[2] = {Cfg128::BN, Cfg256::BN};
+    constexpr int64_t TM = dmak::DBM, TN = dmak::DBN;
+    // a direct-to-LDS problem (or sub-problem) onto its list
+    auto add_dma = [&](Prob p, bool interior) {
+        const int sa = p.shift & 1, sb = (p.shift >> 1) & 1;
+        p.tiles_m[0] = p.tiles_m[1] = (int32_t)((p.m + sa + TM - 1) / TM);
+        p.tiles_n[0] = p.tiles_n[1] = (int32_t)((p.n + sb + TN - 1) / TN);
+        int64_t& t = interior ? tdma : tdmae;
+        p.tile0[0] = p.tile0[1] = (int32_t)t;
+        t += (int64_t)p.tiles_m[0] * p.tiles_n[0];
+        (interior ? out.dma : out.dmae).push_back(p);
+    };
+    // rows [r0, r0 + nr) x columns [c0, c0 + nc) of p as a problem of its own,
+    // its segments re-based and appended; its shifts follow from the new bases
+    auto sub = [&](const Prob& p, int64_t r0, int64_t nr, int64_t c0, int64_t nc) {
+        Prob q = p;
+        q.m = nr;
+        q.n = nc;
+        q.c = p.c + r0 + c0 * p.ldc;
+        q.seg0 = (int32_t)out.segs.size();
+        for (int s = p.seg0; s < p.seg0 + p.nseg; ++s) {
+            Seg g = segs[s];
+            g.a += ak ? r0 * g.lda : r0;
+            g.b += bkc ? c0 * g.ldb : c0;
+            out.segs.push_back(g);
+        }
+        const Seg& g0 = out.segs[q.seg0];
+        const int sa = !ak && (reinterpret_cast<uintptr_t>(g0.a) & 15) == 8;
+        const int sb = !bkc && (reinterpret_cast<uintptr_t>(g0.b) & 15) == 8;
+        q.shift = sa | (sb << 1);
+        return q;
+    };
     for (const Prob& p0 : probs_in) {
         if (p0.m <= 0 || p0.n <= 0 || p0.nseg <= 0) continue;
         bool v = true, d = !no_dma;
@@ -995,16 +1027,31 @@ int prepare_gemm(char opA, char opB, const std::vector<Prob>& probs_in, const st
             steps += (segs[s].klen + dmak::DBK - 1) / dmak::DBK;
             if (segs[s].klen % dmak::DBK) whole = false;
         }
-        if (p.shift || p.m % dmak::DBM || p.n % dmak::DBN || !whole || (steps & 1)) out.dma_edge = true;
-        p.tiles_m[0] = p.tiles_m[1] = (int32_t)((p.m + sa + dmak::DBM - 1) / dmak::DBM);
-        p.tiles_n[0] = p.tiles_n[1] = (int32_t)((p.n + sb + dmak::DBN - 1) / dmak::DBN);
-        p.tile0[0] = p.tile0[1] = (int32_t)tdma;
-        tdma += (int64_t)p.tiles_m[0] * p.tiles_n[0];
-        out.dma.push_back(p);
+        const bool k_ok = whole && !(steps & 1);
+        if (!p.shift && p.m % TM == 0 && p.n % TN == 0 && k_ok) {
+            add_dma(p, true);
+            continue;
+        }
+        // A shifted row-contiguous operand is aligned from row (column) 127
+        // on: the strip before it is one tile row (column) of the shifted
+        // tiling, so cutting there adds no tiles.
+        const int64_t r0 = sa ? TM - 1 : 0, c0 = sb ? TN - 1 : 0;
+        const int64_t mi = p.m > r0 ? (p.m - r0) / TM * TM : 0, ni = p.n > c0 ? (p.n - c0) / TN * TN : 0;
+        if (!split_edges || !k_ok || mi * ni < 8 * TM * TN) {
+            add_dma(p, false);
+            continue;
+        }
+        add_dma(sub(p, r0, mi, c0, ni), true);                                       // interior
+        if (r0 > 0) add_dma(sub(p, 0, r0, 0, p.n), false);                           // rows before it
+        if (r0 + mi < p.m) add_dma(sub(p, r0 + mi, p.m - r0 - mi, 0, p.n), false);   // rows after it
+        if (c0 > 0) add_dma(sub(p, r0, mi, 0, c0), false);                           // columns before it
+        if (c0 + ni < p.n) add_dma(sub(p, r0, mi, c0 + ni, p.n - c0 - ni), false);  // columns after it
     }
-    if (tdma > INT32_MAX || tgen[0] > INT32_MAX || tgen[1] > INT32_MAX)
+    if (tdma > INT32_MAX || tdmae > INT32_MAX || tgen[0] > INT32_MAX || tgen[1] > INT32_MAX ||
+        out.segs.size() > (size_t)INT32_MAX)
         return fail(DG_ERR_INVALID, "gemm: output tiles exceed one launch");
     out.tiles_dma = (int)tdma;
+    out.tiles_dmae = (int)tdmae;
     out.tiles_gen[0] = (int)tgen[0];
     out.tiles_gen[1] = (int)tgen[1];
     return DG_OK;
@@ -1022,8 +1069,12 @@ int launch_gemm_f64_dev(char opA, char opB, int mode, int cfg, const GemmBatchDe
                         hipStream_t stream) {
     if (mode == DG_GEMM_MODE_PERSISTENT && !counter)
         return fail(DG_ERR_INVALID, "gemm: persistent launch without a work counter");
-    if (b.tiles_dma > 0) {
-        const int t0 = b.tiles_dma;
+    for (int e = 0; e < 2; ++e) {  // interior list, then edge list
+        const bool edge = e == 1;
+        const int t0 = edge ? b.tiles_dmae : b.tiles_dma;
+        if (t0 <= 0) continue;
+        const Prob* dp = edge ? b.dmae : b.dma;
+        const int np = edge ? b.ndmae : b.ndma;
         int grid = 0;
         if (mode == DG_GEMM_MODE_PERSISTENT) {
             grid = persistent_grid(DG_GEMM_TILE_128x128);
@@ -1031,10 +1082,10 @@ int launch_gemm_f64_dev(char opA, char opB, int mode, int cfg, const GemmBatchDe
             hipLaunchKernelGGL(zero_counter_kernel, dim3(1), dim3(1), 0, stream, counter);
         }
         const bool ak = opA == 'T', bkc = opB != 'T';
-        if (ak && bkc) launch_dma<true, true>(b.dma_edge, mode, b.dma, b.ndma, b.segs, t0, grid, counter, stream);
-        else if (ak) launch_dma<true, false>(b.dma_edge, mode, b.dma, b.ndma, b.segs, t0, grid, counter, stream);
-        else if (bkc) launch_dma<false, true>(b.dma_edge, mode, b.dma, b.ndma, b.segs, t0, grid, counter, stream);
-        else launch_dma<false, false>(b.dma_edge, mode, b.dma, b.ndma, b.segs, t0, grid, counter, stream);
+        if (ak && bkc) launch_dma<true, true>(edge, mode, dp, np, b.segs, t0, grid, counter, stream);
+        else if (ak) launch_dma<true, false>(edge, mode, dp, np, b.segs, t0, grid, counter, stream);
+        else if (bkc) launch_dma<false, true>(edge, mode, dp, np, b.segs, t0, grid, counter, stream);
+        else launch_dma<false, false>(edge, mode, dp, np, b.segs, t0, grid, counter, stream);
         DG_LAUNCH_CHECK();
     }
     const int t = b.tiles_gen[cfg == DG_GEMM_TILE_256x128 ? 1 : 0];
@@ -1068,7 +1119,7 @@ int launch_gemm_f64(char opA, char opB, const std::vector<Prob>& probs_in, const
     GemmBatch hb;
     int rc = prepare_gemm(opA, opB, probs_in, segs, hb);
     if (rc) return rc;
-    if (hb.dma.empty() && hb.gen.empty()) return DG_OK;
+    if (hb.dma.empty() && hb.dmae.empty() && hb.gen.empty()) return DG_OK;
     GemmBatchDev b;
     TableUseScope uses{stream};  // after the launch below returns
     auto table = [&](const auto& v, auto** dst) -> bool {
@@ -1078,15 +1129,16 @@ int launch_gemm_f64(char opA, char opB, const std::vector<Prob>& probs_in, const
         *dst = static_cast<const T*>(cached_table(v.data(), v.size() * sizeof(T), stream));
         return *dst != nullptr;
     };
-    if (!table(hb.dma, &b.dma) || !table(hb.gen, &b.gen) || !table(segs, &b.segs))
+    if (!table(hb.dma, &b.dma) || !table(hb.dmae, &b.dmae) || !table(hb.gen, &b.gen) || !table(hb.segs, &b.segs))
         return DG_ERR_ALLOC;
     b.ndma = (int)hb.dma.size();
     b.tiles_dma = hb.tiles_dma;
+    b.ndmae = (int)hb.dmae.size();
+    b.tiles_dmae = hb.tiles_dmae;
     b.ngen = (int)hb.gen.size();
     b.tiles_gen[0] = hb.tiles_gen[0];
     b.tiles_gen[1] = hb.tiles_gen[1];
     b.vec = hb.vec;
-    b.dma_edge = hb.dma_edge;
     if (mode == DG_GEMM_MODE_PERSISTENT) mode = DG_GEMM_MODE_XCD;  // needs a counter: plans only
     return launch_gemm_f64_dev(opA, opB, mode, default_tile_cfg(), b, nullptr, stream);
 }

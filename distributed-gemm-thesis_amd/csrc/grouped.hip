@@ -274,7 +274,7 @@ int build_groups(dg_grouped_plan* p, const dg_gemm_desc* descs, int count, const
         GemmBatch hb;
         int rc = prepare_gemm(kv.first.first, kv.first.second, probs0, segs, hb);
         if (rc) return rc;
-        if (hb.dma.empty() && hb.gen.empty()) continue;
+        if (hb.dma.empty() && hb.dmae.empty() && hb.gen.empty()) continue;
         dg_grouped_plan::Group g;
         g.opA = kv.first.first;
         g.opB = kv.first.second;
@@ -289,15 +289,17 @@ int build_groups(dg_grouped_plan* p, const dg_gemm_desc* descs, int count, const
             *dst = static_cast<const T*>(d);
             return hipMemcpy(d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice) == hipSuccess;
         };
-        if (!upload(hb.dma, &b.dma) || !upload(hb.gen, &b.gen) || !upload(segs, &b.segs))
+        if (!upload(hb.dma, &b.dma) || !upload(hb.dmae, &b.dmae) || !upload(hb.gen, &b.gen) ||
+            !upload(hb.segs, &b.segs))
             return fail(DG_ERR_ALLOC, "plan: table upload");
         b.ndma = (int)hb.dma.size();
         b.tiles_dma = hb.tiles_dma;
+        b.ndmae = (int)hb.dmae.size();
+        b.tiles_dmae = hb.tiles_dmae;
         b.ngen = (int)hb.gen.size();
         b.tiles_gen[0] = hb.tiles_gen[0];
         b.tiles_gen[1] = hb.tiles_gen[1];
         b.vec = hb.vec;
-        b.dma_edge = hb.dma_edge;
     }
     return DG_OK;
 }
@@ -572,7 +574,7 @@ int dg_grouped_dgemm_noalloc(const dg_gemm_desc* descs, int count, void* stream)
     // Lay out every op group's tables (problems routed to each kernel, then
     // segments) in one host image, 256-byte aligned pieces, and check the size
     // before anything is launched.
-    struct Piece { char opA, opB; GemmBatch hb; std::vector<Seg> segs; size_t off_dma, off_gen, off_seg; };
+    struct Piece { char opA, opB; GemmBatch hb; std::vector<Seg> segs; size_t off_dma, off_dmae, off_gen, off_seg; };
     std::vector<Piece> pieces;
     size_t total = 0;
     auto take = [&](size_t bytes) { const size_t o = total; total = (total + bytes + 255) / 256 * 256; return o; };
@@ -581,12 +583,14 @@ int dg_grouped_dgemm_noalloc(const dg_gemm_desc* descs, int count, void* stream)
         std::vector<Seg> segs;
         build_group(descs, kv.second, probs, segs);
         if (probs.empty()) continue;
-        Piece pc{kv.first.first, kv.first.second, GemmBatch{}, std::move(segs), 0, 0, 0};
-        rc = prepare_gemm(pc.opA, pc.opB, probs, pc.segs, pc.hb);
+        Piece pc{kv.first.first, kv.first.second, GemmBatch{}, std::move(segs), 0, 0, 0, 0};
+        // no sub-problems here: the tables stay within the documented capacity
+        rc = prepare_gemm(pc.opA, pc.opB, probs, pc.segs, pc.hb, false);
         if (rc) return rc;
         pc.off_dma = take(pc.hb.dma.size() * sizeof(Prob));
+        pc.off_dmae = take(pc.hb.dmae.size() * sizeof(Prob));
         pc.off_gen = take(pc.hb.gen.size() * sizeof(Prob));
-        pc.off_seg = take(pc.segs.size() * sizeof(Seg));
+        pc.off_seg = take(pc.hb.segs.size() * sizeof(Seg));
         pieces.push_back(std::move(pc));
     }
     if (total > kScratchBytes)
@@ -600,8 +604,10 @@ int dg_grouped_dgemm_noalloc(const dg_gemm_desc* descs, int count, void* stream)
     std::vector<unsigned char> image(total, 0);
     for (const Piece& pc : pieces) {
         if (!pc.hb.dma.empty()) memcpy(image.data() + pc.off_dma, pc.hb.dma.data(), pc.hb.dma.size() * sizeof(Prob));
+        if (!pc.hb.dmae.empty())
+            memcpy(image.data() + pc.off_dmae, pc.hb.dmae.data(), pc.hb.dmae.size() * sizeof(Prob));
         if (!pc.hb.gen.empty()) memcpy(image.data() + pc.off_gen, pc.hb.gen.data(), pc.hb.gen.size() * sizeof(Prob));
-        memcpy(image.data() + pc.off_seg, pc.segs.data(), pc.segs.size() * sizeof(Seg));
+        memcpy(image.data() + pc.off_seg, pc.hb.segs.data(), pc.hb.segs.size() * sizeof(Seg));
     }
     rc = store_bytes(base, image, st);
     if (rc) return rc;
@@ -611,12 +617,14 @@ int dg_grouped_dgemm_noalloc(const dg_gemm_desc* descs, int count, void* stream)
         d.dma = pc.hb.dma.empty() ? nullptr : reinterpret_cast<const Prob*>(b + pc.off_dma);
         d.ndma = (int)pc.hb.dma.size();
         d.tiles_dma = pc.hb.tiles_dma;
+        d.dmae = pc.hb.dmae.empty() ? nullptr : reinterpret_cast<const Prob*>(b + pc.off_dmae);
+        d.ndmae = (int)pc.hb.dmae.size();
+        d.tiles_dmae = pc.hb.tiles_dmae;
         d.gen = pc.hb.gen.empty() ? nullptr : reinterpret_cast<const Prob*>(b + pc.off_gen);
         d.ngen = (int)pc.hb.gen.size();
         d.tiles_gen[0] = pc.hb.tiles_gen[0];
         d.tiles_gen[1] = pc.hb.tiles_gen[1];
         d.vec = pc.hb.vec;
-        d.dma_edge = pc.hb.dma_edge;
         d.segs = reinterpret_cast<const Seg*>(b + pc.off_seg);
         rc = launch_gemm_f64_dev(pc.opA, pc.opB, DG_GEMM_MODE_ROUND_ROBIN, default_tile_cfg(), d, nullptr, st);
         if (rc) return rc;
@@ -764,6 +772,7 @@ int dg_grouped_plan_destroy(dg_grouped_plan* p) {
     if (p->capture_stream) (void)hipStreamDestroy(p->capture_stream);
     for (auto& g : p->groups) {
         if (g.b.dma) (void)hipFree(const_cast<dg::Prob*>(g.b.dma));
+        if (g.b.dmae) (void)hipFree(const_cast<dg::Prob*>(g.b.dmae));
         if (g.b.gen) (void)hipFree(const_cast<dg::Prob*>(g.b.gen));
         if (g.b.segs) (void)hipFree(const_cast<dg::Seg*>(g.b.segs));
     }
