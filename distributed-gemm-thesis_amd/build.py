@@ -49,9 +49,28 @@ def _run(cmd):
     return out
 
 
+# Kernels that feed LDS straight from global memory count their own
+# outstanding loads and place their waits by hand; a register spill would put
+# scratch traffic between those loads and the waits (and has gone with wrong
+# results), so any spill in them fails the build.
+NO_SPILL = ("_dma_", "crt_gemm_kernel", "oz_plane_kernel")
+
+
+def _check_spills(src, remarks):
+    name = None
+    for line in remarks.splitlines():
+        if "Function Name:" in line:
+            name = line.split("Function Name:")[1].split("[")[0].strip()
+        elif name and ("VGPRs Spill:" in line or "SGPRs Spill:" in line or "ScratchSize [bytes/lane]:" in line):
+            n = int(line.split(":")[-1].split("[")[0].strip())
+            if n and any(t in name for t in NO_SPILL):
+                raise RuntimeError("build failed: %s spills (%s) in %s" % (name, line.split("remark:")[-1].strip(), src))
+
+
 def _compile(src):
     obj = os.path.join(OBJ, os.path.basename(src) + ".o")
-    _run([HIPCC] + HIP_FLAGS + ["-c", src, "-o", obj])
+    out = _run([HIPCC] + HIP_FLAGS + ["-Rpass-analysis=kernel-resource-usage", "-c", src, "-o", obj])
+    _check_spills(src, out)
     return obj
 
 

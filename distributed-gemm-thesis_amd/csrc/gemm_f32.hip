@@ -27,6 +27,10 @@
 // then receives step k+3 (+1.4 % over a 4-stage ring with the barrier after
 // the MFMAs, at 16384^3).
 //
+// Interior problems with a k-contiguous operand (opA = T, or opA = opB = N)
+// run sgemm_dma_xr_kernel instead, which keeps that operand in registers and
+// only the other in LDS (+2.3 % at NN 16384^3; described at the kernel).
+//
 // Edges: rows past the end of a k-contiguous operand read its last row;
 // row chunks past the end of a row-contiguous operand read the last 16-byte
 // aligned chunk, which holds a valid element and so cannot cross a page;
@@ -305,6 +309,199 @@ __global__ __launch_bounds__(256, 2) void sgemm_dma_kernel(SgemmArgs P) {
     }
 }
 
+// Interior problems (m, n multiples of 128, K a multiple of 16 with an even
+// step count) with one k-contiguous operand X straight into registers and the
+// other, Y, through the LDS ring (8 KB stages): each wave owns 32 rows of X
+// (rows of C for X = A, columns for X = B) against all 128 rows of Y,
+// loads them from global memory into a 3-step register ring (one 16-byte
+// load per lane and 16-row block: the lane's four k of a step) and reads
+// only Y from LDS.  A stays the MFMA's first operand and k = 4 lg + s at
+// sub-step s, sub-steps in order within each step, so every element is
+// summed in the order of sgemm_dma_kernel: the two agree bit for bit.
+// X = A for opA = T (any opB), X = B for opA = N, opB = N; both
+// row-contiguous (opA = N, opB = T) stays on sgemm_dma_kernel.  With X in
+// registers only Y crosses LDS, and the next step's Y fragments are all read
+// before the barrier (a row-contiguous Y lost 5 % when half of them were
+// read after it).  Loads in issue order: Y(k+3) at the barrier of step k,
+// X(k+3) after its MFMAs, so the barrier's wait for Y(k+2) leaves the loads
+// of X(k+2) in flight.
+// XB: X is B.  YKC: Y is k-contiguous.
+template <bool XB, bool YKC>
+__global__ __launch_bounds__(256, 2) void sgemm_dma_xr_kernel(SgemmArgs P) {
+    using namespace sk;
+    typedef __attribute__((address_space(1))) const fv4 gfv4;
+    extern __shared__ __attribute__((aligned(1024))) char lds[];
+    const int total = P.tiles_m * P.tiles_n;
+    const int tile = xcd_remap(blockIdx.x, total);
+    const int per_group = GROUP_M * P.tiles_n;
+    const int group = tile / per_group;
+    const int first_m = group * GROUP_M;
+    const int gsize = min(P.tiles_m - first_m, GROUP_M);
+    const int in_group = tile - group * per_group;
+    const int64_t m0 = (int64_t)(first_m + in_group % gsize) * BM;
+    const int64_t n0 = (int64_t)(in_group / gsize) * BN;
+
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lr = lane & 15, lg = lane >> 4;
+    const int64_t x0 = (XB ? n0 : m0) + 32 * wave;  // the wave's first row of X
+    const int64_t y0 = XB ? m0 : n0;
+    const int64_t ldx = XB ? P.ldb : P.lda;
+
+    // this lane's X at the next step to load: row x0 + 16i + lr, k = 4lg .. 4lg + 3
+    const float* xp = (XB ? P.b : P.a) + (x0 + lr) * ldx + 4 * lg;
+    const int64_t x_blk = 16 * ldx;
+    // rx[slot][i] of the slot's step: X(row x0 + 16i + lr, k = 4lg + s) in component s
+    fv4 rx[3][2];
+    auto load_x = [&](fv4 (&r)[2]) {
+        r[0] = *(gfv4*)xp;
+        r[1] = *(gfv4*)(xp + x_blk);
+        xp += BK;
+    };
+    Cursor cy = make_cursor<YKC, false>(XB ? P.a : P.b, XB ? P.lda : P.ldb, y0, XB ? P.m : P.n, wave, lane);
+    int ist = 0;
+    auto issue_y = [&]() {
+        issue_operand<YKC>(lds + ist * PART, cy, wave, lane, 0);
+        ist = ist + 1 == STG ? 0 : ist + 1;
+    };
+    const int64_t nk = P.k / BK;  // even, >= 2
+
+    fv4 acc[2][8];  // acc[i][j]: X block i against Y block j = 4h + jj
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 8; ++j) acc[i][j] = fv4{0.f, 0.f, 0.f, 0.f};
+
+    static_assert(STG == 3, "the waits below assume a 3-stage ring");
+    issue_y();
+    load_x(rx[0]);
+    issue_y();
+    load_x(rx[1]);
+    if (nk > 2) {
+        issue_y();
+        load_x(rx[2]);
+        wait_vmcnt<6>();  // Y of steps 0 and 1 landed; X(1), Y(2), X(2) may be in flight
+    } else {
+        wait_vmcnt<0>();
+    }
+    __builtin_amdgcn_s_barrier();
+    int rst = 1;  // stage of step kt + 1
+    float fy0[2][4][4], fy1[2][4][4];  // f[h][jj][s]: Y block 4h + jj, k = 4lg + s
+    read_operand<YKC>(fy0[0], lds, 0, lr, lg);
+    read_operand<YKC>(fy0[1], lds, 64, lr, lg);
+    int64_t kt = 0;
+    // MFMAs f = 32h + 8s + 4i + jj in order, the barrier after f = 47
+#define XR_MFMAS(SLOT, CY, F0, F1)                                                                  \
+    _Pragma("unroll") for (int f_ = F0; f_ < F1; ++f_) {                                            \
+        const int h_ = f_ >> 5, s_ = (f_ >> 3) & 3, i_ = (f_ >> 2) & 1, jj_ = f_ & 3;               \
+        const float xv_ = rx[SLOT][i_][s_], yv_ = CY[h_][jj_][s_];                                  \
+        acc[i_][4 * h_ + jj_] = XB ? __builtin_amdgcn_mfma_f32_16x16x4f32(yv_, xv_, acc[i_][4 * h_ + jj_], 0, 0, 0) \
+                                   : __builtin_amdgcn_mfma_f32_16x16x4f32(xv_, yv_, acc[i_][4 * h_ + jj_], 0, 0, 0); \
+    }
+// FULL: kt + 3 < nk is known (the barrier, the issue of Y(kt+3) and the load of X(kt+3) all run)
+#define XR_STEP(FULL, SLOT, CY, NY)                                                                 \
+    {                                                                                               \
+        const char* nst = lds + rst * PART; /* harmless past the last step */                       \
+        /* row-contiguous Y: the previous step's fragment reads (all before its barrier) are done;  \
+           saying so spares the compiler's own waits, which cannot count across the LDS-direct      \
+           loads.  Not for a k-contiguous Y: there the wait makes the compiler sink MFMAs of        \
+           earlier steps past the barriers until it spills */                                       \
+        if constexpr (!YKC) __builtin_amdgcn_s_waitcnt(0xF | (0x3 << 14) | (0x7 << 4));             \
+        read_operand<YKC>(NY[0], nst, 0, lr, lg);                                                   \
+        read_operand<YKC>(NY[1], nst, 64, lr, lg);                                                  \
+        XR_MFMAS(SLOT, CY, 0, 48)                                                                   \
+        __builtin_amdgcn_sched_group_barrier(0x008, 2, 0);                                          \
+        _Pragma("unroll") for (int g_ = 0; g_ < 8; ++g_) {                                          \
+            __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);                                      \
+            __builtin_amdgcn_sched_group_barrier(0x008, 4, 0);                                      \
+        }                                                                                           \
+        rst = rst + 1 == STG ? 0 : rst + 1;                                                         \
+        if (FULL || kt + 2 < nk) {                                                                  \
+            /* the stage of step kt was read during step kt-1: once at most the 8 reads just        \
+               issued are pending, none of it is still being read */                                \
+            __builtin_amdgcn_s_waitcnt(0xF | (0x3 << 14) | (0x7 << 4) | (8 << 8));                  \
+            wait_vmcnt<2>(); /* Y(kt+2); X(kt+2), issued after it, may be in flight */              \
+            __builtin_amdgcn_s_barrier();                                                           \
+            if (FULL || kt + STG < nk) issue_y();                                                   \
+        }                                                                                           \
+        XR_MFMAS(SLOT, CY, 48, 64)                                                                  \
+        if (FULL || kt + STG < nk) load_x(rx[SLOT]);                                                \
+        ++kt;                                                                                       \
+    }
+    // X slot = step % 3, Y fragments alternate: six steps per turn.  The
+    // steady turns have no conditions (so the waits the compiler derives
+    // across the loop stay partial); the last 2 to 8 steps, nk even, follow.
+    while (kt + 9 <= nk) {
+        XR_STEP(true, 0, fy0, fy1);
+        XR_STEP(true, 1, fy1, fy0);
+        XR_STEP(true, 2, fy0, fy1);
+        XR_STEP(true, 0, fy1, fy0);
+        XR_STEP(true, 1, fy0, fy1);
+        XR_STEP(true, 2, fy1, fy0);
+    }
+    while (true) {
+        XR_STEP(false, 0, fy0, fy1);
+        XR_STEP(false, 1, fy1, fy0);
+        if (kt >= nk) break;
+        XR_STEP(false, 2, fy0, fy1);
+        XR_STEP(false, 0, fy1, fy0);
+        if (kt >= nk) break;
+        XR_STEP(false, 1, fy0, fy1);
+        XR_STEP(false, 2, fy1, fy0);
+        if (kt >= nk) break;
+    }
+#undef XR_STEP
+#undef XR_MFMAS
+
+    // Result v of block (i, j = 4h + jj) in a lane: the MFMA's row 4lg + v
+    // and column lr.  X = A: C(x0 + 16i + 4lg + v, n0 + yrow(h, jj, lr));
+    // X = B: C(m0 + yrow(h, jj, 4lg + v), x0 + 16i + lr), with yrow(h, jj, r)
+    // = 64h + 16jj + r for a k-contiguous Y and 64h + 4r + jj for a
+    // row-contiguous one.  Each lane stores runs of four consecutive rows.
+    const float alpha = P.alpha, beta = P.beta;
+    auto store4 = [&](int64_t m, int64_t n, fv4 acc4) {
+        float* c = P.c + m + n * P.ldc;
+        fv4 val = alpha * acc4;
+        if (P.cvec) {
+            __attribute__((address_space(1))) fv4* cp = (__attribute__((address_space(1))) fv4*)c;
+            if (beta != 0.f) {
+                const fv4 old = *cp;
+#pragma unroll
+                for (int v = 0; v < 4; ++v) val[v] = fmaf(beta, old[v], val[v]);
+            }
+            *cp = val;
+        } else {
+            __attribute__((address_space(1))) float* cp = (__attribute__((address_space(1))) float*)c;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                float x = val[v];
+                if (beta != 0.f) x = fmaf(beta, cp[v], x);
+                cp[v] = x;
+            }
+        }
+    };
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            if constexpr (!XB) {
+#pragma unroll
+                for (int jj = 0; jj < 4; ++jj)
+                    store4(x0 + 16 * i + 4 * lg, n0 + (YKC ? 64 * h + 16 * jj + lr : 64 * h + 4 * lr + jj),
+                           acc[i][4 * h + jj]);
+            } else if constexpr (YKC) {
+#pragma unroll
+                for (int jj = 0; jj < 4; ++jj) store4(m0 + 64 * h + 16 * jj + 4 * lg, x0 + 16 * i + lr, acc[i][4 * h + jj]);
+            } else {
+#pragma unroll
+                for (int v = 0; v < 4; ++v)
+                    store4(m0 + 64 * h + 16 * lg + 4 * v, x0 + 16 * i + lr,
+                           fv4{acc[i][4 * h][v], acc[i][4 * h + 1][v], acc[i][4 * h + 2][v], acc[i][4 * h + 3][v]});
+            }
+        }
+    }
+}
+
 // Any layout: 64x64 tiles, k-steps of 16 staged through LDS with guarded
 // loads (zeros past the edges), each thread a 4x4 block of C in FP32 FMAs.
 __global__ __launch_bounds__(256) void sgemm_generic_kernel(int ta, int tb, SgemmArgs P) {
@@ -369,6 +566,16 @@ __global__ void sgemm_scale_kernel(int64_t m, int64_t n, float beta, float* __re
 
 template <bool AK, bool BKC>
 void launch_sgemm_dma(bool edge, const SgemmArgs& a, int tiles, hipStream_t st) {
+    // DG_GEMM_NO_AREG keeps both operands in LDS (same results; for comparisons)
+    static const bool no_areg = getenv("DG_GEMM_NO_AREG") != nullptr;
+    constexpr size_t xr_lds = (size_t)sk::STG * sk::PART;
+    if constexpr (AK || BKC) {
+        if (!edge && !no_areg) {
+            // the k-contiguous operand into registers: A if it is, else B (opA = N, opB = N)
+            hipLaunchKernelGGL((sgemm_dma_xr_kernel<!AK, AK ? BKC : false>), dim3(tiles), dim3(256), xr_lds, st, a);
+            return;
+        }
+    }
     if (edge)
         hipLaunchKernelGGL((sgemm_dma_kernel<AK, BKC, true>), dim3(tiles), dim3(256), sk::LDS_BYTES, st, a);
     else

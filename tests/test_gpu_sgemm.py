@@ -152,3 +152,41 @@ def test_generate_f32_bit_exact(dev, dist):
     got = t.cpu().numpy()
     ref = oracle.generate_f32(777, dist, n)
     assert np.array_equal(got.view(np.uint32), ref.view(np.uint32))
+
+
+@pytest.mark.parametrize("opA,opB", [("N", "N"), ("N", "T"), ("T", "N"), ("T", "T")])
+@pytest.mark.parametrize("m,n,k", [(128, 128, 32), (256, 384, 160), (384, 128, 224), (512, 256, 288),
+                                   (256, 256, 1024)])
+def test_sgemm_areg_matches_lds_kernel(dev, opA, opB, m, n, k):
+    """Interior problems (whole tiles, an even number of whole k-steps) with a
+    k-contiguous operand run the kernel that keeps that operand in registers
+    (A for opA = T, B for NN; NT stays on the direct-to-LDS kernel); one more
+    row or column of C sends the same operands to the direct-to-LDS kernel's
+    edge variant, which sums every element in the same order.
+    The rows and columns both compute must agree bit for bit.  Step counts
+    2, 10, 14, 18 and 64 cover the short path and one, two or ten steady
+    turns with tails of 4, 8 and 6 steps."""
+    g = torch.Generator(device="cpu").manual_seed(m + n + k)
+    ra, ca, lda = (m + 1, k, m + 4) if opA == "N" else (k, m + 1, k)
+    rb, cb, ldb = (k, n + 1, k) if opB == "N" else (n + 1, k, n + 4)
+    A = (torch.rand(ca, lda, generator=g) * 2 - 1).to(dev)
+    B = (torch.rand(cb, ldb, generator=g) * 2 - 1).to(dev)
+    ldc = m + 4
+    C0 = (torch.rand(n + 1, ldc, generator=g) * 2 - 1).to(dev)
+    out = {}
+    for mm, nn in ((m, n), (m + 1, n), (m, n + 1)):
+        C = C0.clone()
+        dgamd.sgemm_t(opA, opB, mm, nn, k, 1.5, A, lda, B, ldb, 0.5, C, ldc)
+        torch.cuda.synchronize()
+        out[(mm, nn)] = C[:n, :m].cpu()
+    areg = out[(m, n)]
+    assert torch.equal(areg.view(torch.int32), out[(m + 1, n)].view(torch.int32))
+    assert torch.equal(areg.view(torch.int32), out[(m, n + 1)].view(torch.int32))
+    # and the interior result is the FP32 product (not only self-consistent)
+    a = A.cpu().double().T[:ra, :ca]
+    b = B.cpu().double().T[:rb, :cb]
+    aa = (a if opA == "N" else a.T)[:m, :]
+    bb = (b if opB == "N" else b.T)[:, :n]
+    ref = 1.5 * (aa @ bb) + 0.5 * C0.cpu().double().T[:m, :n]
+    err = ((areg.double().T - ref).norm() / ref.norm()).item()
+    assert err < 1e-5, err
