@@ -84,6 +84,18 @@ def test_sgemm_edges_exact_integer(dev, opA, opB, m, n, k):
     assert np.array_equal(got, ref)
 
 
+@pytest.mark.parametrize("opA,opB", [("N", "N"), ("N", "T"), ("T", "N"), ("T", "T")])
+@pytest.mark.parametrize("padc,offc", [(1, 0), (0, 1), (3, 2)])
+def test_sgemm_interior_element_stores_exact_integer(dev, opA, opB, padc, offc):
+    # interior problems whose C is not 16-byte aligned or whose ldc is not a
+    # multiple of 4: the element-store epilogue of the interior kernels (the
+    # register-operand kernel, or the direct-to-LDS one for NT); the C
+    # padding rows stay untouched
+    got, ref, _ = run(dev, opA, opB, 256, 128, 96, 1.0, 2.0, pad=(0, 0, padc), integer=True, seed=padc + offc,
+                      offset=(0, 0, offc))
+    assert np.array_equal(got, ref)
+
+
 @pytest.mark.parametrize("opA,opB,k,pad,offset", [
     ("N", "N", 77, (0, 0, 0), (0, 0, 0)),   # K % 4 != 0 with a k-contiguous B: generic kernel
     ("N", "T", 77, (0, 0, 0), (0, 0, 0)),   # both row-contiguous: direct-to-LDS kernel, any K
