@@ -582,6 +582,49 @@ void launch_sgemm_dma(bool edge, const SgemmArgs& a, int tiles, hipStream_t st) 
         hipLaunchKernelGGL((sgemm_dma_kernel<AK, BKC, false>), dim3(tiles), dim3(256), sk::LDS_BYTES, st, a);
 }
 
+// One direct-to-LDS problem.  With a k-contiguous operand, whole even
+// k-steps and at least 8 whole interior tiles, an uneven problem is cut into
+// its interior block (register-operand kernel) and the strips after it (rows
+// past the interior, then the columns past it beside the interior rows),
+// which run the edge kernel: the cut adds no tiles, and since both kernels
+// sum every element in the same order it changes no result bit.
+template <bool AK, bool BKC>
+void run_sgemm_dma(const SgemmArgs& a, hipStream_t st) {
+    auto one = [&](SgemmArgs p) {
+        p.tiles_m = (int)((p.m + sk::BM - 1) / sk::BM);
+        p.tiles_n = (int)((p.n + sk::BN - 1) / sk::BN);
+        const int64_t steps = (p.k + sk::BK - 1) / sk::BK;
+        const bool edge = p.m % sk::BM || p.n % sk::BN || p.k % sk::BK || (steps & 1);
+        launch_sgemm_dma<AK, BKC>(edge, p, p.tiles_m * p.tiles_n, st);
+    };
+    static const bool no_areg = getenv("DG_GEMM_NO_AREG") != nullptr;
+    const int64_t mi = a.m / sk::BM * sk::BM, ni = a.n / sk::BN * sk::BN;
+    const bool k_ok = a.k % sk::BK == 0 && !((a.k / sk::BK) & 1);
+    if (!(AK || BKC) || no_areg || !k_ok || (mi == a.m && ni == a.n) || mi * ni < 8 * sk::BM * sk::BN) {
+        one(a);
+        return;
+    }
+    SgemmArgs in = a;
+    in.m = mi;
+    in.n = ni;
+    one(in);
+    if (mi < a.m) {
+        SgemmArgs s = a;  // rows mi .. m of every column
+        s.a = AK ? a.a + mi * a.lda : a.a + mi;
+        s.c = a.c + mi;
+        s.m = a.m - mi;
+        one(s);
+    }
+    if (ni < a.n) {
+        SgemmArgs s = a;  // columns ni .. n of rows 0 .. mi
+        s.b = BKC ? a.b + ni * a.ldb : a.b + ni;
+        s.c = a.c + ni * a.ldc;
+        s.m = mi;
+        s.n = a.n - ni;
+        one(s);
+    }
+}
+
 bool aligned16(const void* p) { return ((uintptr_t)p & 15) == 0; }
 
 }  // namespace
@@ -622,13 +665,10 @@ extern "C" int dg_sgemm(char opA, char opB, int64_t m, int64_t n, int64_t k, flo
     a.cvec = aligned16(C) && ldc % 4 == 0;
     const bool dma = aligned16(A) && aligned16(B) && lda % 4 == 0 && ldb % 4 == 0 && (!(ak || bkc) || k % 4 == 0);
     if (dma) {
-        const int64_t steps = (k + sk::BK - 1) / sk::BK;
-        const bool edge = m % sk::BM || n % sk::BN || k % sk::BK || (steps & 1);
-        const int tiles = (int)(tm * tn);
-        if (ak && bkc) launch_sgemm_dma<true, true>(edge, a, tiles, st);
-        else if (ak) launch_sgemm_dma<true, false>(edge, a, tiles, st);
-        else if (bkc) launch_sgemm_dma<false, true>(edge, a, tiles, st);
-        else launch_sgemm_dma<false, false>(edge, a, tiles, st);
+        if (ak && bkc) run_sgemm_dma<true, true>(a, st);
+        else if (ak) run_sgemm_dma<true, false>(a, st);
+        else if (bkc) run_sgemm_dma<false, true>(a, st);
+        else run_sgemm_dma<false, false>(a, st);
     } else {
         hipLaunchKernelGGL(sgemm_generic_kernel, dim3((unsigned)((m + 63) / 64), (unsigned)((n + 63) / 64)), dim3(256),
                            0, st, (int)ak, (int)(ob == 'T'), a);
