@@ -156,14 +156,21 @@ int launch_gemm_f64(char opA, char opB, const std::vector<Prob>& probs,
 // `dma`, and strips on `dmae` around it, placed so that the tile count does
 // not grow (a row-shifted problem's interior starts at its row 127).  The
 // sub-problems' segments are appended to `segs` (which starts as a copy of
-// the input segments); every element still sums in the same order.
+// the input segments); every element still sums in the same order.  The
+// strips past the interior that are narrower than a tile go instead to
+// `nar[0]` (columns past it, 128x32 tiles) and `nar[1]` (rows past it, 64x128
+// tiles), which the register-staged kernel runs in the same k order
+// (DG_GEMM_NO_NARROW keeps them on `dmae`).
 struct GemmBatch {
     std::vector<Prob> dma, dmae, gen;  // tile ranges assigned per list
     int tiles_dma = 0;                 // 128x128 tiles of `dma`
     int tiles_dmae = 0;                // 128x128 tiles of `dmae`
     int tiles_gen[2] = {0, 0};         // tiles of `gen` per tile shape
     bool vec = true;                   // every `gen` segment allows 128-bit loads
-    std::vector<Seg> segs;             // the segment table all three lists index
+    std::vector<Prob> nar[2];          // narrow strips: 128x32 tiles, 64x128 tiles
+    int tiles_nar[2] = {0, 0};
+    bool vec_nar[2] = {true, true};
+    std::vector<Seg> segs;             // the segment table all lists index
 };
 int prepare_gemm(char opA, char opB, const std::vector<Prob>& probs_in, const std::vector<Seg>& segs,
                  GemmBatch& out, bool split_edges = true);
@@ -177,6 +184,9 @@ struct GemmBatchDev {
     const Prob* gen = nullptr;
     int ngen = 0, tiles_gen[2] = {0, 0};
     bool vec = true;
+    const Prob* nar[2] = {nullptr, nullptr};
+    int nnar[2] = {0, 0}, tiles_nar[2] = {0, 0};
+    bool vec_nar[2] = {true, true};
     const Seg* segs = nullptr;
 };
 // Device half.  The persistent mode needs a device int `counter`, which it

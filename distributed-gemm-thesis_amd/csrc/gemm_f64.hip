@@ -64,6 +64,12 @@ struct TileCfg {
 using Cfg128 = TileCfg<128, 128, 256, 2>;  // 73.7 KB LDS, two workgroups per CU
 using Cfg256 = TileCfg<256, 128, 512, 1>;  // 110.6 KB LDS, 8 waves, half the B traffic per flop
 using Cfg128w8 = TileCfg<128, 128, 512, 2, 64, 32>;  // 8 waves of 64 x 32: four waves per SIMD
+// Strips narrower than a 128x128 tile, cut off uneven direct-to-LDS problems:
+// columns past the interior (fewer than 128) and rows past it.
+using CfgN32 = TileCfg<128, 32, 256, 2, 32, 32>;  // 4 waves of 32 x 32 stacked in M
+using CfgM64 = TileCfg<64, 128, 256, 2, 32, 64>;  // 2 x 2 waves of 32 x 64 (2 ms per step faster than
+                                                  // 32 x 128 tiles on the 20000 batch's 41-row strips)
+constexpr int64_t kNarrowMax = 64;  // widest strip (rows or columns) put on narrow tiles
 
 // Block id -> tile id so that each XCD (blocks are dealt round-robin to the
 // 8 XCDs) receives one contiguous range of tiles, which the grouped raster
@@ -189,7 +195,7 @@ template <class C, bool TA, bool TB, bool VEC>
 __device__ __forceinline__ void gemm_tile(const Prob* __restrict__ probs, int nprob, const Seg* __restrict__ segs,
                                           int tile, double (&As)[2][C::BM * LDK], double (&Bs)[2][C::BN * LDK]) {
     constexpr int BM = C::BM, BN = C::BN;
-    constexpr int ci = BM == 128 ? 0 : 1;  // which tile range of the problem table
+    constexpr int ci = BM == 256 ? 1 : 0;  // which tile range of the problem table
 
     // Problem owning this tile: last problem with tile0 <= tile.
     int lo = 0, hi = nprob - 1;
@@ -268,17 +274,20 @@ __device__ __forceinline__ void gemm_tile(const Prob* __restrict__ probs, int np
 
         const double* as = As[buf];
         const double* bs = Bs[buf];
-        // Each lane's 4 consecutive k values are read as two 128-bit LDS
-        // reads, one per half of the k-step, to keep register pressure low.
+        // Each half h of the k-step is one 8-deep step of the direct-to-LDS
+        // kernels, summed in their order: lane group lg supplies k = 8h + 2lg
+        // + q to MFMA q (one 128-bit LDS read per half), so every element
+        // takes the same sequence of MFMAs in all three kernels and a
+        // problem cut between them gives the same bits.
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
             dv2 af[FM], bf[FN];
 #pragma unroll
             for (int i = 0; i < FM; ++i)
-                af[i] = *reinterpret_cast<const dv2*>(&as[(wm + 16 * i + lr) * LDK + 4 * lg + 2 * h]);
+                af[i] = *reinterpret_cast<const dv2*>(&as[(wm + 16 * i + lr) * LDK + 8 * h + 2 * lg]);
 #pragma unroll
             for (int j = 0; j < FN; ++j)
-                bf[j] = *reinterpret_cast<const dv2*>(&bs[(wn + 16 * j + lr) * LDK + 4 * lg + 2 * h]);
+                bf[j] = *reinterpret_cast<const dv2*>(&bs[(wn + 16 * j + lr) * LDK + 8 * h + 2 * lg]);
 #pragma unroll
             for (int q = 0; q < 2; ++q)
 #pragma unroll
@@ -993,6 +1002,27 @@ int prepare_gemm(char opA, char opB, const std::vector<Prob>& probs_in, const st
         q.shift = sa | (sb << 1);
         return q;
     };
+    // a strip narrower than a tile onto narrow list w (0: 128x32 tiles, 1: 64x128)
+    static const bool no_narrow = getenv("DG_GEMM_NO_NARROW") != nullptr;
+    const int nbm[2] = {CfgN32::BM, CfgM64::BM}, nbn[2] = {CfgN32::BN, CfgM64::BN};
+    int64_t tnar[2] = {0, 0};
+    auto add_nar = [&](Prob q, int w) {
+        for (int s = q.seg0; s < q.seg0 + q.nseg; ++s) {
+            const Seg& g = out.segs[s];
+            if ((reinterpret_cast<uintptr_t>(g.a) & 15) || (reinterpret_cast<uintptr_t>(g.b) & 15) || (g.lda & 1) ||
+                (g.ldb & 1))
+                out.vec_nar[w] = false;
+        }
+        q.tiles_m[0] = q.tiles_m[1] = (int32_t)((q.m + nbm[w] - 1) / nbm[w]);
+        q.tiles_n[0] = q.tiles_n[1] = (int32_t)((q.n + nbn[w] - 1) / nbn[w]);
+        q.tile0[0] = q.tile0[1] = (int32_t)tnar[w];
+        tnar[w] += (int64_t)q.tiles_m[0] * q.tiles_n[0];
+        out.nar[w].push_back(q);
+    };
+    // a strip of `width` rows (columns) fills 128-wide tiles to width / 128,
+    // a 64-row (32-column) tile to width / 64 (width / (32 ceil(width / 32))),
+    // and the narrow tiles pay up to this width
+    auto narrow = [&](int64_t width) { return !no_narrow && width <= kNarrowMax; };
     for (const Prob& p0 : probs_in) {
         if (p0.m <= 0 || p0.n <= 0 || p0.nseg <= 0) continue;
         bool v = true, d = !no_dma;
@@ -1043,13 +1073,21 @@ int prepare_gemm(char opA, char opB, const std::vector<Prob>& probs_in, const st
         }
         add_dma(sub(p, r0, mi, c0, ni), true);                                       // interior
         if (r0 > 0) add_dma(sub(p, 0, r0, 0, p.n), false);                           // rows before it
-        if (r0 + mi < p.m) add_dma(sub(p, r0 + mi, p.m - r0 - mi, 0, p.n), false);   // rows after it
+        if (r0 + mi < p.m) {                                                         // rows after it
+            const Prob q = sub(p, r0 + mi, p.m - r0 - mi, 0, p.n);
+            if (narrow(q.m)) add_nar(q, 1); else add_dma(q, false);
+        }
         if (c0 > 0) add_dma(sub(p, r0, mi, 0, c0), false);                           // columns before it
-        if (c0 + ni < p.n) add_dma(sub(p, r0, mi, c0 + ni, p.n - c0 - ni), false);  // columns after it
+        if (c0 + ni < p.n) {                                                         // columns after it
+            const Prob q = sub(p, r0, mi, c0 + ni, p.n - c0 - ni);
+            if (narrow(q.n)) add_nar(q, 0); else add_dma(q, false);
+        }
     }
     if (tdma > INT32_MAX || tdmae > INT32_MAX || tgen[0] > INT32_MAX || tgen[1] > INT32_MAX ||
-        out.segs.size() > (size_t)INT32_MAX)
+        tnar[0] > INT32_MAX || tnar[1] > INT32_MAX || out.segs.size() > (size_t)INT32_MAX)
         return fail(DG_ERR_INVALID, "gemm: output tiles exceed one launch");
+    out.tiles_nar[0] = (int)tnar[0];
+    out.tiles_nar[1] = (int)tnar[1];
     out.tiles_dma = (int)tdma;
     out.tiles_dmae = (int)tdmae;
     out.tiles_gen[0] = (int)tgen[0];
@@ -1088,6 +1126,19 @@ int launch_gemm_f64_dev(char opA, char opB, int mode, int cfg, const GemmBatchDe
         else launch_dma<false, false>(edge, mode, dp, np, b.segs, t0, grid, counter, stream);
         DG_LAUNCH_CHECK();
     }
+    for (int w = 0; w < 2; ++w) {  // narrow strips: 128x32 tiles, then 64x128 tiles
+        const int tw = b.tiles_nar[w];
+        if (tw <= 0) continue;
+        int grid = 0;
+        if (mode == DG_GEMM_MODE_PERSISTENT) {
+            grid = persistent_grid(DG_GEMM_TILE_128x128);  // two workgroups per CU, as CfgN32 / CfgM64
+            if (grid > tw) grid = tw;
+            hipLaunchKernelGGL(zero_counter_kernel, dim3(1), dim3(1), 0, stream, counter);
+        }
+        if (w == 0) launch_cfg<CfgN32>(opA, opB, b.vec_nar[0], mode, b.nar[0], b.nnar[0], b.segs, tw, grid, counter, stream);
+        else launch_cfg<CfgM64>(opA, opB, b.vec_nar[1], mode, b.nar[1], b.nnar[1], b.segs, tw, grid, counter, stream);
+        DG_LAUNCH_CHECK();
+    }
     const int t = b.tiles_gen[cfg == DG_GEMM_TILE_256x128 ? 1 : 0];
     if (t <= 0) return DG_OK;
     int grid = 0;
@@ -1119,7 +1170,7 @@ int launch_gemm_f64(char opA, char opB, const std::vector<Prob>& probs_in, const
     GemmBatch hb;
     int rc = prepare_gemm(opA, opB, probs_in, segs, hb);
     if (rc) return rc;
-    if (hb.dma.empty() && hb.dmae.empty() && hb.gen.empty()) return DG_OK;
+    if (hb.dma.empty() && hb.dmae.empty() && hb.gen.empty() && hb.nar[0].empty() && hb.nar[1].empty()) return DG_OK;
     GemmBatchDev b;
     TableUseScope uses{stream};  // after the launch below returns
     auto table = [&](const auto& v, auto** dst) -> bool {
@@ -1129,8 +1180,14 @@ int launch_gemm_f64(char opA, char opB, const std::vector<Prob>& probs_in, const
         *dst = static_cast<const T*>(cached_table(v.data(), v.size() * sizeof(T), stream));
         return *dst != nullptr;
     };
-    if (!table(hb.dma, &b.dma) || !table(hb.dmae, &b.dmae) || !table(hb.gen, &b.gen) || !table(hb.segs, &b.segs))
+    if (!table(hb.dma, &b.dma) || !table(hb.dmae, &b.dmae) || !table(hb.gen, &b.gen) || !table(hb.segs, &b.segs) ||
+        !table(hb.nar[0], &b.nar[0]) || !table(hb.nar[1], &b.nar[1]))
         return DG_ERR_ALLOC;
+    for (int w = 0; w < 2; ++w) {
+        b.nnar[w] = (int)hb.nar[w].size();
+        b.tiles_nar[w] = hb.tiles_nar[w];
+        b.vec_nar[w] = hb.vec_nar[w];
+    }
     b.ndma = (int)hb.dma.size();
     b.tiles_dma = hb.tiles_dma;
     b.ndmae = (int)hb.dmae.size();

@@ -274,7 +274,7 @@ int build_groups(dg_grouped_plan* p, const dg_gemm_desc* descs, int count, const
         GemmBatch hb;
         int rc = prepare_gemm(kv.first.first, kv.first.second, probs0, segs, hb);
         if (rc) return rc;
-        if (hb.dma.empty() && hb.dmae.empty() && hb.gen.empty()) continue;
+        if (hb.dma.empty() && hb.dmae.empty() && hb.gen.empty() && hb.nar[0].empty() && hb.nar[1].empty()) continue;
         dg_grouped_plan::Group g;
         g.opA = kv.first.first;
         g.opB = kv.first.second;
@@ -290,8 +290,13 @@ int build_groups(dg_grouped_plan* p, const dg_gemm_desc* descs, int count, const
             return hipMemcpy(d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice) == hipSuccess;
         };
         if (!upload(hb.dma, &b.dma) || !upload(hb.dmae, &b.dmae) || !upload(hb.gen, &b.gen) ||
-            !upload(hb.segs, &b.segs))
+            !upload(hb.nar[0], &b.nar[0]) || !upload(hb.nar[1], &b.nar[1]) || !upload(hb.segs, &b.segs))
             return fail(DG_ERR_ALLOC, "plan: table upload");
+        for (int w = 0; w < 2; ++w) {
+            b.nnar[w] = (int)hb.nar[w].size();
+            b.tiles_nar[w] = hb.tiles_nar[w];
+            b.vec_nar[w] = hb.vec_nar[w];
+        }
         b.ndma = (int)hb.dma.size();
         b.tiles_dma = hb.tiles_dma;
         b.ndmae = (int)hb.dmae.size();
@@ -584,7 +589,8 @@ int dg_grouped_dgemm_noalloc(const dg_gemm_desc* descs, int count, void* stream)
         build_group(descs, kv.second, probs, segs);
         if (probs.empty()) continue;
         Piece pc{kv.first.first, kv.first.second, GemmBatch{}, std::move(segs), 0, 0, 0, 0};
-        // no sub-problems here: the tables stay within the documented capacity
+        // no sub-problems here (so no narrow strips either): the tables stay
+        // within the documented capacity
         rc = prepare_gemm(pc.opA, pc.opB, probs, pc.segs, pc.hb, false);
         if (rc) return rc;
         pc.off_dma = take(pc.hb.dma.size() * sizeof(Prob));
@@ -774,6 +780,8 @@ int dg_grouped_plan_destroy(dg_grouped_plan* p) {
         if (g.b.dma) (void)hipFree(const_cast<dg::Prob*>(g.b.dma));
         if (g.b.dmae) (void)hipFree(const_cast<dg::Prob*>(g.b.dmae));
         if (g.b.gen) (void)hipFree(const_cast<dg::Prob*>(g.b.gen));
+        for (const dg::Prob* t : g.b.nar)
+            if (t) (void)hipFree(const_cast<dg::Prob*>(t));
         if (g.b.segs) (void)hipFree(const_cast<dg::Seg*>(g.b.segs));
     }
     if (p->counter) (void)hipFree(p->counter);

@@ -1,9 +1,12 @@
 """Uneven FP64 problems are cut into an aligned interior block (A-in-registers
-kernel) and strips around it (edge-handling kernel).  The cut must not change
-a single bit: dg_dgemm (which cuts) against dg_grouped_dgemm_noalloc (which
-runs every problem whole on the edge-handling kernel), for every op
-combination, bases 8 bytes off 16-byte alignment, ragged sizes and K-step
-counts the cut accepts; and both within 1e-13 of the host dgemm."""
+kernel) and strips around it: edge-handling direct-to-LDS tiles, or, for
+strips past the interior at most 64 rows or columns wide, the register-staged
+kernel's 64x128 and 128x32 tiles.  The cut must not change a single bit:
+dg_dgemm (which cuts) against dg_grouped_dgemm_noalloc (which runs every
+problem whole on the edge-handling kernel), for every op combination, bases
+8 bytes off 16-byte alignment, ragged sizes, both sides of the narrow-strip
+width limit and K-step counts the cut accepts; and both within 1e-13 of the
+host dgemm."""
 import ctypes
 import itertools
 
@@ -20,6 +23,8 @@ SHAPES = [
     (1031, 777, 272),    # 34 k-steps
     (2857, 600, 512),    # a rank tile of the 7x5 grid over 20000
     (384, 384, 256),     # exactly 9 interior tiles when aligned
+    (1088, 832, 256),    # 64 rows and 64 columns past the interior: the widest narrow strips
+    (1089, 833, 256),    # 65 of each: strips on the 128x128 edge tiles again
 ]
 
 
