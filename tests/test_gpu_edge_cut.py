@@ -72,3 +72,46 @@ def test_cut_is_bit_identical(dev, opA, opB, m, n, k, off):
     # the padding rows between columns are untouched
     whole = cut.cpu().numpy()[off:off + ldc * n].reshape(n, ldc).T
     assert np.array_equal(whole[m:, :], c0.cpu().numpy()[off:off + ldc * n].reshape(n, ldc).T[m:, :])
+
+
+@pytest.mark.parametrize("opA,opB", [("N", "N"), ("T", "T")])
+@pytest.mark.parametrize("mode", ["nonpersistent", "persistent"])
+def test_cut_plans_and_graph_replays_bit_identical(dev, opA, opB, mode):
+    """A batch whose problems are cut (narrow strips included, bases on and
+    off 16-byte alignment) through a plan in either scheduling mode, then
+    through two replays of the plan's graph with the outputs poisoned in
+    between: every run gives the bits of the uncut no-allocation path."""
+    shapes = [(1031, 777, 272, 0), (2857, 600, 512, 1), (1088, 832, 256, 0), (1000, 900, 512, 1)]
+    L = dgamd.lib()
+    stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    keep, descs, outs = [], [], []
+    for i, (m, n, k, off) in enumerate(shapes):
+        print("plan cut case", opA, opB, mode, m, n, k, off, flush=True)
+        ra, ca = (m, k) if opA == "N" else (k, m)
+        rb, cb = (k, n) if opB == "N" else (n, k)
+        bufa, pa, lda, _ = operand(dev, ra, ca, off, 31 + i)
+        bufb, pb, ldb, _ = operand(dev, rb, cb, off, 41 + i)
+        bufc, pc, ldc, _ = operand(dev, m, n, off, 51 + i)
+        keep += [bufa, bufb]
+        outs.append((bufc, bufc.clone()))
+        descs.append(dgamd.gemm_desc(opA, opB, m, n, k, 0.75, pa, lda, pb, ldb, 0.5, pc, ldc))
+    arr = (dgamd.GemmDesc * len(descs))(*descs)
+    assert L.dg_grouped_dgemm_noalloc(arr, len(descs), stream) == dgamd.DG_OK, dgamd.last_error()
+    torch.cuda.synchronize()
+    ref = [c.clone() for c, _ in outs]
+    for c, c0 in outs:
+        c.copy_(c0)
+    plan = dgamd.GroupedPlan(descs, dgamd.DG_PLAN_PERSISTENT if mode == "persistent" else dgamd.DG_PLAN_NONPERSISTENT)
+    plan.execute()
+    torch.cuda.synchronize()
+    for (c, _), r in zip(outs, ref):
+        assert torch.equal(c.view(torch.int64), r.view(torch.int64))
+    for rep in range(2):
+        for c, c0 in outs:
+            c.copy_(c0)   # beta != 0: each replay starts from the same C
+        torch.cuda.synchronize()
+        plan.graph_launch()
+        torch.cuda.synchronize()
+        for (c, _), r in zip(outs, ref):
+            assert torch.equal(c.view(torch.int64), r.view(torch.int64)), rep
+    plan.close()
