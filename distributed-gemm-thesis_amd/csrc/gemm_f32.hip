@@ -29,7 +29,9 @@
 //
 // Interior problems with a k-contiguous operand (opA = T, or opA = opB = N)
 // run sgemm_dma_xr_kernel instead, which keeps that operand in registers and
-// only the other in LDS (+2.3 % at NN 16384^3; described at the kernel).
+// only the other in LDS (+2.3 % at NN 16384^3; described at the kernel), and
+// uneven ones are cut into such an interior block and edge strips
+// (run_sgemm_dma), with the same bits as the whole problem on this kernel.
 //
 // Edges: rows past the end of a k-contiguous operand read its last row;
 // row chunks past the end of a row-contiguous operand read the last 16-byte
