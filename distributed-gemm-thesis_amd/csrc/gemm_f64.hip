@@ -278,7 +278,8 @@ __device__ __forceinline__ void gemm_tile(const Prob* __restrict__ probs, int np
         // kernels, summed in their order: lane group lg supplies k = 8h + 2lg
         // + q to MFMA q (one 128-bit LDS read per half), so every element
         // takes the same sequence of MFMAs in all three kernels and a
-        // problem cut between them gives the same bits.
+        // problem cut between them gives the same bits (but for the sign of
+        // an exactly zero sum when K leaves this kernel a zero-padded half).
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
             dv2 af[FM], bf[FN];
