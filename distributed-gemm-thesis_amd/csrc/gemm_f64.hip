@@ -587,7 +587,8 @@ __device__ __forceinline__ void dma_tile(const Prob* __restrict__ probs, int npr
         const int q_ = f_ >> 4, i_ = (f_ >> 2) & 3, j_ = f_ & 3;                                    \
         acc[i_][j_] = __builtin_amdgcn_mfma_f64_16x16x4f64(CB[j_][q_], CA[i_][q_], acc[i_][j_], 0, 0, 0); \
     }
-#define DMA_STEP(CA, CB, NA, NB)                                                                    \
+// FULL: kt + 3 < nk is known (the barrier and the issue of step kt+3 both run)
+#define DMA_STEP(FULL, CA, CB, NA, NB)                                                                    \
     {                                                                                               \
         const char* nst = lds + rst * SSZ; /* harmless past the last step */                        \
         read_operand<AK>(NA, nst, wm, lr, lg);                                                      \
@@ -599,13 +600,13 @@ __device__ __forceinline__ void dma_tile(const Prob* __restrict__ probs, int npr
         }                                                                                           \
         __builtin_amdgcn_sched_group_barrier(0x008, 14, 0);                                         \
         rst = rst + 1 == DSTG ? 0 : rst + 1;                                                        \
-        if (kt + 2 < nk) {                                                                          \
+        if (FULL || kt + 2 < nk) {                                                                          \
             /* the stage of step kt was read during step kt-1; LDS reads retire in order, so once  \
                at most the 4 reads just issued are pending, none of it is still being read */       \
             __builtin_amdgcn_s_waitcnt(0xF | (0x3 << 14) | (0x7 << 4) | (4 << 8));                  \
             wait_vmcnt<0>(); /* step kt+2, the only loads in flight */                              \
             __builtin_amdgcn_s_barrier();                                                           \
-            if (kt + DSTG < nk) issue_next();                                                        \
+            if (FULL || kt + DSTG < nk) issue_next();                                                        \
         }                                                                                           \
         read_operand<BKC>(NB, nst + APART, wn, lr, lg);                                             \
         DMA_MFMAS(CA, CB, 24, 32)                                                                   \
@@ -616,9 +617,28 @@ __device__ __forceinline__ void dma_tile(const Prob* __restrict__ probs, int npr
         }                                                                                           \
         ++kt;                                                                                       \
     }
+    // Steady turns of twelve steps without conditions, then the last steps
+    // two at a time.  The compiler drains every load at a loop head (it
+    // cannot count across the direct-to-LDS loads there), which with a
+    // two-step loop was a wait for step kt+2's loads every other step:
+    // twelve-step turns run 5.5 % faster at 16384^3.
+    while (kt + 15 <= nk) {
+        DMA_STEP(true, fa0, fb0, fa1, fb1);
+        DMA_STEP(true, fa1, fb1, fa0, fb0);
+        DMA_STEP(true, fa0, fb0, fa1, fb1);
+        DMA_STEP(true, fa1, fb1, fa0, fb0);
+        DMA_STEP(true, fa0, fb0, fa1, fb1);
+        DMA_STEP(true, fa1, fb1, fa0, fb0);
+        DMA_STEP(true, fa0, fb0, fa1, fb1);
+        DMA_STEP(true, fa1, fb1, fa0, fb0);
+        DMA_STEP(true, fa0, fb0, fa1, fb1);
+        DMA_STEP(true, fa1, fb1, fa0, fb0);
+        DMA_STEP(true, fa0, fb0, fa1, fb1);
+        DMA_STEP(true, fa1, fb1, fa0, fb0);
+    }
     while (kt < nk) {
-        DMA_STEP(fa0, fb0, fa1, fb1);
-        DMA_STEP(fa1, fb1, fa0, fb0);
+        DMA_STEP(false, fa0, fb0, fa1, fb1);
+        DMA_STEP(false, fa1, fb1, fa0, fb0);
     }
 #undef DMA_STEP
 #undef DMA_MFMAS
